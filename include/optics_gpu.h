@@ -1,0 +1,334 @@
+/*
+ * optics_gpu.h — C ABI of the MI355X (gfx950) propagation library.
+ *
+ * Each entry point replaces one public Rust function of the reference
+ * (optics-core / optics-inverse); the comment above it names the file:line it
+ * stands in for.  Conventions shared by every call:
+ *
+ *  - A complex buffer is interleaved (re, im): `double*` for complex128 and
+ *    `float*` for complex64, row-major with sample (ix, iy) at index
+ *    iy*nx + ix, exactly the layout of `Vec<Complex64>` in `Field`
+ *    (src/field.rs:99-105).
+ *  - Entry points without a `_dev` suffix take and return HOST memory, like
+ *    the Rust API (a `&BandField` in, a new owned `BandField` out).  `_dev`
+ *    entry points take device pointers and enqueue on the context's stream
+ *    without synchronising.
+ *  - Every call returns an `oc_status`.  Validation order and the z == 0
+ *    identity follow the reference engines (src/propagation/asm.rs:439-479).
+ *    Sampling reports are filled on every call that computes one, including
+ *    the OC_ALIASING_RISK failure (the Rust error boxes the same report).
+ *  - No entry point aborts or panics; device failures return OC_ERR_DEVICE
+ *    and the message is available from oc_last_error().
+ *
+ * Threading: a context owns one HIP stream, a scratch arena and plan caches.
+ * Calls on one context are serialised by the caller; separate contexts may be
+ * used from separate threads.
+ */
+#ifndef OPTICS_GPU_H
+#define OPTICS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes.  One code per variant of the reference error enums.         */
+/* ------------------------------------------------------------------------ */
+typedef enum oc_status {
+    OC_OK = 0,
+    /* PropagationError / FresnelError / RayleighSommerfeldError
+       (src/propagation/asm.rs:218-250, fresnel.rs:225-254,
+       rayleigh_sommerfeld.rs:197-223) */
+    OC_INVALID_DISTANCE = 1,
+    OC_INVALID_WAVELENGTH = 2,
+    OC_INVALID_MEDIUM = 3,
+    OC_EMPTY_GRID = 4,
+    OC_NON_FINITE_SAMPLE = 5,
+    OC_ALIASING_RISK = 6,
+    /* OpticsError (src/element.rs:60-170) */
+    OC_INVALID_APERTURE = 20,
+    OC_OUTSIDE_FIELD = 21,
+    OC_INVALID_AXIS = 22,
+    OC_INVALID_PHASE = 23,
+    OC_INVALID_ORIENTATION = 24,
+    OC_INVALID_GRID = 25,
+    OC_INVALID_GRID_PITCH = 26,
+    /* InverseError (crates/optics-inverse/src/error.rs:11-60) */
+    OC_INV_NOT_FINITE = 40,
+    OC_INV_OUT_OF_BOUNDS = 41,
+    OC_INV_WRONG_DIMENSION = 42,
+    OC_INV_CARRIER = 43,
+    OC_INV_RENDER = 44,
+    OC_INV_ADJOINT = 45,
+    /* Library-level failures (no reference counterpart). */
+    OC_ERR_INVALID_ARGUMENT = 100,
+    OC_ERR_UNSUPPORTED = 101,
+    OC_ERR_DEVICE = 102,
+    OC_ERR_OUT_OF_MEMORY = 103
+} oc_status;
+
+/* ------------------------------------------------------------------------ */
+/* Data and configuration PODs.                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Geometry of a BandField: grid dims, pitch (m) and wavelength (m)
+   (src/field.rs:99-105, 287-290). */
+typedef struct oc_band_geom {
+    uint64_t nx, ny;
+    double dx, dy;
+    double lambda;
+} oc_band_geom;
+
+/* AsmConfig (src/propagation/asm.rs:100-137). Default: {2, 1, 1, 8, 1}. */
+typedef struct oc_asm_config {
+    double padding;
+    int32_t band_limited;
+    int32_t strict;
+    double max_padding;
+    double medium;
+} oc_asm_config;
+
+/* SamplingReport (src/propagation/asm.rs:144-179). */
+typedef struct oc_asm_report {
+    double lambda, medium, z;
+    uint64_t nx, ny;
+    double dx, dy;
+    double requested_padding, used_padding;
+    uint64_t nx_pad, ny_pad;
+    double band_limit_fx, band_limit_fy;
+    double max_phase_step_x, max_phase_step_y;
+    int32_t nyquist_ok;
+    int32_t _pad0;
+} oc_asm_report;
+
+/* FresnelConfig (src/propagation/fresnel.rs:99-123). Default: {2, 1, 8}. */
+typedef struct oc_fresnel_config {
+    double padding;
+    int32_t strict;
+    int32_t _pad0;
+    double max_padding;
+} oc_fresnel_config;
+
+/* FresnelSamplingReport (src/propagation/fresnel.rs:130-217). */
+typedef struct oc_fresnel_report {
+    double lambda, z;
+    uint64_t nx, ny;
+    double dx, dy;
+    double requested_padding, used_padding;
+    uint64_t nx_pad, ny_pad;
+    double max_phase_step_x, max_phase_step_y;
+    int32_t nyquist_ok;
+    int32_t kernel_nyquist_ok;
+    double max_kernel_step_x, max_kernel_step_y;
+    double max_phase_error_vs_asm;
+    int32_t paraxial_ok;
+    int32_t _pad0;
+} oc_fresnel_report;
+
+/* RayleighSommerfeldConfig (src/propagation/rayleigh_sommerfeld.rs:95-116). */
+typedef struct oc_rs_config {
+    double padding;
+    int32_t strict;
+    int32_t _pad0;
+} oc_rs_config;
+
+/* RayleighSommerfeldSamplingReport (rayleigh_sommerfeld.rs:123-189). */
+typedef struct oc_rs_report {
+    double lambda, z;
+    uint64_t nx, ny;
+    double dx, dy;
+    double padding;
+    uint64_t nx_pad, ny_pad;
+    double max_kernel_step_x, max_kernel_step_y;
+    int32_t kernel_nyquist_ok;
+    int32_t paraxial_ok;
+    double max_phase_error_vs_fresnel;
+} oc_rs_report;
+
+/* ClearAperture (src/element.rs:384-406). */
+typedef enum oc_aperture_kind {
+    OC_APERTURE_FIELD = 0,
+    OC_APERTURE_CIRCLE = 1,    /* a = radius */
+    OC_APERTURE_ELLIPSE = 2,   /* a, b = semi-axes */
+    OC_APERTURE_RECTANGLE = 3  /* a, b = half-widths */
+} oc_aperture_kind;
+
+typedef struct oc_clear_aperture {
+    int32_t kind;
+    int32_t _pad0;
+    double a, b;
+} oc_clear_aperture;
+
+/* PhaseMask::sampled (src/element.rs:1321-1368): lattice values (radians,
+   row-major nx*ny), lattice pitch, orientation psi (radians; wrapped into
+   [0, 2pi) like common::Angle), clear aperture and centre (m).
+   `values` is a host pointer for host entry points and a device pointer for
+   `_dev` entry points. */
+typedef struct oc_phase_mask {
+    const double* values;
+    uint64_t nx, ny;
+    double dx, dy;
+    double orientation;
+    oc_clear_aperture aperture;
+    double cx, cy;
+} oc_phase_mask;
+
+/* ------------------------------------------------------------------------ */
+/* Context.                                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct oc_context oc_context;
+
+/* Creates a context on HIP device `device` with its own stream. */
+int oc_context_create(int device, oc_context** out);
+void oc_context_destroy(oc_context* ctx);
+/* Replaces the context stream (a hipStream_t passed as void*; NULL = the
+   context's own stream).  The caller keeps ownership of a supplied stream. */
+int oc_context_set_stream(oc_context* ctx, void* stream);
+void* oc_context_stream(oc_context* ctx);
+int oc_context_synchronize(oc_context* ctx);
+/* Message of the last failure on this context (never NULL). */
+const char* oc_last_error(oc_context* ctx);
+/* Static name of a status code. */
+const char* oc_status_name(int status);
+/* Library version string. */
+const char* oc_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Angular spectrum (src/propagation/asm.rs).                                */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces asm::sampling_report (src/propagation/asm.rs:418-420). Host only. */
+int oc_asm_sampling_report(const oc_band_geom* geom, double z,
+                           const oc_asm_config* cfg, oc_asm_report* report);
+
+/* Replaces asm::propagate (src/propagation/asm.rs:438-538): f64 drop-in, host
+   buffers in and out (2*nx*ny doubles each; `in` and `out` may alias). */
+int oc_asm_propagate(oc_context* ctx, const oc_band_geom* geom,
+                     const double* in, double z, const oc_asm_config* cfg,
+                     double* out, oc_asm_report* report);
+
+/* Device-resident asm::propagate: complex128 (`double*`) and the FP32-bulk
+   complex64 form (`float*`, FP64 phase arguments and twiddles).  `in` and
+   `out` are device pointers and may alias.  A non-NULL `mask` (device
+   `values`) is applied before propagating, fused into the first pass: it is
+   Element::apply for PhaseMask (src/element.rs:1443-1484) followed by
+   asm::propagate, as Beamline::trace does (src/beamline.rs:109-130). */
+int oc_asm_propagate_dev_c128(oc_context* ctx, const oc_band_geom* geom,
+                              const double* in, double z,
+                              const oc_asm_config* cfg,
+                              const oc_phase_mask* mask, double* out,
+                              oc_asm_report* report);
+int oc_asm_propagate_dev_c64(oc_context* ctx, const oc_band_geom* geom,
+                             const float* in, double z,
+                             const oc_asm_config* cfg,
+                             const oc_phase_mask* mask, float* out,
+                             oc_asm_report* report);
+
+/* ------------------------------------------------------------------------ */
+/* Fresnel transfer-function form (src/propagation/fresnel.rs).              */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces fresnel::sampling_report (src/propagation/fresnel.rs:382-384). */
+int oc_fresnel_sampling_report(const oc_band_geom* geom, double z,
+                               const oc_fresnel_config* cfg,
+                               oc_fresnel_report* report);
+/* Replaces fresnel::propagate (src/propagation/fresnel.rs:430-509). */
+int oc_fresnel_propagate(oc_context* ctx, const oc_band_geom* geom,
+                         const double* in, double z,
+                         const oc_fresnel_config* cfg, double* out,
+                         oc_fresnel_report* report);
+int oc_fresnel_propagate_dev_c128(oc_context* ctx, const oc_band_geom* geom,
+                                  const double* in, double z,
+                                  const oc_fresnel_config* cfg,
+                                  const oc_phase_mask* mask, double* out,
+                                  oc_fresnel_report* report);
+int oc_fresnel_propagate_dev_c64(oc_context* ctx, const oc_band_geom* geom,
+                                 const float* in, double z,
+                                 const oc_fresnel_config* cfg,
+                                 const oc_phase_mask* mask, float* out,
+                                 oc_fresnel_report* report);
+
+/* ------------------------------------------------------------------------ */
+/* Rayleigh-Sommerfeld first kind (src/propagation/rayleigh_sommerfeld.rs).  */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces rayleigh_sommerfeld::sampling_report (rayleigh_sommerfeld.rs:333). */
+int oc_rs_sampling_report(const oc_band_geom* geom, double z,
+                          const oc_rs_config* cfg, oc_rs_report* report);
+/* Replaces rayleigh_sommerfeld::propagate (rayleigh_sommerfeld.rs:384-442). */
+int oc_rs_propagate(oc_context* ctx, const oc_band_geom* geom,
+                    const double* in, double z, const oc_rs_config* cfg,
+                    double* out, oc_rs_report* report);
+int oc_rs_propagate_dev_c128(oc_context* ctx, const oc_band_geom* geom,
+                             const double* in, double z,
+                             const oc_rs_config* cfg, double* out,
+                             oc_rs_report* report);
+int oc_rs_propagate_dev_c64(oc_context* ctx, const oc_band_geom* geom,
+                            const float* in, double z,
+                            const oc_rs_config* cfg, float* out,
+                            oc_rs_report* report);
+
+/* ------------------------------------------------------------------------ */
+/* Batched propagation: many (grid, wavelength, z) problems per launch.      */
+/* ------------------------------------------------------------------------ */
+
+typedef enum oc_engine {
+    OC_ENGINE_ASM = 0,
+    OC_ENGINE_FRESNEL = 1,
+    OC_ENGINE_RS = 2
+} oc_engine;
+
+/* One problem of a batch.  `in`/`out` are device pointers (complex64 for the
+   c64 batch, complex128 for the c128 batch); `mask` may be NULL (ASM and
+   Fresnel only).  `padding` and `strict`/`band_limited`/`max_padding`/
+   `medium` follow the engine's config; fields an engine does not have are
+   ignored. `status` receives the per-problem status. */
+typedef struct oc_prop_desc {
+    oc_band_geom geom;
+    double z;
+    int32_t engine;
+    int32_t band_limited;
+    int32_t strict;
+    int32_t status;
+    double padding;
+    double max_padding;
+    double medium;
+    const oc_phase_mask* mask;
+    const void* in;
+    void* out;
+} oc_prop_desc;
+
+/* Runs every problem of `descs` with persistent launches (one per pass for
+   the whole batch).  Problems with a failing guard are skipped with their
+   status set; the call returns the first non-OK status, or OC_OK. */
+int oc_propagate_batch_dev_c64(oc_context* ctx, oc_prop_desc* descs,
+                               size_t count);
+int oc_propagate_batch_dev_c128(oc_context* ctx, oc_prop_desc* descs,
+                                size_t count);
+
+/* ------------------------------------------------------------------------ */
+/* Phase mask (src/element.rs).                                              */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces PhaseMask::sampled validation (src/element.rs:1321-1368). Host. */
+int oc_phasemask_validate(const oc_phase_mask* mask);
+/* Replaces Element::apply for PhaseMask (src/element.rs:1443-1484): host f64
+   buffers in and out (may alias).  On OC_INVALID_PHASE nothing is written. */
+int oc_phasemask_apply(oc_context* ctx, const oc_band_geom* geom,
+                       const double* in, const oc_phase_mask* mask,
+                       double* out);
+int oc_phasemask_apply_dev_c128(oc_context* ctx, const oc_band_geom* geom,
+                                const double* in, const oc_phase_mask* mask,
+                                double* out);
+int oc_phasemask_apply_dev_c64(oc_context* ctx, const oc_band_geom* geom,
+                               const float* in, const oc_phase_mask* mask,
+                               float* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPTICS_GPU_H */

@@ -1,0 +1,137 @@
+"""ctypes mirrors of the POD structs and status codes in include/optics_gpu.h.
+
+Kept free of any library loading so that both the product binding
+(optics_gpu.py) and the test checker can share one definition of the ABI.
+Each struct cites the Rust type of the reference it mirrors.
+"""
+import ctypes as C
+
+c_u64 = C.c_uint64
+c_i32 = C.c_int32
+c_f64 = C.c_double
+
+STATUS = {
+    0: "OC_OK",
+    1: "OC_INVALID_DISTANCE",
+    2: "OC_INVALID_WAVELENGTH",
+    3: "OC_INVALID_MEDIUM",
+    4: "OC_EMPTY_GRID",
+    5: "OC_NON_FINITE_SAMPLE",
+    6: "OC_ALIASING_RISK",
+    20: "OC_INVALID_APERTURE",
+    21: "OC_OUTSIDE_FIELD",
+    22: "OC_INVALID_AXIS",
+    23: "OC_INVALID_PHASE",
+    24: "OC_INVALID_ORIENTATION",
+    25: "OC_INVALID_GRID",
+    26: "OC_INVALID_GRID_PITCH",
+    40: "OC_INV_NOT_FINITE",
+    41: "OC_INV_OUT_OF_BOUNDS",
+    42: "OC_INV_WRONG_DIMENSION",
+    43: "OC_INV_CARRIER",
+    44: "OC_INV_RENDER",
+    45: "OC_INV_ADJOINT",
+    100: "OC_ERR_INVALID_ARGUMENT",
+    101: "OC_ERR_UNSUPPORTED",
+    102: "OC_ERR_DEVICE",
+    103: "OC_ERR_OUT_OF_MEMORY",
+}
+CODE = {v: k for k, v in STATUS.items()}
+
+APERTURE_FIELD, APERTURE_CIRCLE, APERTURE_ELLIPSE, APERTURE_RECTANGLE = 0, 1, 2, 3
+ENGINE_ASM, ENGINE_FRESNEL, ENGINE_RS = 0, 1, 2
+
+
+class BandGeom(C.Structure):
+    """BandField geometry (src/field.rs:99-105, 287-290)."""
+    _fields_ = [("nx", c_u64), ("ny", c_u64), ("dx", c_f64), ("dy", c_f64), ("lambda_", c_f64)]
+
+
+class AsmConfig(C.Structure):
+    """AsmConfig (src/propagation/asm.rs:100-137)."""
+    _fields_ = [("padding", c_f64), ("band_limited", c_i32), ("strict", c_i32),
+                ("max_padding", c_f64), ("medium", c_f64)]
+
+    @classmethod
+    def default(cls, **kw):
+        c = cls(2.0, 1, 1, 8.0, 1.0)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+class AsmReport(C.Structure):
+    """SamplingReport (src/propagation/asm.rs:144-179)."""
+    _fields_ = [("lambda_", c_f64), ("medium", c_f64), ("z", c_f64), ("nx", c_u64), ("ny", c_u64),
+                ("dx", c_f64), ("dy", c_f64), ("requested_padding", c_f64), ("used_padding", c_f64),
+                ("nx_pad", c_u64), ("ny_pad", c_u64), ("band_limit_fx", c_f64), ("band_limit_fy", c_f64),
+                ("max_phase_step_x", c_f64), ("max_phase_step_y", c_f64), ("nyquist_ok", c_i32),
+                ("_pad0", c_i32)]
+
+
+class FresnelConfig(C.Structure):
+    """FresnelConfig (src/propagation/fresnel.rs:99-123)."""
+    _fields_ = [("padding", c_f64), ("strict", c_i32), ("_pad0", c_i32), ("max_padding", c_f64)]
+
+    @classmethod
+    def default(cls, **kw):
+        c = cls(2.0, 1, 0, 8.0)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+class FresnelReport(C.Structure):
+    """FresnelSamplingReport (src/propagation/fresnel.rs:130-217)."""
+    _fields_ = [("lambda_", c_f64), ("z", c_f64), ("nx", c_u64), ("ny", c_u64), ("dx", c_f64), ("dy", c_f64),
+                ("requested_padding", c_f64), ("used_padding", c_f64), ("nx_pad", c_u64), ("ny_pad", c_u64),
+                ("max_phase_step_x", c_f64), ("max_phase_step_y", c_f64), ("nyquist_ok", c_i32),
+                ("kernel_nyquist_ok", c_i32), ("max_kernel_step_x", c_f64), ("max_kernel_step_y", c_f64),
+                ("max_phase_error_vs_asm", c_f64), ("paraxial_ok", c_i32), ("_pad0", c_i32)]
+
+
+class RsConfig(C.Structure):
+    """RayleighSommerfeldConfig (src/propagation/rayleigh_sommerfeld.rs:95-116)."""
+    _fields_ = [("padding", c_f64), ("strict", c_i32), ("_pad0", c_i32)]
+
+    @classmethod
+    def default(cls, **kw):
+        c = cls(2.0, 1, 0)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
+class RsReport(C.Structure):
+    """RayleighSommerfeldSamplingReport (rayleigh_sommerfeld.rs:123-189)."""
+    _fields_ = [("lambda_", c_f64), ("z", c_f64), ("nx", c_u64), ("ny", c_u64), ("dx", c_f64), ("dy", c_f64),
+                ("padding", c_f64), ("nx_pad", c_u64), ("ny_pad", c_u64), ("max_kernel_step_x", c_f64),
+                ("max_kernel_step_y", c_f64), ("kernel_nyquist_ok", c_i32), ("paraxial_ok", c_i32),
+                ("max_phase_error_vs_fresnel", c_f64)]
+
+
+class ClearAperture(C.Structure):
+    """ClearAperture (src/element.rs:384-406)."""
+    _fields_ = [("kind", c_i32), ("_pad0", c_i32), ("a", c_f64), ("b", c_f64)]
+
+
+class PhaseMask(C.Structure):
+    """PhaseMask::sampled arguments (src/element.rs:1321-1368)."""
+    _fields_ = [("values", C.c_void_p), ("nx", c_u64), ("ny", c_u64), ("dx", c_f64), ("dy", c_f64),
+                ("orientation", c_f64), ("aperture", ClearAperture), ("cx", c_f64), ("cy", c_f64)]
+
+
+class PropDesc(C.Structure):
+    """One problem of a batched propagation (no single reference type: a
+    (BandField, z, engine config) triple of asm/fresnel/rayleigh_sommerfeld)."""
+    _fields_ = [("geom", BandGeom), ("z", c_f64), ("engine", c_i32), ("band_limited", c_i32),
+                ("strict", c_i32), ("status", c_i32), ("padding", c_f64), ("max_padding", c_f64),
+                ("medium", c_f64), ("mask", C.POINTER(PhaseMask)), ("in_", C.c_void_p), ("out", C.c_void_p)]
+
+
+def geom(nx, ny, dx, dy, lam):
+    return BandGeom(int(nx), int(ny), float(dx), float(dy), float(lam))
+
+
+def report_dict(rep):
+    return {name: getattr(rep, name) for name, _ in rep._fields_ if not name.startswith("_pad")}

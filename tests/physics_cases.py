@@ -1,0 +1,440 @@
+"""Physics and convention cases restated from the reference's own tests.
+
+Each function takes an engine module `eng` exposing the oracle's call shapes
+(asm_propagate, fresnel_propagate, rs_propagate, phasemask_apply, ...) and a
+tolerance scale `tol` (1 for f64 engines, larger for the FP32-bulk form).
+They pin the f64 oracle (tests/test_oracle_physics.py) and are re-run on the
+GPU engines (tests/test_gpu_physics.py).  Sources cited per case.
+"""
+import math
+
+import numpy as np
+
+import oc_types as T
+
+LAMBDA = 500e-9
+
+
+def wrap_phase(d):
+    return (d + math.pi) % (2 * math.pi) - math.pi
+
+
+def uniform(a, nx, ny, phase=0.0):
+    return np.full((ny, nx), a * complex(math.cos(phase), math.sin(phase)), np.complex128)
+
+
+def gaussian(nx, ny, d, sigma, c=(0.0, 0.0)):
+    x = (np.arange(nx) - (nx - 1) / 2.0) * d - c[0]
+    y = (np.arange(ny) - (ny - 1) / 2.0) * d - c[1]
+    r2 = x[None, :] ** 2 + y[:, None] ** 2
+    return np.exp(-r2 / (2 * sigma * sigma)).astype(np.complex128)
+
+
+def rel_frob(a, b):
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2)))
+
+
+def ok(st):
+    assert st == 0, T.STATUS.get(st, st)
+
+
+# --------------------------------------------------------------------------
+# ASM: src/propagation/asm.rs:565-790 and tests/asm.rs
+# --------------------------------------------------------------------------
+
+def asm_z_zero_identity(eng, tol=1.0):
+    u = uniform(1.5, 16, 16)
+    st, out, _ = eng.asm_propagate(u, 20e-6, 20e-6, LAMBDA, 0.0, T.AsmConfig.default())
+    ok(st)
+    assert np.array_equal(out, u)
+
+
+def asm_plane_wave_picks_up_kz(eng, tol=1.0):
+    z = 1e-3
+    u = uniform(1.0, 16, 16)
+    st, out, _ = eng.asm_propagate(u, 20e-6, 20e-6, LAMBDA, z, T.AsmConfig.default(padding=1.0))
+    ok(st)
+    k = 2 * math.pi / LAMBDA
+    assert abs(out[9, 3] - complex(math.cos(k * z), math.sin(k * z))) < 1e-9 * tol
+
+
+def asm_back_propagation_inverts(eng, tol=1.0):
+    z = 1e-3
+    u = uniform(1.0, 16, 16)
+    cfg = T.AsmConfig.default(padding=1.0)
+    st, f, _ = eng.asm_propagate(u, 20e-6, 20e-6, LAMBDA, z, cfg)
+    ok(st)
+    st, rt, _ = eng.asm_propagate(f, 20e-6, 20e-6, LAMBDA, -z, cfg)
+    ok(st)
+    assert np.max(np.abs(rt - u)) < 1e-9 * tol
+
+
+def asm_invalid_inputs(eng, tol=1.0):
+    u = uniform(1.0, 8, 8)
+    st, _, _ = eng.asm_propagate(u, 20e-6, 20e-6, LAMBDA, float("nan"), T.AsmConfig.default())
+    assert st == T.CODE["OC_INVALID_DISTANCE"]
+    for m in [0.0, -1.5, float("nan"), float("inf")]:
+        st, _, _ = eng.asm_propagate(u, 20e-6, 20e-6, LAMBDA, 1e-3, T.AsmConfig.default(medium=m))
+        assert st == T.CODE["OC_INVALID_MEDIUM"]
+    st, _, _ = eng.asm_propagate(u, 20e-6, 20e-6, -1.0, 1e-3, T.AsmConfig.default())
+    assert st == T.CODE["OC_INVALID_WAVELENGTH"]
+    bad = u.copy()
+    bad[3, 4] = complex(float("nan"), 0.0)
+    st, _, _ = eng.asm_propagate(bad, 20e-6, 20e-6, LAMBDA, 1e-3, T.AsmConfig.default())
+    assert st == T.CODE["OC_NON_FINITE_SAMPLE"]
+    # validation order: z before lambda before medium before samples
+    st, _, _ = eng.asm_propagate(bad, 20e-6, 20e-6, -1.0, float("inf"), T.AsmConfig.default(medium=0.0))
+    assert st == T.CODE["OC_INVALID_DISTANCE"]
+    st, _, _ = eng.asm_propagate(bad, 20e-6, 20e-6, -1.0, 1e-3, T.AsmConfig.default(medium=0.0))
+    assert st == T.CODE["OC_INVALID_WAVELENGTH"]
+
+
+def asm_strict_flags_undersampled(eng, tol=1.0):
+    u = uniform(1.0, 32, 32)
+    z = 0.5
+    cfg = T.AsmConfig(1.0, 0, 1, 1.0, 1.0)
+    st, _, rep = eng.asm_propagate(u, 1e-6, 1e-6, LAMBDA, z, cfg)
+    assert st == T.CODE["OC_ALIASING_RISK"]
+    assert not rep.nyquist_ok and rep.max_phase_step_x > math.pi
+    st, out, rep = eng.asm_propagate(u, 1e-6, 1e-6, LAMBDA, z, T.AsmConfig(1.0, 0, 0, 1.0, 1.0))
+    ok(st)
+    assert np.all(np.isfinite(out))
+
+
+def asm_medium_plane_wave_nkz(eng, tol=1.0):
+    lam, t, n = 500e-9, 1e-3, 1.5
+    u = uniform(1.0, 16, 16)
+    st, out, _ = eng.asm_propagate(u, 20e-6, 20e-6, lam, t, T.AsmConfig.default(padding=1.0, medium=n))
+    ok(st)
+    k = 2 * math.pi / lam
+    assert abs(out[9, 3] - complex(math.cos(n * k * t), math.sin(n * k * t))) < 1e-9 * tol
+    st, vac, _ = eng.asm_propagate(u, 20e-6, 20e-6, lam, t, T.AsmConfig.default(padding=1.0))
+    ok(st)
+    ratio = out[9, 3] / vac[9, 3]
+    assert abs(ratio - complex(math.cos((n - 1) * k * t), math.sin((n - 1) * k * t))) < 1e-9 * tol
+
+
+def asm_medium_round_trip(eng, tol=1.0):
+    u = uniform(1.0, 16, 16)
+    cfg = T.AsmConfig.default(padding=1.0, medium=2.0)
+    st, f, _ = eng.asm_propagate(u, 20e-6, 20e-6, LAMBDA, 1e-3, cfg)
+    ok(st)
+    st, rt, _ = eng.asm_propagate(f, 20e-6, 20e-6, LAMBDA, -1e-3, cfg)
+    ok(st)
+    assert np.max(np.abs(rt - u)) < 1e-9 * tol
+
+
+def asm_strict_mitigates(eng, tol=1.0):
+    u = uniform(1.0, 32, 32)
+    z = 4e-4
+    cfg = T.AsmConfig(1.0, 0, 1, 8.0, 1.0)
+    st, out, rep = eng.asm_propagate(u, 1e-6, 1e-6, LAMBDA, z, cfg)
+    ok(st)
+    assert rep.used_padding > 1.0 and rep.nyquist_ok
+    assert np.all(np.isfinite(out))
+
+
+def asm_energy_conservation(eng, tol=1.0):
+    u = gaussian(256, 256, 10e-6, 0.15e-3)
+    st, out, _ = eng.asm_propagate(u, 10e-6, 10e-6, LAMBDA, 0.1, T.AsmConfig.default())
+    ok(st)
+    p_in, p_out = np.sum(np.abs(u) ** 2), np.sum(np.abs(out) ** 2)
+    assert abs((p_out - p_in) / p_in) <= 1e-9 * tol
+
+
+def asm_plane_wave_invariance(eng, tol=1.0):
+    z = 0.01
+    u = uniform(1.0, 128, 128)
+    st, out, _ = eng.asm_propagate(u, 10e-6, 10e-6, LAMBDA, z, T.AsmConfig(1.0, 1, 0, 1.0, 1.0))
+    ok(st)
+    assert np.max(np.abs(np.abs(out) - 1.0)) < 1e-6 * tol
+    err = np.abs((np.angle(out) - (2 * math.pi / LAMBDA) * z + math.pi) % (2 * math.pi) - math.pi)
+    assert err.max() < 1e-6 * tol
+
+
+def asm_tilted_plane_wave_ramp(eng, tol=1.0):
+    nx = ny = 128
+    dx = 10e-6
+    m = 3.0
+    alpha = m * LAMBDA / (nx * dx)
+    k = 2 * math.pi / LAMBDA
+    x = (np.arange(nx) - (nx - 1) / 2.0) * dx
+    u = np.tile(np.exp(1j * k * (math.sin(math.asin(alpha)) * x)), (ny, 1))
+    st, out, _ = eng.asm_propagate(u, dx, dx, LAMBDA, 0.01, T.AsmConfig(1.0, 1, 0, 1.0, 1.0))
+    ok(st)
+    expected = complex(math.cos(k * alpha * dx), math.sin(k * alpha * dx))
+    for iy in [0, 64, 127]:
+        assert abs(out[iy, 60] / out[iy, 59] - expected) < 1e-6 * tol
+    assert abs(out[0, 60] / out[0, 59] - out[127, 60] / out[127, 59]) < 1e-8 * tol
+
+
+def asm_round_trip_padded(eng, tol=1.0):
+    u = gaussian(256, 256, 10e-6, 0.15e-3)
+    cfg = T.AsmConfig.default()
+    st, f, _ = eng.asm_propagate(u, 10e-6, 10e-6, LAMBDA, 0.1, cfg)
+    ok(st)
+    st, b, _ = eng.asm_propagate(f, 10e-6, 10e-6, LAMBDA, -0.1, cfg)
+    ok(st)
+    assert rel_frob(b, u) < 1e-6 * tol
+
+
+def asm_guard_and_mitigation(eng, tol=1.0):
+    u = uniform(1.0, 64, 64)
+    z = 7.68e-4
+    st, _, rep = eng.asm_propagate(u, 1e-6, 1e-6, LAMBDA, z, T.AsmConfig(1.0, 0, 1, 1.0, 1.0))
+    assert st == T.CODE["OC_ALIASING_RISK"]
+    assert rep.used_padding == 1.0 and rep.requested_padding == 1.0 and rep.max_phase_step_x > math.pi
+    st, out, _ = eng.asm_propagate(u, 1e-6, 1e-6, LAMBDA, z, T.AsmConfig(1.0, 0, 0, 1.0, 1.0))
+    ok(st)
+    st, out, rep = eng.asm_propagate(u, 1e-6, 1e-6, LAMBDA, z, T.AsmConfig(1.0, 0, 1, 8.0, 1.0))
+    ok(st)
+    assert rep.used_padding == 8.0 and rep.nx_pad == 512
+
+
+def asm_adjoint_inner_product(eng, tol=1.0):
+    """adjoint.rs:406-427: <w, A(z) d> == <A(-z) w, d>, band limit off and on."""
+    n, d, z = 48, 10e-6, 1.5e-3
+
+    def chirp():
+        x = (np.arange(n) - (n - 1) / 2.0) * d
+        X, Y = np.meshgrid(x, x)
+        r2 = X * X + Y * Y
+        env = np.exp(-r2 / (2 * (60e-6) ** 2))
+        ph = 3.7 * X / 1e-3 + 2.1 * Y / 1e-3 + 9.0e6 * r2
+        return env * np.cos(ph) + 1j * env * np.sin(ph)
+
+    dvec = chirp()
+    w = chirp() * np.exp(0.3j)
+    for bl in (0, 1):
+        cfg = T.AsmConfig(2.0, bl, 0, 8.0, 1.0)
+        st, ad, _ = eng.asm_propagate(dvec, d, d, LAMBDA, z, cfg)
+        ok(st)
+        st, aw, _ = eng.asm_propagate(w, d, d, LAMBDA, -z, cfg)
+        ok(st)
+        lhs = np.vdot(w, ad)
+        rhs = np.vdot(aw, dvec)
+        assert abs(lhs - rhs) / abs(lhs) < 1e-12 * tol
+
+
+# --------------------------------------------------------------------------
+# Fresnel TF: src/propagation/fresnel.rs:602-921
+# --------------------------------------------------------------------------
+
+def fresnel_z_zero_identity(eng, tol=1.0):
+    u = uniform(1.5, 16, 16)
+    st, out, _ = eng.fresnel_propagate(u, 20e-6, 20e-6, LAMBDA, 0.0, T.FresnelConfig.default())
+    ok(st)
+    assert np.array_equal(out, u)
+
+
+def fresnel_plane_wave_bin_phase(eng, tol=1.0):
+    nx = ny = 32
+    d = 20e-6
+    z = 1e-3
+    m = 2.0
+    alpha = m * LAMBDA / (nx * d)
+    k = 2 * math.pi / LAMBDA
+    x = (np.arange(nx) - (nx - 1) / 2.0) * d
+    u = np.tile(np.exp(1j * k * (math.sin(math.asin(alpha)) * x)), (ny, 1))
+    st, out, _ = eng.fresnel_propagate(u, d, d, LAMBDA, z, T.FresnelConfig(1.0, 0, 0, 1.0))
+    ok(st)
+    fx = m / (nx * d)
+    ph = k * z - math.pi * LAMBDA * z * fx * fx
+    for iy in [0, 16, 31]:
+        assert abs(out[iy, 10] / u[iy, 10] - complex(math.cos(ph), math.sin(ph))) < 1e-9 * tol
+
+
+def fresnel_round_trip(eng, tol=1.0):
+    u = uniform(1.0, 16, 16)
+    cfg = T.FresnelConfig(1.0, 0, 0, 1.0)
+    st, f, _ = eng.fresnel_propagate(u, 20e-6, 20e-6, LAMBDA, 2e-3, cfg)
+    ok(st)
+    st, rt, _ = eng.fresnel_propagate(f, 20e-6, 20e-6, LAMBDA, -2e-3, cfg)
+    ok(st)
+    assert np.max(np.abs(rt - u)) < 1e-9 * tol
+
+
+def fresnel_strict_flags_and_mitigates(eng, tol=1.0):
+    u = uniform(1.0, 32, 32)
+    z = 3e-4
+    st, _, rep = eng.fresnel_propagate(u, 1e-6, 1e-6, LAMBDA, z, T.FresnelConfig(1.0, 1, 0, 1.0))
+    assert st == T.CODE["OC_ALIASING_RISK"] and rep.max_phase_step_x > math.pi
+    st, _, _ = eng.fresnel_propagate(u, 1e-6, 1e-6, LAMBDA, z, T.FresnelConfig(1.0, 0, 0, 1.0))
+    ok(st)
+    st, out, rep = eng.fresnel_propagate(u, 1e-6, 1e-6, LAMBDA, z, T.FresnelConfig(1.0, 1, 0, 8.0))
+    ok(st)
+    assert out.shape == u.shape and rep.used_padding == 8.0
+
+
+def fresnel_invalid_inputs(eng, tol=1.0):
+    u = uniform(1.0, 8, 8)
+    st, _, _ = eng.fresnel_propagate(u, 20e-6, 20e-6, LAMBDA, float("inf"), T.FresnelConfig.default())
+    assert st == T.CODE["OC_INVALID_DISTANCE"]
+
+
+def fresnel_gaussian_vs_asm_paraxial(eng, tol=1.0):
+    """Paraxial geometry: Fresnel TF == ASM to the truncation bound."""
+    u = gaussian(64, 64, 20e-6, 150e-6)
+    st, a, _ = eng.asm_propagate(u, 20e-6, 20e-6, LAMBDA, 5e-3, T.AsmConfig.default())
+    ok(st)
+    st, f, rep = eng.fresnel_propagate(u, 20e-6, 20e-6, LAMBDA, 5e-3, T.FresnelConfig.default())
+    ok(st)
+    assert rep.paraxial_ok
+    assert rel_frob(f, a) < 1e-3
+
+
+# --------------------------------------------------------------------------
+# Rayleigh-Sommerfeld: src/propagation/rayleigh_sommerfeld.rs:444-738 and
+# tests/phase4.rs with tests/common/mod.rs rs_direct
+# --------------------------------------------------------------------------
+
+def rs_delta_is_sampled_kernel(eng, tol=1.0):
+    n, d, z = 32, 1e-5, 1e-3
+    u = np.zeros((n, n), np.complex128)
+    u[0, 0] = 1.0
+    st, out, _ = eng.rs_propagate(u, d, d, LAMBDA, z, T.RsConfig(1.0, 0, 0))
+    ok(st)
+
+    def kv(x, y):
+        k = 2 * math.pi / LAMBDA
+        r = math.sqrt(x * x + y * y + z * z)
+        amp = abs(z) / (2 * math.pi * r * r)
+        return amp * complex(1 / r, -k) * complex(math.cos(k * r), math.sin(k * r)) * d * d
+
+    assert abs(out[0, 0] - kv(0.0, 0.0)) < 1e-12 * tol
+    for ix, iy in [(1, 0), (5, 3), (16, 16), (31, 31)]:
+        x = (ix if ix <= n // 2 else ix - n) * d
+        y = (iy if iy <= n // 2 else iy - n) * d
+        assert abs(out[iy, ix] - kv(x, y)) < 1e-12 * tol
+
+
+def rs_invalid_inputs(eng, tol=1.0):
+    u = uniform(1.0, 8, 8)
+    for z in (0.0, float("inf"), float("nan")):
+        st, _, _ = eng.rs_propagate(u, 20e-6, 20e-6, LAMBDA, z, T.RsConfig.default())
+        assert st == T.CODE["OC_INVALID_DISTANCE"]
+
+
+def rs_strict_guard(eng, tol=1.0):
+    u = uniform(1.0, 64, 64)
+    st, _, rep = eng.rs_propagate(u, 10e-6, 10e-6, LAMBDA, 1e-3, T.RsConfig(1.0, 1, 0))
+    assert st == T.CODE["OC_ALIASING_RISK"] and rep.max_kernel_step_x > math.pi
+    st, _, _ = eng.rs_propagate(u, 10e-6, 10e-6, LAMBDA, 1e-3, T.RsConfig(1.0, 0, 0))
+    ok(st)
+
+
+def rs_direct(u, d, lam, z):
+    """tests/common/mod.rs:49-88 brute-force linear RS-I convolution."""
+    ny, nx = u.shape
+    k = 2 * math.pi / lam
+    s = 1.0 if z >= 0 else -1.0
+    az = abs(z)
+    xx = (np.arange(2 * nx - 1) - (nx - 1)) * d
+    yy = (np.arange(2 * ny - 1) - (ny - 1)) * d
+    X, Y = np.meshgrid(xx, yy)
+    r = np.sqrt(X * X + Y * Y + az * az)
+    tab = (az / (2 * math.pi * r ** 3) - 1j * s * k * az / (2 * math.pi * r * r)) * np.exp(1j * s * k * r) * d * d
+    out = np.zeros_like(u)
+    for jy in range(ny):
+        for jx in range(nx):
+            sub = tab[jy:jy + ny, jx:jx + nx][::-1, ::-1]
+            out[jy, jx] = np.sum(u * sub)
+    return out
+
+
+def rs_matches_direct_quadrature(eng, tol=1.0):
+    u = gaussian(48, 48, 2e-6, 10e-6)
+    st, fast, rep = eng.rs_propagate(u, 2e-6, 2e-6, LAMBDA, 2e-3, T.RsConfig(2.0, 1, 0))
+    ok(st)
+    assert rep.kernel_nyquist_ok
+    slow = rs_direct(u, 2e-6, LAMBDA, 2e-3)
+    assert rel_frob(fast, slow) < 1e-2
+    st, a, _ = eng.asm_propagate(u, 2e-6, 2e-6, LAMBDA, 2e-3, T.AsmConfig.default())
+    ok(st)
+    assert rel_frob(fast, a) < 2e-2
+    st, rt, _ = eng.rs_propagate(fast, 2e-6, 2e-6, LAMBDA, -2e-3, T.RsConfig(2.0, 1, 0))
+    ok(st)
+    assert rel_frob(rt, u) < 3e-2
+
+
+def rs_agrees_with_asm_smooth_band(eng, tol=1.0):
+    n, d, z = 200, 2e-6, 5e-3
+    u = gaussian(n, n, d, 30e-6)
+    st, rs, rep = eng.rs_propagate(u, d, d, LAMBDA, z, T.RsConfig(2.0, 1, 0))
+    ok(st)
+    st, a, _ = eng.asm_propagate(u, d, d, LAMBDA, z, T.AsmConfig.default(padding=2.0))
+    ok(st)
+    assert rel_frob(rs, a) < 5e-2
+
+
+# --------------------------------------------------------------------------
+# Phase mask: src/element.rs:3300-3412, adjoint.rs:429-469
+# --------------------------------------------------------------------------
+
+def mask_identity_is_exact(eng, tol=1.0):
+    n = 24
+    u = gaussian(n, n, 10e-6, 60e-6) * np.exp(0.7j)
+    m = eng.Mask(np.zeros((n, n)), 10e-6, 10e-6)
+    st, out = eng.phasemask_apply(u, 10e-6, 10e-6, LAMBDA, m)
+    ok(st)
+    assert np.array_equal(out, u)
+
+
+def mask_constant_pi_stop_and_span(eng, tol=1.0):
+    n = 101
+    u = uniform(1.0, n, n, math.pi / 3.0)
+    m = eng.Mask(np.full((2, 2), math.pi), 20e-3, 20e-3, aperture=(T.APERTURE_CIRCLE, 30e-3, 0.0))
+    st, out = eng.phasemask_apply(u, 1e-3, 1e-3, LAMBDA, m)
+    ok(st)
+    p = u[50, 50]
+    assert abs(out[50, 50] + p) < 1e-12 * tol
+    assert abs(out[50, 65] - p) < 1e-12 * tol
+    assert out[75, 75] == 0
+
+
+def mask_rotation_conjugates_sample(eng, tol=1.0):
+    n, d = 41, 1e-3
+    c = n // 2
+    u = uniform(1.0, n, n)
+    grid = np.tile(0.3 * (np.arange(n) - c) * d, (n, 1))
+    plain = eng.Mask(grid, d, d)
+    rot = eng.Mask(grid, d, d, orientation=math.pi / 2.0)
+    st, po = eng.phasemask_apply(u, d, d, LAMBDA, plain)
+    ok(st)
+    st, ro = eng.phasemask_apply(u, d, d, LAMBDA, rot)
+    ok(st)
+    for iy in range(n):
+        for ix in range(n):
+            assert abs(ro[iy, ix] - po[2 * c - ix, iy]) < 1e-15 + 1e-7 * (tol - 1.0)
+
+
+def mask_transmission_probe(eng, tol=1.0):
+    n, d = 32, 10e-6
+    phase = np.sin(np.arange(n * n) * 0.37).reshape(n, n)
+    m = eng.Mask(phase, d, d, aperture=(T.APERTURE_CIRCLE, 120e-6, 0.0))
+    st, t = eng.phasemask_apply(np.ones((n, n), np.complex128), d, d, LAMBDA, m)
+    ok(st)
+    x = (np.arange(n) - (n - 1) / 2.0) * d
+    inside = x[None, :] ** 2 + x[:, None] ** 2 <= (120e-6) ** 2
+    want = np.where(inside, np.exp(1j * phase), 0)
+    assert np.max(np.abs(t - want)) < 1e-12 * tol
+
+
+def mask_outside_field_rejected(eng, tol=1.0):
+    n, d = 16, 10e-6
+    m = eng.Mask(np.zeros((n, n)), d, d, aperture=(T.APERTURE_CIRCLE, 100e-6, 0.0))
+    st, _ = eng.phasemask_apply(np.ones((n, n), np.complex128), d, d, LAMBDA, m)
+    assert st == T.CODE["OC_OUTSIDE_FIELD"]
+
+
+ASM_CASES = [asm_z_zero_identity, asm_plane_wave_picks_up_kz, asm_back_propagation_inverts, asm_invalid_inputs,
+             asm_strict_flags_undersampled, asm_medium_plane_wave_nkz, asm_medium_round_trip,
+             asm_strict_mitigates, asm_energy_conservation, asm_plane_wave_invariance,
+             asm_tilted_plane_wave_ramp, asm_round_trip_padded, asm_guard_and_mitigation,
+             asm_adjoint_inner_product]
+FRESNEL_CASES = [fresnel_z_zero_identity, fresnel_plane_wave_bin_phase, fresnel_round_trip,
+                 fresnel_strict_flags_and_mitigates, fresnel_invalid_inputs, fresnel_gaussian_vs_asm_paraxial]
+RS_CASES = [rs_delta_is_sampled_kernel, rs_invalid_inputs, rs_strict_guard, rs_matches_direct_quadrature,
+            rs_agrees_with_asm_smooth_band]
+MASK_CASES = [mask_identity_is_exact, mask_constant_pi_stop_and_span, mask_rotation_conjugates_sample,
+              mask_transmission_probe, mask_outside_field_rejected]
