@@ -1,0 +1,226 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the propagation hot path on one MI355X.
+
+Workload (the polychromatic BandField case): 16 bands, lambda 450-650 nm,
+2048^2 at 2 um pitch, Gaussian beam w0 = 1 mm, 8-level DOE phase mask in 4x4
+pixel blocks, then ASM z = 20 mm (padding 2 -> P = 4096, band-limited,
+strict) and Fresnel transfer-function z = 200 mm (padding 2, strict off).
+One step = one pass over the whole batch in the FP32-bulk form (complex64
+field, FP64 phase arguments and twiddles): 16 masked ASM propagations and 16
+Fresnel propagations, every problem of a stage in one persistent launch per
+pass.  value = propagations/s (32 per step), inputs resident in HBM.
+
+Also reported: per-pass kernel times from HIP events on the library stream
+(roofline of the dominant kernel), the complex128 form, the f64 host drop-in
+(PCIe included), the FP32 form's relative L2 error against the f64 oracle,
+and the oracle itself timed on the host (cpu_baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "optics-studio_amd"))
+
+METRIC = "propagations/s @2048² FP32-vs-f64 (rel L2≤1e-5), GS iter/s, adjoint grad/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--bands", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the f64 and host drop-in legs")
+    return ap.parse_args()
+
+
+def read_traffic():
+    """HBM bytes per launch of the column pass from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
+    p = os.path.join(REPO, "profiles", "traffic_col_c64_workload2.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch="):
+            return float(line.split("=", 1)[1])
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by band/problem, "
+              "see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import oc_types as T
+    import optics_gpu as G
+    import workloads as W
+
+    c = W.CFG2
+    n, d = c["n"], c["d"]
+    lams = W.workload2_lambdas()[: args.bands]
+    nb = len(lams)
+    ctx = G.context()
+    phi = W.workload2_doe()
+    mask, keep = G.dev_mask(phi, d, d)
+    field = W.workload2_field(lams[0])  # the waist field is wavelength-independent
+    u_in = torch.as_tensor(field.astype(np.complex64)).cuda().unsqueeze(0).repeat(nb, 1, 1).contiguous()
+    u_asm = torch.empty_like(u_in)
+    u_fr = torch.empty_like(u_in)
+
+    def descs(engine, src, dst, dbl=False):
+        arr = (T.PropDesc * nb)()
+        for b, lam in enumerate(lams):
+            x = arr[b]
+            x.geom = T.geom(n, n, d, d, lam)
+            x.engine = engine
+            if engine == T.ENGINE_ASM:
+                x.z, x.padding, x.band_limited, x.strict, x.max_padding, x.medium = c["z_asm"], 2.0, 1, 1, 8.0, 1.0
+                x.mask = T.C.pointer(mask)
+            else:
+                x.z, x.padding, x.strict, x.max_padding = c["z_fresnel"], 2.0, 0, 2.0
+            x.in_ = src[b].data_ptr()
+            x.out = dst[b].data_ptr()
+        return arr
+
+    d_asm = descs(T.ENGINE_ASM, u_in, u_asm)
+    d_fr = descs(T.ENGINE_FRESNEL, u_asm, u_fr)
+
+    def step(da, df, dbl=False):
+        st = G.propagate_batch_dev(da, double=dbl, ctx=ctx)
+        if st:
+            raise RuntimeError(f"ASM batch failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
+        st = G.propagate_batch_dev(df, double=dbl, ctx=ctx)
+        if st:
+            raise RuntimeError(f"Fresnel batch failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
+
+    log(f"workload: {nb} bands x 2048^2, c64; warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step(d_asm, d_fr)
+    torch.cuda.synchronize()
+    ctx.pass_timing(reset=True)
+    ctx.enable_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(d_asm, d_fr)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.enable_timing(False)
+    timing = ctx.pass_timing(reset=True)
+    elapsed = t1 - t0
+    props = 2 * nb * args.steps
+    value = props / elapsed
+    ms_step = 1e3 * elapsed / args.steps
+    log(f"c64: {ms_step:.3f} ms/step, {value:.1f} propagations/s; passes {timing}")
+
+    # Algorithmic bytes per launch (minimal-sweep model, complex64 = 8 B):
+    # row_fwd reads N^2 and writes N P; col reads and writes N P; row_inv
+    # reads N P and writes N^2 (per band; 16 bands per launch).
+    B = 8
+    P = 2 * n
+    alg = {"row_fwd": nb * (n * n + n * P) * B, "col": nb * (2 * n * P) * B, "row_inv": nb * (n * P + n * n) * B}
+    dom = max(alg, key=lambda k: timing[k][0])
+    avg_ms = timing[dom][0] / max(timing[dom][1], 1)
+    achieved = alg[dom] / (avg_ms * 1e-3) / 1e9
+    traffic = read_traffic() if dom == "col" else None
+    passes = {k: {"avg_ms": timing[k][0] / max(timing[k][1], 1), "launches": timing[k][1],
+                  "alg_GBps": alg[k] / (timing[k][0] / max(timing[k][1], 1) * 1e-3) / 1e9}
+              for k in alg if timing[k][1]}
+    step_bytes = 2 * nb * (2 * n * n + 4 * n * P) * B + nb * n * n * 8  # + f64 mask lattice read per ASM band
+    extra = {"passes": passes, "step_alg_GB": step_bytes / 1e9,
+             "step_alg_GBps": step_bytes / (elapsed / args.steps) / 1e9}
+
+    # parity of this run's last band output against the f64 oracle (and the
+    # oracle timed on the host as the CPU baseline)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle
+        lam = lams[-1]
+        u = W.workload2_field(lam)
+        om = oracle.Mask(phi, d, d)
+        t_c0 = time.perf_counter()
+        st, um = oracle.phasemask_apply(u, d, d, lam, om)
+        st, ra, _ = oracle.asm_propagate(um, d, d, lam, c["z_asm"], T.AsmConfig.default())
+        st2, rf, _ = oracle.fresnel_propagate(ra, d, d, lam, c["z_fresnel"], T.FresnelConfig(2.0, 0, 0, 2.0))
+        t_c1 = time.perf_counter()
+        if st or st2:
+            raise RuntimeError("oracle failed")
+        ga = u_asm[nb - 1].cpu().numpy().astype(np.complex128)
+        gf = u_fr[nb - 1].cpu().numpy().astype(np.complex128)
+        e_asm = float(np.linalg.norm(ga - ra) / np.linalg.norm(ra))
+        e_fr = float(np.linalg.norm(gf - rf) / np.linalg.norm(rf))
+        extra["rel_l2_c64_vs_f64_oracle"] = {"asm": e_asm, "fresnel": e_fr, "band_nm": lam * 1e9}
+        cpu = {"value": 2.0 / (t_c1 - t_c0), "unit": "propagations/s", "cores": 1, "kind": "port",
+               "sample": f"1 of {nb} bands ({lam * 1e9:.1f} nm): DOE mask + ASM 20 mm + Fresnel 200 mm at 2048^2 "
+                         f"(P=4096), f64 oracle, single thread, {t_c1 - t_c0:.1f} s"}
+        log(f"cpu oracle: {t_c1 - t_c0:.1f} s for 2 propagations; rel L2 asm {e_asm:.2e} fresnel {e_fr:.2e}")
+
+    if not args.no_extra:
+        # complex128 device form of the same batch
+        v_in = u_in.to(torch.complex128)
+        v_asm = torch.empty_like(v_in)
+        v_fr = torch.empty_like(v_in)
+        e_asm_d = descs(T.ENGINE_ASM, v_in, v_asm)
+        e_fr_d = descs(T.ENGINE_FRESNEL, v_asm, v_fr)
+        step(e_asm_d, e_fr_d, True)
+        torch.cuda.synchronize()
+        k = max(3, args.steps // 10)
+        s0 = time.perf_counter()
+        for _ in range(k):
+            step(e_asm_d, e_fr_d, True)
+        torch.cuda.synchronize()
+        s1 = time.perf_counter()
+        extra["c128_device_propagations_per_s"] = 2 * nb * k / (s1 - s0)
+        del v_in, v_asm, v_fr
+        # f64 host drop-in (oc_asm_propagate: host memory in and out, PCIe included)
+        um = (W.workload2_field(lams[0]) * np.exp(1j * phi)).astype(np.complex128)
+        G.asm_propagate(um, d, d, lams[0], c["z_asm"], T.AsmConfig.default(), ctx=ctx)
+        h0 = time.perf_counter()
+        for _ in range(3):
+            G.asm_propagate(um, d, d, lams[0], c["z_asm"], T.AsmConfig.default(), ctx=ctx)
+        h1 = time.perf_counter()
+        extra["f64_host_dropin_propagations_per_s"] = 3 / (h1 - h0)
+        log(f"c128 device {extra['c128_device_propagations_per_s']:.1f}/s, f64 host drop-in "
+            f"{extra['f64_host_dropin_propagations_per_s']:.2f}/s")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "propagations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": "synthetic",
+        "config": {"workload": f"polychromatic BandField {nb} bands 450-650 nm, 2048^2 at 2 um, 8-level DOE mask "
+                               "+ ASM 20 mm (P=4096) + Fresnel TF 200 mm, FP32 bulk / FP64 phase",
+                   "bands": nb, "grid": n, "padded": P, "propagations_per_step": 2 * nb},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": alg[dom], "avg_launch_ms": avg_ms},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

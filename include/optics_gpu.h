@@ -196,6 +196,16 @@ const char* oc_status_name(int status);
 /* Library version string. */
 const char* oc_version(void);
 
+/* Per-pass kernel timing (no reference counterpart; measurement support).
+   When enabled, every pass launch is bracketed by HIP events recorded on the
+   context stream; oc_context_pass_timing returns the accumulated milliseconds
+   and launch counts per pass: [0] row forward, [1] column, [2] row inverse,
+   [3] RS kernel generation, [4] GS fused passes, [5] other kernels. */
+#define OC_NUM_PASS_KINDS 6
+int oc_context_enable_timing(oc_context* ctx, int enable);
+int oc_context_pass_timing(oc_context* ctx, double* ms, uint64_t* launches,
+                           int reset);
+
 /* ------------------------------------------------------------------------ */
 /* Angular spectrum (src/propagation/asm.rs).                                */
 /* ------------------------------------------------------------------------ */
@@ -326,6 +336,129 @@ int oc_phasemask_apply_dev_c128(oc_context* ctx, const oc_band_geom* geom,
 int oc_phasemask_apply_dev_c64(oc_context* ctx, const oc_band_geom* geom,
                                const float* in, const oc_phase_mask* mask,
                                float* out);
+
+/* ------------------------------------------------------------------------ */
+/* Gerchberg-Saxton (crates/optics-inverse/src/gs.rs).                       */
+/* ------------------------------------------------------------------------ */
+
+/* GsProblem + GsConfig (gs.rs:125-305) for a GridPhase carrier with the
+   MseToTarget merit (merit.rs:112-141) as the merit seam.  Host pointers.
+   init_kind: 0 = GsInit::Zero (carrier_values), 2 = GsInit::Given
+   (init_phase); GsInit::Random (rand 0.8 StdRng) is not offered.
+   variant: 0 = Classic, 1 = Dvr{beta}, 2 = Fpr{beta}. */
+typedef struct oc_gs_problem {
+    oc_band_geom geom;
+    const double* input;            /* complex128 input field, host */
+    double z_total;
+    const double* target_amplitude; /* nx*ny */
+    const double* merit_target;     /* nx*ny, MseToTarget target intensity */
+    oc_asm_config asm_cfg;
+    oc_clear_aperture aperture;     /* carrier stop */
+    double cx, cy;                  /* carrier centre */
+    double lo, hi;                  /* carrier slot bounds */
+    const double* carrier_values;   /* nx*ny (GsInit::Zero start) */
+    uint64_t max_iterations, patience;
+    double merit_tolerance;
+    int32_t variant;
+    int32_t init_kind;
+    double beta;
+    const double* init_phase;       /* nx*ny when init_kind == 2 */
+} oc_gs_problem;
+
+/* GsRecord (gs.rs:254-268). */
+typedef struct oc_gs_record {
+    uint64_t iteration;
+    double merit, best_merit, amplitude_mse, target_error_rms,
+        phase_update_rms;
+} oc_gs_record;
+
+/* GsSolution (gs.rs:271-305); every array is caller-allocated host memory
+   of nx*ny doubles, history holds max_iterations records. */
+typedef struct oc_gs_solution {
+    double* phase;
+    double merit;
+    uint64_t best_iteration;
+    double amplitude_mse;
+    int32_t converged;
+    int32_t _pad0;
+    uint64_t iterations;
+    oc_gs_record* history;
+    double* target_intensity;
+    double* target_amplitude_error;
+    double* phase_error;
+    double* illumination_amplitude;
+} oc_gs_solution;
+
+/* Replaces gs::run (gs.rs:347-597): two ASM half-transforms per iteration
+   with the amplitude/phase projections, merit statistics and best-so-far
+   tracking fused into the propagation passes. */
+int oc_gs_run(oc_context* ctx, const oc_gs_problem* problem,
+              oc_gs_solution* solution);
+
+/* Gerchberg-Saxton in the Fourier (FFT) form, `restarts` independent runs
+   batched in one persistent launch per pass (no reference counterpart: the
+   reference loop uses ASM half-transforms).  Projection semantics of
+   gs.rs:484-564 (Classic), unpadded fft2 forward and ifft2 back, phase
+   bounds [0, 2 pi).  Device pointers: source_amp and target_amp are n*n
+   real (target_amp in DFT bin order), phases is restarts*n*n, updated in
+   place.  `graph_chunk` > 0 captures that many iterations in a hipGraph and
+   replays it.  err_out (host, restarts) receives
+   ||I/sum I - T^2/sum T^2|| / ||T^2/sum T^2|| of the final phases. */
+int oc_gs_fft_run_dev_c64(oc_context* ctx, uint64_t n, uint64_t restarts,
+                          const float* source_amp, const float* target_amp,
+                          float* phases, uint64_t iterations,
+                          uint64_t graph_chunk, double* err_out);
+int oc_gs_fft_run_dev_c128(oc_context* ctx, uint64_t n, uint64_t restarts,
+                           const double* source_amp, const double* target_amp,
+                           double* phases, uint64_t iterations,
+                           uint64_t graph_chunk, double* err_out);
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint gradients (crates/optics-inverse/src/adjoint.rs).                 */
+/* ------------------------------------------------------------------------ */
+
+/* One Design step (carrier.rs:35-43) whose element, if any, is a PhaseMask:
+   apply the mask, then propagate z_after through `medium` (beamline.rs:
+   109-130).  Element index = order of appearance. */
+typedef struct oc_design_step {
+    int32_t has_element;
+    int32_t _pad0;
+    oc_phase_mask mask;
+    double z_after;
+    double medium;
+} oc_design_step;
+
+/* Replaces adjoint::asm_adjoint (adjoint.rs:90-92): propagate by -z. Host. */
+int oc_adjoint_asm(oc_context* ctx, const oc_band_geom* geom, const double* in,
+                   double z, const oc_asm_config* cfg, double* out,
+                   oc_asm_report* report);
+
+/* Replaces adjoint::transmission (adjoint.rs:102-108): t = apply(1). Host. */
+int oc_adjoint_transmission(oc_context* ctx, const oc_band_geom* geom,
+                            const oc_phase_mask* mask, double* t_out);
+
+/* Replaces adjoint::push_back (adjoint.rs:140-246): forward tape, seed
+   w = dmdi * U_det, reverse sweep.  The tape keeps only each element's
+   incoming field (per-plane checkpoint); t and V are recomputed from the
+   mask lattice inside the fused gradient epilogue.  Host buffers; dphi_out
+   holds one nx*ny gradient per element. */
+int oc_adjoint_push_back(oc_context* ctx, const oc_design_step* steps,
+                         size_t nsteps, const oc_band_geom* geom,
+                         const double* input, const oc_asm_config* cfg,
+                         const double* dmdi, double* dphi_out);
+
+/* Replaces adjoint::grid_gradient (adjoint.rs:268-340) for one band and one
+   scene with MseToTarget (merit.rs:112-166): render, merit, seed and
+   push_back in one forward and one reverse sweep.  Device pointers (mask
+   values included).  dphi_out (device, n_elements*nx*ny) may be NULL: then
+   only the forward render and merit run.  merit_out (host) may be NULL. */
+int oc_adjoint_grid_gradient_dev_c128(oc_context* ctx,
+                                      const oc_design_step* steps,
+                                      size_t nsteps, const oc_band_geom* geom,
+                                      const double* input,
+                                      const oc_asm_config* cfg,
+                                      const double* target, double* dphi_out,
+                                      double* merit_out);
 
 #ifdef __cplusplus
 }

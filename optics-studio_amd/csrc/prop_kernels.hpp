@@ -1,0 +1,113 @@
+// Shared host/device descriptors of the propagation passes.
+#pragma once
+
+#include <cstdint>
+
+#include "fft_device.hpp"
+
+namespace ocd {
+
+enum EngineKind : int {
+    ENG_ASM = 0,
+    ENG_FRESNEL = 1,
+    ENG_RS = 2,
+    ENG_NONE = 3,        // plain 2-D FFT round trip (identity)
+    ENG_GS_PROJECT = 4,  // Fourier-plane amplitude projection (FFT-form GS)
+    ENG_MEASURE = 5      // forward only: accumulate intensity statistics
+};
+// Element-wise op fused into the first pass's loads.
+enum PreOp : int { PRE_NONE = 0, PRE_MASK = 1, PRE_POLAR = 2, PRE_SEED_MSE = 3 };
+// Element-wise op fused into the last pass's (cropped) stores.
+enum PostOp : int { POST_NONE = 0, POST_GS_TARGET = 1, POST_GS_PHASE = 2, POST_ADJ_GRAD = 3 };
+enum MaskMode : int { MASK_NONE = 0, MASK_SEPARABLE = 1, MASK_GENERAL = 2 };
+
+// PhaseMask::apply ingredients (src/element.rs:1443-1484).  The separable form
+// is used when psi == 0: element-frame coordinates are then exactly the
+// centre-relative field coordinates, so lattice indices/weights are per axis.
+struct MaskDev {
+    int mode;
+    int ap_kind;
+    double ap_a, ap_b;
+    // general form
+    const double* values;  // lattice, row-major mny x mnx
+    int mnx, mny;
+    double mdx, mdy;
+    double cpsi, spsi;
+    double cx, cy;
+    int fnx, fny;
+    double fdx, fdy;
+    // separable form, per field axis
+    const int* i0x;
+    const int* i1x;
+    const double* tx;
+    const double* xr;
+    const unsigned char* inx;
+    const int* j0y;
+    const int* j1y;
+    const double* ty;
+    const double* yr;
+    const unsigned char* iny;
+};
+
+struct EngineDev {
+    int kind;
+    int _pad;
+    double kz;  // ASM: (2 pi/lam_n)*z ; Fresnel: (2 pi/lambda)*z
+    double c;   // Fresnel: (pi*lambda)*z
+    const double* ax;  // ASM: (lam_n*fx)^2 ; Fresnel: fx*fx
+    const double* ay;
+    const unsigned char* mx;  // ASM band-limit mask per axis (1 = masked)
+    const unsigned char* my;
+    // Rayleigh-Sommerfeld kernel sampling
+    double lambda, z, k, sgn, dx, dy;
+};
+
+// Parameters of the fused element-wise ops (gs.rs:484-564, adjoint.rs:200-236,
+// merit.rs:143-165).  Real arrays have the transform's element type R unless
+// noted; `sums` are per-problem f64 accumulators.
+struct OpsDev {
+    int pre, post;
+    // PRE_POLAR: v = open[k] ? from_polar(amp[k], phase[k]) : 0
+    const void* amp;
+    const void* phase;
+    const unsigned char* open;
+    // PRE_SEED_MSE: v = (2 (|u|^2 - target)/n) u   (target f64, n = frame size)
+    const double* seed_target;
+    double n_frame;  // frame size n as f64 (divisor)
+    // POST_GS_TARGET: |U| vs target amplitude, relaxed projection
+    const double* tamp;          // target amplitude (f64)
+    const double* merit_target;  // MseToTarget target intensity (f64)
+    double beta_f;
+    double* err_out;  // |U| - T (f64, may be null)
+    // POST_GS_PHASE: dphi = wrap_pi(arg V - phi); phi <- encode(phi + beta_p dphi)
+    void* phase_io;  // element type R
+    double beta_p, lo, hi;
+    double* dphi_out;  // f64, may be null
+    // POST_ADJ_GRAD: mu = output; V = ckpt * t; grad += -2 Im(conj(mu) V); out = conj(t) mu
+    const void* ckpt;
+    double* grad;
+    // ENG_GS_PROJECT / ENG_MEASURE: Fourier-plane target amplitude, blocked
+    // like the intermediate ([P_x/B][P_y][B], element type R)
+    const void* tamp_blk;
+    double* sums;  // accumulators (4 per problem)
+};
+
+struct ProbDev {
+    int nx, ny, px, py, ox, oy;
+    int rows_cnt;  // transforms per workgroup in the row passes
+    int B;         // column block width of the intermediate layout
+    int base_row, base_col, base_kgen;  // tile prefix offsets per pass
+    int tiles_row, tiles_col, tiles_kgen;
+    FftPlanDev prow, pcol;
+    const void* in;
+    void* out;
+    void* work;  // N_y x P_x intermediate, blocked [P_x/B][N_y][B]
+    void* kbuf;  // RS kernel row spectra, blocked [P_x/B][P_y][B]
+    double scale;  // 1/(P_x P_y)
+    MaskDev mask;
+    EngineDev eng;
+    OpsDev ops;
+    int* flag;  // bit 0: non-finite input sample; bit 1: non-finite output
+};
+
+}  // namespace ocd

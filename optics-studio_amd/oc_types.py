@@ -129,6 +129,35 @@ class PropDesc(C.Structure):
                 ("medium", c_f64), ("mask", C.POINTER(PhaseMask)), ("in_", C.c_void_p), ("out", C.c_void_p)]
 
 
+class GsProblem(C.Structure):
+    """GsProblem + GsConfig (crates/optics-inverse/src/gs.rs:125-305), GridPhase carrier, MseToTarget merit."""
+    _fields_ = [("geom", BandGeom), ("input", C.c_void_p), ("z_total", c_f64), ("target_amplitude", C.c_void_p),
+                ("merit_target", C.c_void_p), ("asm_cfg", AsmConfig), ("aperture", ClearAperture),
+                ("cx", c_f64), ("cy", c_f64), ("lo", c_f64), ("hi", c_f64), ("carrier_values", C.c_void_p),
+                ("max_iterations", c_u64), ("patience", c_u64), ("merit_tolerance", c_f64),
+                ("variant", c_i32), ("init_kind", c_i32), ("beta", c_f64), ("init_phase", C.c_void_p)]
+
+
+class GsRecord(C.Structure):
+    """GsRecord (gs.rs:254-268)."""
+    _fields_ = [("iteration", c_u64), ("merit", c_f64), ("best_merit", c_f64), ("amplitude_mse", c_f64),
+                ("target_error_rms", c_f64), ("phase_update_rms", c_f64)]
+
+
+class GsSolution(C.Structure):
+    """GsSolution (gs.rs:271-305) with caller-allocated arrays."""
+    _fields_ = [("phase", C.c_void_p), ("merit", c_f64), ("best_iteration", c_u64), ("amplitude_mse", c_f64),
+                ("converged", c_i32), ("_pad0", c_i32), ("iterations", c_u64), ("history", C.c_void_p),
+                ("target_intensity", C.c_void_p), ("target_amplitude_error", C.c_void_p),
+                ("phase_error", C.c_void_p), ("illumination_amplitude", C.c_void_p)]
+
+
+class DesignStep(C.Structure):
+    """Design step (crates/optics-inverse/src/carrier.rs:35-43) with a PhaseMask element."""
+    _fields_ = [("has_element", c_i32), ("_pad0", c_i32), ("mask", PhaseMask), ("z_after", c_f64),
+                ("medium", c_f64)]
+
+
 def geom(nx, ny, dx, dy, lam):
     return BandGeom(int(nx), int(ny), float(dx), float(dy), float(lam))
 

@@ -1,0 +1,460 @@
+"""ctypes binding of liboptics_gpu.so (the C ABI in include/optics_gpu.h).
+
+The binding mirrors the reference's Rust call shapes (asm::propagate,
+fresnel::propagate, rayleigh_sommerfeld::propagate, PhaseMask::apply, ...):
+each takes a complex field plus grid pitch, wavelength, distance and the
+engine config, and returns (status, out, report).  Host-memory functions run
+the f64 drop-in; `*_dev_*` functions take torch CUDA tensors.
+
+There is no CPU fallback: importing this module loads the HIP library and
+raises if it is absent.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+import oc_types as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboptics_gpu.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` (hipcc, gfx950)")
+
+lib = C.CDLL(LIB_PATH)
+
+vp, c_f64, c_i32, c_u64, c_sz = C.c_void_p, C.c_double, C.c_int32, C.c_uint64, C.c_size_t
+P = C.POINTER
+
+EXPORTS = {
+    "oc_context_create": ([C.c_int, P(vp)], C.c_int),
+    "oc_context_destroy": ([vp], None),
+    "oc_context_set_stream": ([vp, vp], C.c_int),
+    "oc_context_stream": ([vp], vp),
+    "oc_context_synchronize": ([vp], C.c_int),
+    "oc_last_error": ([vp], C.c_char_p),
+    "oc_status_name": ([C.c_int], C.c_char_p),
+    "oc_version": ([], C.c_char_p),
+    "oc_context_enable_timing": ([vp, C.c_int], C.c_int),
+    "oc_context_pass_timing": ([vp, P(c_f64), P(c_u64), C.c_int], C.c_int),
+    "oc_asm_sampling_report": ([P(T.BandGeom), c_f64, P(T.AsmConfig), P(T.AsmReport)], C.c_int),
+    "oc_asm_propagate": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), vp, P(T.AsmReport)], C.c_int),
+    "oc_asm_propagate_dev_c128": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), P(T.PhaseMask), vp,
+                                   P(T.AsmReport)], C.c_int),
+    "oc_asm_propagate_dev_c64": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), P(T.PhaseMask), vp,
+                                  P(T.AsmReport)], C.c_int),
+    "oc_fresnel_sampling_report": ([P(T.BandGeom), c_f64, P(T.FresnelConfig), P(T.FresnelReport)], C.c_int),
+    "oc_fresnel_propagate": ([vp, P(T.BandGeom), vp, c_f64, P(T.FresnelConfig), vp, P(T.FresnelReport)],
+                             C.c_int),
+    "oc_fresnel_propagate_dev_c128": ([vp, P(T.BandGeom), vp, c_f64, P(T.FresnelConfig), P(T.PhaseMask), vp,
+                                       P(T.FresnelReport)], C.c_int),
+    "oc_fresnel_propagate_dev_c64": ([vp, P(T.BandGeom), vp, c_f64, P(T.FresnelConfig), P(T.PhaseMask), vp,
+                                      P(T.FresnelReport)], C.c_int),
+    "oc_rs_sampling_report": ([P(T.BandGeom), c_f64, P(T.RsConfig), P(T.RsReport)], C.c_int),
+    "oc_rs_propagate": ([vp, P(T.BandGeom), vp, c_f64, P(T.RsConfig), vp, P(T.RsReport)], C.c_int),
+    "oc_rs_propagate_dev_c128": ([vp, P(T.BandGeom), vp, c_f64, P(T.RsConfig), vp, P(T.RsReport)], C.c_int),
+    "oc_rs_propagate_dev_c64": ([vp, P(T.BandGeom), vp, c_f64, P(T.RsConfig), vp, P(T.RsReport)], C.c_int),
+    "oc_propagate_batch_dev_c64": ([vp, P(T.PropDesc), c_sz], C.c_int),
+    "oc_propagate_batch_dev_c128": ([vp, P(T.PropDesc), c_sz], C.c_int),
+    "oc_phasemask_validate": ([P(T.PhaseMask)], C.c_int),
+    "oc_phasemask_apply": ([vp, P(T.BandGeom), vp, P(T.PhaseMask), vp], C.c_int),
+    "oc_phasemask_apply_dev_c128": ([vp, P(T.BandGeom), vp, P(T.PhaseMask), vp], C.c_int),
+    "oc_phasemask_apply_dev_c64": ([vp, P(T.BandGeom), vp, P(T.PhaseMask), vp], C.c_int),
+    "oc_gs_run": ([vp, P(T.GsProblem), P(T.GsSolution)], C.c_int),
+    "oc_gs_fft_run_dev_c64": ([vp, c_u64, c_u64, vp, vp, vp, c_u64, c_u64, P(c_f64)], C.c_int),
+    "oc_gs_fft_run_dev_c128": ([vp, c_u64, c_u64, vp, vp, vp, c_u64, c_u64, P(c_f64)], C.c_int),
+    "oc_adjoint_asm": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), vp, P(T.AsmReport)], C.c_int),
+    "oc_adjoint_transmission": ([vp, P(T.BandGeom), P(T.PhaseMask), vp], C.c_int),
+    "oc_adjoint_push_back": ([vp, P(T.DesignStep), c_sz, P(T.BandGeom), vp, P(T.AsmConfig), vp, vp], C.c_int),
+    "oc_adjoint_grid_gradient_dev_c128": ([vp, P(T.DesignStep), c_sz, P(T.BandGeom), vp, P(T.AsmConfig), vp, vp,
+                                           P(c_f64)], C.c_int),
+}
+
+
+def _declare():
+    for name, (args, res) in EXPORTS.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+
+
+_declare()
+
+
+class OcError(RuntimeError):
+    pass
+
+
+class Context:
+    """oc_context: device, stream, scratch arena, plan caches."""
+
+    def __init__(self, device=0):
+        h = vp()
+        st = lib.oc_context_create(device, C.byref(h))
+        if st != 0:
+            raise OcError(f"oc_context_create failed: {T.STATUS.get(st, st)}")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.oc_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return lib.oc_context_stream(self.h)
+
+    def synchronize(self):
+        return lib.oc_context_synchronize(self.h)
+
+    def enable_timing(self, on=True):
+        lib.oc_context_enable_timing(self.h, 1 if on else 0)
+
+    def pass_timing(self, reset=True):
+        """{pass: (total_ms, launches)} for row_fwd, col, row_inv, rs_kgen, gs, other."""
+        ms = (c_f64 * 6)()
+        n = (c_u64 * 6)()
+        lib.oc_context_pass_timing(self.h, ms, n, 1 if reset else 0)
+        names = ["row_fwd", "col", "row_inv", "rs_kgen", "gs", "other"]
+        return {names[i]: (ms[i], n[i]) for i in range(6)}
+
+    def last_error(self):
+        return lib.oc_last_error(self.h).decode()
+
+
+_CTX = None
+
+
+def context():
+    global _CTX
+    if _CTX is None:
+        _CTX = Context(0)
+    return _CTX
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _geom(u, dx, dy, lam):
+    ny, nx = u.shape
+    return T.geom(nx, ny, dx, dy, lam)
+
+
+class Mask:
+    """PhaseMask::sampled arguments with HOST lattice values (f64 drop-in)."""
+
+    def __init__(self, values, dx, dy, orientation=0.0, aperture=(T.APERTURE_FIELD, 0.0, 0.0),
+                 center=(0.0, 0.0)):
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+        ny, nx = self.values.shape
+        kind, a, b = aperture
+        self.struct = T.PhaseMask(self.values.ctypes.data, nx, ny, dx, dy, orientation,
+                                  T.ClearAperture(kind, 0, a, b), center[0], center[1])
+
+
+# ---------------------------------------------------------------------------
+# f64 drop-in (host memory in, host memory out)
+# ---------------------------------------------------------------------------
+
+def asm_propagate(u, dx, dy, lam, z, cfg, ctx=None):
+    ctx = ctx or context()
+    u = _c128(u)
+    out = np.empty_like(u)
+    rep = T.AsmReport()
+    st = lib.oc_asm_propagate(ctx.h, C.byref(_geom(u, dx, dy, lam)), u.ctypes.data, z, C.byref(cfg),
+                              out.ctypes.data, C.byref(rep))
+    return st, out, rep
+
+
+def fresnel_propagate(u, dx, dy, lam, z, cfg, ctx=None):
+    ctx = ctx or context()
+    u = _c128(u)
+    out = np.empty_like(u)
+    rep = T.FresnelReport()
+    st = lib.oc_fresnel_propagate(ctx.h, C.byref(_geom(u, dx, dy, lam)), u.ctypes.data, z, C.byref(cfg),
+                                  out.ctypes.data, C.byref(rep))
+    return st, out, rep
+
+
+def rs_propagate(u, dx, dy, lam, z, cfg, ctx=None):
+    ctx = ctx or context()
+    u = _c128(u)
+    out = np.empty_like(u)
+    rep = T.RsReport()
+    st = lib.oc_rs_propagate(ctx.h, C.byref(_geom(u, dx, dy, lam)), u.ctypes.data, z, C.byref(cfg),
+                             out.ctypes.data, C.byref(rep))
+    return st, out, rep
+
+
+def phasemask_apply(u, dx, dy, lam, mask, ctx=None):
+    ctx = ctx or context()
+    u = _c128(u)
+    out = np.empty_like(u)
+    st = lib.oc_phasemask_apply(ctx.h, C.byref(_geom(u, dx, dy, lam)), u.ctypes.data, C.byref(mask.struct),
+                                out.ctypes.data)
+    return st, out
+
+
+def phasemask_validate(mask):
+    return lib.oc_phasemask_validate(C.byref(mask.struct))
+
+
+def asm_sampling_report(shape, dx, dy, lam, z, cfg):
+    ny, nx = shape
+    rep = T.AsmReport()
+    lib.oc_asm_sampling_report(C.byref(T.geom(nx, ny, dx, dy, lam)), z, C.byref(cfg), C.byref(rep))
+    return rep
+
+
+def fresnel_sampling_report(shape, dx, dy, lam, z, cfg):
+    ny, nx = shape
+    rep = T.FresnelReport()
+    lib.oc_fresnel_sampling_report(C.byref(T.geom(nx, ny, dx, dy, lam)), z, C.byref(cfg), C.byref(rep))
+    return rep
+
+
+def rs_sampling_report(shape, dx, dy, lam, z, cfg):
+    ny, nx = shape
+    rep = T.RsReport()
+    lib.oc_rs_sampling_report(C.byref(T.geom(nx, ny, dx, dy, lam)), z, C.byref(cfg), C.byref(rep))
+    return rep
+
+
+# ---------------------------------------------------------------------------
+# Device-resident forms over torch CUDA tensors
+# ---------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def dev_mask(values, dx, dy, orientation=0.0, aperture=(T.APERTURE_FIELD, 0.0, 0.0), center=(0.0, 0.0)):
+    """PhaseMask with its lattice copied to the device; returns (struct, keepalive tensor)."""
+    torch = _torch()
+    v = torch.as_tensor(np.ascontiguousarray(values, np.float64)).cuda()
+    ny, nx = v.shape
+    kind, a, b = aperture
+    s = T.PhaseMask(v.data_ptr(), nx, ny, dx, dy, orientation, T.ClearAperture(kind, 0, a, b), center[0],
+                    center[1])
+    return s, v
+
+
+def _dev_call(fn, u_dev, out_dev, geom, z, cfg, rep, mask_struct, ctx):
+    torch = _torch()
+    torch.cuda.synchronize()
+    args = [ctx.h, C.byref(geom), u_dev.data_ptr(), z, C.byref(cfg)]
+    if mask_struct is not False:
+        args.append(C.byref(mask_struct) if mask_struct is not None else None)
+    args += [out_dev.data_ptr(), C.byref(rep)]
+    return fn(*args)
+
+
+def asm_propagate_dev(u_dev, dx, dy, lam, z, cfg, mask=None, out=None, ctx=None):
+    """u_dev: torch complex64 or complex128 CUDA tensor (ny, nx)."""
+    torch = _torch()
+    ctx = ctx or context()
+    out = torch.empty_like(u_dev) if out is None else out
+    ny, nx = u_dev.shape
+    fn = lib.oc_asm_propagate_dev_c64 if u_dev.dtype == torch.complex64 else lib.oc_asm_propagate_dev_c128
+    rep = T.AsmReport()
+    st = _dev_call(fn, u_dev, out, T.geom(nx, ny, dx, dy, lam), z, cfg, rep, mask, ctx)
+    return st, out, rep
+
+
+def fresnel_propagate_dev(u_dev, dx, dy, lam, z, cfg, mask=None, out=None, ctx=None):
+    torch = _torch()
+    ctx = ctx or context()
+    out = torch.empty_like(u_dev) if out is None else out
+    ny, nx = u_dev.shape
+    fn = lib.oc_fresnel_propagate_dev_c64 if u_dev.dtype == torch.complex64 else lib.oc_fresnel_propagate_dev_c128
+    rep = T.FresnelReport()
+    st = _dev_call(fn, u_dev, out, T.geom(nx, ny, dx, dy, lam), z, cfg, rep, mask, ctx)
+    return st, out, rep
+
+
+def rs_propagate_dev(u_dev, dx, dy, lam, z, cfg, out=None, ctx=None):
+    torch = _torch()
+    ctx = ctx or context()
+    out = torch.empty_like(u_dev) if out is None else out
+    ny, nx = u_dev.shape
+    fn = lib.oc_rs_propagate_dev_c64 if u_dev.dtype == torch.complex64 else lib.oc_rs_propagate_dev_c128
+    rep = T.RsReport()
+    st = _dev_call(fn, u_dev, out, T.geom(nx, ny, dx, dy, lam), z, cfg, rep, False, ctx)
+    return st, out, rep
+
+
+def phasemask_apply_dev(u_dev, dx, dy, lam, mask_struct, out=None, ctx=None):
+    torch = _torch()
+    ctx = ctx or context()
+    out = torch.empty_like(u_dev) if out is None else out
+    ny, nx = u_dev.shape
+    fn = lib.oc_phasemask_apply_dev_c64 if u_dev.dtype == torch.complex64 else lib.oc_phasemask_apply_dev_c128
+    torch.cuda.synchronize()
+    st = fn(ctx.h, C.byref(T.geom(nx, ny, dx, dy, lam)), u_dev.data_ptr(), C.byref(mask_struct), out.data_ptr())
+    ctx.synchronize()
+    return st, out
+
+
+def propagate_batch_dev(descs, double=False, ctx=None):
+    """descs: ctypes array of PropDesc (device pointers).  Returns status."""
+    ctx = ctx or context()
+    _torch().cuda.synchronize()
+    fn = lib.oc_propagate_batch_dev_c128 if double else lib.oc_propagate_batch_dev_c64
+    return fn(ctx.h, descs, len(descs))
+
+
+class DevEngine:
+    """Adapter giving the device-resident forms the oracle's host call shapes
+    (numpy in, numpy out), for running the same parity/physics cases."""
+
+    def __init__(self, dtype="c64"):
+        torch = _torch()
+        self.cdt = torch.complex64 if dtype == "c64" else torch.complex128
+        self.ndt = np.complex64 if dtype == "c64" else np.complex128
+
+    def _up(self, u):
+        torch = _torch()
+        return torch.as_tensor(np.ascontiguousarray(u, self.ndt)).cuda()
+
+    @staticmethod
+    def _down(t):
+        return t.cpu().numpy().astype(np.complex128)
+
+    def asm_propagate(self, u, dx, dy, lam, z, cfg):
+        st, out, rep = asm_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
+        return st, self._down(out), rep
+
+    def fresnel_propagate(self, u, dx, dy, lam, z, cfg):
+        st, out, rep = fresnel_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
+        return st, self._down(out), rep
+
+    def rs_propagate(self, u, dx, dy, lam, z, cfg):
+        st, out, rep = rs_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
+        return st, self._down(out), rep
+
+    class Mask:
+        def __init__(self, values, dx, dy, orientation=0.0, aperture=(T.APERTURE_FIELD, 0.0, 0.0),
+                     center=(0.0, 0.0)):
+            self.struct, self.keep = dev_mask(values, dx, dy, orientation, aperture, center)
+
+    def phasemask_apply(self, u, dx, dy, lam, mask):
+        st, out = phasemask_apply_dev(self._up(u), dx, dy, lam, mask.struct)
+        return st, self._down(out)
+
+
+# ---------------------------------------------------------------------------
+# Gerchberg-Saxton
+# ---------------------------------------------------------------------------
+
+def gs_run(u_in, dx, dy, lam, z_total, target_amplitude, merit_target, asm_cfg, aperture=(0, 0.0, 0.0),
+           center=(0.0, 0.0), lo=-np.pi, hi=np.pi, carrier_values=None, max_iterations=100, patience=10,
+           merit_tolerance=1e-4, variant=0, beta=1.0, init_phase=None, ctx=None):
+    """gs::run (gs.rs:347) with the MseToTarget merit; same call shape as the oracle's."""
+    ctx = ctx or context()
+    u_in = _c128(u_in)
+    ny, nx = u_in.shape
+    ta = np.ascontiguousarray(target_amplitude, np.float64)
+    mt = np.ascontiguousarray(merit_target, np.float64)
+    cv = np.ascontiguousarray(carrier_values if carrier_values is not None else np.zeros((ny, nx)), np.float64)
+    ip = np.ascontiguousarray(init_phase if init_phase is not None else np.zeros((ny, nx)), np.float64)
+    prob = T.GsProblem(T.geom(nx, ny, dx, dy, lam), u_in.ctypes.data, z_total, ta.ctypes.data, mt.ctypes.data,
+                       asm_cfg, T.ClearAperture(aperture[0], 0, aperture[1], aperture[2]), center[0], center[1],
+                       lo, hi, cv.ctypes.data, max_iterations, patience, merit_tolerance, variant,
+                       2 if init_phase is not None else 0, beta, ip.ctypes.data)
+    phase, ti, tae, pe, il = (np.empty((ny, nx)) for _ in range(5))
+    hist = (T.GsRecord * max_iterations)()
+    sol = T.GsSolution(phase.ctypes.data, 0.0, 0, 0.0, 0, 0, 0, C.cast(hist, vp), ti.ctypes.data,
+                       tae.ctypes.data, pe.ctypes.data, il.ctypes.data)
+    st = lib.oc_gs_run(ctx.h, C.byref(prob), C.byref(sol))
+    if st != 0:
+        return st, None
+    k = sol.iterations
+    history = [(h.iteration, h.merit, h.best_merit, h.amplitude_mse, h.target_error_rms, h.phase_update_rms)
+               for h in hist[:k]]
+    return st, dict(phase=phase, merit=sol.merit, best_iteration=sol.best_iteration,
+                    amplitude_mse=sol.amplitude_mse, converged=bool(sol.converged), iterations=k, history=history,
+                    target_intensity=ti, target_amplitude_error=tae, phase_error=pe, illumination_amplitude=il)
+
+
+def gs_fft_run_dev(source_amp, target_amp, phases, iterations, graph_chunk=0, ctx=None):
+    """FFT-form GS over restarts on torch CUDA tensors (float32 -> c64 form,
+    float64 -> c128 form).  phases (R, n, n) is updated in place.
+    Returns (status, final normalised intensity error per restart)."""
+    torch = _torch()
+    ctx = ctx or context()
+    r, n, _ = phases.shape
+    err = (c_f64 * r)()
+    fn = lib.oc_gs_fft_run_dev_c64 if phases.dtype == torch.float32 else lib.oc_gs_fft_run_dev_c128
+    torch.cuda.synchronize()
+    st = fn(ctx.h, n, r, source_amp.data_ptr(), target_amp.data_ptr(), phases.data_ptr(), iterations, graph_chunk,
+            err)
+    return st, np.array(err[:])
+
+
+# ---------------------------------------------------------------------------
+# Adjoint
+# ---------------------------------------------------------------------------
+
+def _steps(steps):
+    """steps: list of (Mask or None, z_after, medium) -> ctypes array."""
+    arr = (T.DesignStep * len(steps))()
+    for i, (m, z, med) in enumerate(steps):
+        arr[i].has_element = 1 if m is not None else 0
+        if m is not None:
+            arr[i].mask = m.struct
+        arr[i].z_after = z
+        arr[i].medium = med
+    return arr
+
+
+def adjoint_push_back(steps, u, dx, dy, lam, asm_cfg, dmdi, ctx=None):
+    """adjoint::push_back (adjoint.rs:140) with host buffers; oracle call shape."""
+    ctx = ctx or context()
+    u = _c128(u)
+    ne = sum(1 for s in steps if s[0] is not None)
+    out = np.zeros((ne,) + u.shape)
+    d = np.ascontiguousarray(dmdi, np.float64)
+    st = lib.oc_adjoint_push_back(ctx.h, _steps(steps), len(steps), C.byref(_geom(u, dx, dy, lam)), u.ctypes.data,
+                                  C.byref(asm_cfg), d.ctypes.data, out.ctypes.data)
+    return st, out
+
+
+def adjoint_transmission(mask, shape, dx, dy, lam, ctx=None):
+    ctx = ctx or context()
+    ny, nx = shape
+    out = np.empty((ny, nx), np.complex128)
+    st = lib.oc_adjoint_transmission(ctx.h, C.byref(T.geom(nx, ny, dx, dy, lam)), C.byref(mask.struct),
+                                     out.ctypes.data)
+    return st, out
+
+
+def adjoint_grid_gradient_dev(dev_steps, u_dev, dx, dy, lam, asm_cfg, target_dev, grads_dev=None, ctx=None):
+    """adjoint::grid_gradient (adjoint.rs:268) for one band/scene with MseToTarget.
+    dev_steps: list of (mask struct with device values or None, z_after, medium).
+    Returns (status, merit); gradients land in grads_dev (n_elements, ny, nx) float64."""
+    torch = _torch()
+    ctx = ctx or context()
+    arr = (T.DesignStep * len(dev_steps))()
+    for i, (ms, z, med) in enumerate(dev_steps):
+        arr[i].has_element = 1 if ms is not None else 0
+        if ms is not None:
+            arr[i].mask = ms
+        arr[i].z_after = z
+        arr[i].medium = med
+    ny, nx = u_dev.shape
+    merit = c_f64()
+    torch.cuda.synchronize()
+    st = lib.oc_adjoint_grid_gradient_dev_c128(ctx.h, arr, len(dev_steps), C.byref(T.geom(nx, ny, dx, dy, lam)),
+                                               u_dev.data_ptr(), C.byref(asm_cfg), target_dev.data_ptr(),
+                                               grads_dev.data_ptr() if grads_dev is not None else None,
+                                               C.byref(merit))
+    return st, merit.value

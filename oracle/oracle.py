@@ -39,22 +39,7 @@ def lib():
     return _lib
 
 
-class GsProblem(C.Structure):
-    _fields_ = [("geom", T.BandGeom), ("input", vp), ("z_total", c_f64), ("target_amplitude", vp),
-                ("merit_target", vp), ("asm_cfg", T.AsmConfig), ("aperture", T.ClearAperture),
-                ("cx", c_f64), ("cy", c_f64), ("lo", c_f64), ("hi", c_f64), ("carrier_values", vp),
-                ("max_iterations", c_u64), ("patience", c_u64), ("merit_tolerance", c_f64),
-                ("variant", c_i32), ("init_kind", c_i32), ("beta", c_f64), ("init_phase", vp)]
-
-
-class GsRecord(C.Structure):
-    _fields_ = [("iteration", c_u64), ("merit", c_f64), ("best_merit", c_f64), ("amplitude_mse", c_f64),
-                ("target_error_rms", c_f64), ("phase_update_rms", c_f64)]
-
-
-class DesignStep(C.Structure):
-    _fields_ = [("has_element", c_i32), ("_pad0", c_i32), ("mask", T.PhaseMask), ("z_after", c_f64),
-                ("medium", c_f64)]
+GsProblem, GsRecord, DesignStep = T.GsProblem, T.GsRecord, T.DesignStep
 
 
 def _declare(L):
