@@ -40,7 +40,25 @@ def parse():
     ap.add_argument("--bands", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the f64 and host drop-in legs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of 1 GiB (8 launches) for PMC counter calibration")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain copy kernel of known size: under rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE its per-dispatch counters give the KiB each counter reports
+    per byte actually moved."""
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(8):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read_per_launch": nbytes, "bytes_written_per_launch": nbytes,
+                      "launches": 8}), flush=True)
 
 
 def read_traffic():
@@ -57,6 +75,9 @@ def read_traffic():
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by band/problem, "
               "see DESIGN.md).", file=sys.stderr)
@@ -106,6 +127,9 @@ def main():
         if st:
             raise RuntimeError(f"Fresnel batch failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
 
+    # pick the fastest fast-kernel layout for the padded length (untimed)
+    variant, variant_ms = ctx.autotune(2 * n, double=False)
+    log(f"autotune P={2 * n} c64: variant {variant}, ms per variant {variant_ms}")
     log(f"workload: {nb} bands x 2048^2, c64; warmup {args.warmup}")
     for _ in range(args.warmup):
         step(d_asm, d_fr)
@@ -140,7 +164,7 @@ def main():
                   "alg_GBps": alg[k] / (timing[k][0] / max(timing[k][1], 1) * 1e-3) / 1e9}
               for k in alg if timing[k][1]}
     step_bytes = 2 * nb * (2 * n * n + 4 * n * P) * B + nb * n * n * 8  # + f64 mask lattice read per ASM band
-    extra = {"passes": passes, "step_alg_GB": step_bytes / 1e9,
+    extra = {"passes": passes, "fft_variant": variant, "autotune_ms": variant_ms, "step_alg_GB": step_bytes / 1e9,
              "step_alg_GBps": step_bytes / (elapsed / args.steps) / 1e9}
 
     # parity of this run's last band output against the f64 oracle (and the

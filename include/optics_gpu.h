@@ -206,6 +206,24 @@ int oc_context_enable_timing(oc_context* ctx, int enable);
 int oc_context_pass_timing(oc_context* ctx, double* ms, uint64_t* launches,
                            int reset);
 
+/* Autotuner (no reference counterpart).  Power-of-two padded lengths
+   64..8192 have compile-time-specialised kernels in up to two workgroup
+   layouts (variant 0: up to 1024 threads and wider contiguous segments;
+   variant 1: up to 512 threads and twice the registers per thread).
+   oc_autotune_fft times a full masked-ASM propagation round of every variant
+   at length n on a synthetic batch and keeps the fastest for this context;
+   ms_out (optional, 2 entries) receives the per-variant times (negative for
+   variants that do not exist).  oc_context_set_fft_variant forces a choice;
+   oc_context_fft_variant reports the current one (-1: no specialised kernel).
+   oc_context_force_generic(ctx, 1) routes every batch through the
+   runtime-plan kernels instead. */
+int oc_autotune_fft(oc_context* ctx, uint64_t n, int double_precision,
+                    int* variant_out, double* ms_out);
+int oc_context_set_fft_variant(oc_context* ctx, uint64_t n,
+                               int double_precision, int variant);
+int oc_context_fft_variant(oc_context* ctx, uint64_t n, int double_precision);
+int oc_context_force_generic(oc_context* ctx, int enable);
+
 /* ------------------------------------------------------------------------ */
 /* Angular spectrum (src/propagation/asm.rs).                                */
 /* ------------------------------------------------------------------------ */

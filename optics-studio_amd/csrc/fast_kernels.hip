@@ -1,0 +1,356 @@
+// Power-of-two pass kernels (P_x = P_y = N, 64 <= N <= 8192) on the
+// register-resident FFT of fft_fast.hpp.  Same passes, fused ops and blocked
+// intermediate as prop_kernels.hip; every global access is coalesced and no
+// transform data goes through LDS except the inter-stage exchanges.
+//
+// Thread maps:
+//   row passes : G rows per workgroup, thread (tb, g, tq) with t = tb + B*tq,
+//                so a wave touches B*G contiguous elements of the blocked
+//                intermediate per access.
+//   column pass: B columns per workgroup, thread (g, t) with g fastest, so a
+//                wave reads/writes 64 contiguous elements of the slab.
+//
+// Kernels are specialised at compile time on what a launch holds uniformly
+// (the pre-op of the row pass, the engine of the column pass, whether the
+// inverse row pass has a post-op), so a plain propagation does not pay
+// registers for the GS and adjoint epilogues.  When a launch mixes kinds the
+// general kernels dispatch per problem.
+//
+// The file is compiled once per element type: OC_FAST_DOUBLE selects
+// complex128, otherwise complex64.
+#include <hip/hip_runtime.h>
+
+#include "fft_fast.hpp"
+#include "pass_ops.hpp"
+
+namespace ocd {
+
+// Rows (and columns) per workgroup for a given element type and length:
+// at most MAXT threads, LDS within the 140 KB budget, at most 16.
+// Variant V = 0 allows 1024-thread workgroups (more rows/columns per
+// workgroup, wider contiguous segments, 128 VGPRs); V = 1 caps them at 512
+// threads (256 VGPRs, no spills at large N).  The autotuner picks per size.
+template <typename R, int N, int V = 0>
+struct FastLayout {
+    static constexpr int T = N / 16;
+    static constexpr int ESZ = (int)sizeof(cpx<R>);
+    static constexpr int BYTES = Pow2Plan<N>::SLOTS * ESZ;
+    static constexpr int MAXT = V == 0 ? 1024 : 512;
+    static constexpr int G0 = (MAXT / T) < (140 * 1024 / BYTES) ? (MAXT / T) : (140 * 1024 / BYTES);
+    static constexpr int G = G0 < 16 ? G0 : 16;
+    static constexpr int B = G < T ? G : T;
+    static_assert(G >= 1 && B >= 1 && (T % B) == 0, "layout");
+};
+
+// PK: pre-op (PRE_NONE, PRE_MASK, or -1 for the descriptor's).
+template <typename R, int N, int V, int PK>
+__global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd(const ProbDev* __restrict__ probs,
+                                                                         int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, G = LY::G, Q = LY::B;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
+    const int t = tb + Q * tq;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
+        const int iy = (tile - P.base_row) * G + g;
+        const bool rowok = iy < P.ny;
+        const int ox = P.ox, nx = P.nx;
+        bool bad = false;
+        double acc = 0.0;
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int x = t + m * T;
+            v[m] = {R(0), R(0)};
+            if (rowok && x >= ox && x < ox + nx) v[m] = load_pre<R, PK>(P, x - ox, iy, bad, acc);
+        }
+        if (bad) atomicOr(P.flag, 1);
+        if (PK < 0 && P.ops.pre == PRE_SEED_MSE)
+            accumulate4(acc, 0.0, 0.0, 0.0, P.ops.sums ? P.ops.sums + 3 : nullptr);
+        fft_regs<R, N, G, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        if (rowok) {
+            gmem<cpx<R>>* __restrict__ work = gp(static_cast<cpx<R>*>(P.work));
+            const int ny = P.ny;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) work[((size_t)(tq + m * (T / Q)) * ny + iy) * Q + tb] = v[m];
+        }
+    }
+}
+
+// EK: engine (ENG_ASM, ENG_FRESNEL, or -1 for the descriptor's); RS: the
+// Rayleigh-Sommerfeld product with the kernel spectrum instead.
+template <typename R, int N, int V, bool RS, int EK>
+__global__ __launch_bounds__((FastLayout<R, N, V>::B * (N / 16))) void f_col(const ProbDev* __restrict__ probs, int nprob,
+                                                                     int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, B = LY::B;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int g = tid % B, t = tid / B;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
+        const int q = tile - P.base_col;
+        const int ny = P.ny, oy = P.oy;
+        const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
+        gmem<cpx<R>>* __restrict__ work = gp(static_cast<cpx<R>*>(P.work));
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int y = t + m * T;
+            v[m] = {R(0), R(0)};
+            if (y >= oy && y < oy + ny) v[m] = work[((size_t)q * ny + (y - oy)) * B + g];
+        }
+        fft_regs<R, N, B, -1>(v, lds, g, t, tw);
+        const int kx = q * B + g;
+        if constexpr (RS) {
+            const gmem<const cpx<R>>* __restrict__ kb = gp(static_cast<const cpx<R>*>(P.kbuf));
+            cpx<R> w[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) w[m] = kb[((size_t)q * N + (t + m * T)) * B + g];
+            fft_regs<R, N, B, -1>(w, lds, g, t, tw);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], w[m]);
+        } else {
+            const int ek = EK >= 0 ? EK : P.eng.kind;
+            if (ek == ENG_MEASURE) {
+                const gmem<const R>* tb = gp(static_cast<const R*>(P.ops.tamp_blk));
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                    const double in2 = (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
+                    const double ta = tb[((size_t)q * N + (t + m * T)) * B + g];
+                    s0 += in2;
+                    s1 += in2 * in2;
+                    s2 += in2 * (ta * ta);
+                }
+                accumulate4(s0, s1, s2, 0.0, P.ops.sums);
+                continue;
+            } else if (ek == ENG_GS_PROJECT) {
+                const gmem<const R>* tb = gp(static_cast<const R*>(P.ops.tamp_blk));
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                    const R a = tb[((size_t)q * N + (t + m * T)) * B + g];
+                    const R mg = hypot(v[m].re, v[m].im);
+                    v[m] = mg > R(0) ? cpx<R>{v[m].re * (a / mg), v[m].im * (a / mg)} : cpx<R>{a, R(0)};
+                }
+            } else if (ek != ENG_NONE) {
+#pragma unroll
+                for (int m = 0; m < 16; ++m) v[m] = transfer<R, EK>(P.eng, kx, t + m * T, v[m]);
+            }
+        }
+        fft_regs<R, N, B, +1>(v, lds, g, t, tw);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int y = t + m * T;
+            if (y >= oy && y < oy + ny) work[((size_t)q * ny + (y - oy)) * B + g] = v[m];
+        }
+    }
+}
+
+// PLAIN: every problem of the launch stores the cropped, scaled field as is
+// (no post-op).
+template <typename R, int N, int V, bool PLAIN>
+__global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv(const ProbDev* __restrict__ probs,
+                                                                         int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, G = LY::G, Q = LY::B;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
+    const int t = tb + Q * tq;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
+        const int iy = (tile - P.base_row) * G + g;
+        const bool rowok = iy < P.ny;
+        const int ny = P.ny, nx = P.nx, ox = P.ox;
+        const gmem<const cpx<R>>* __restrict__ work = gp(static_cast<const cpx<R>*>(P.work));
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            v[m] = {R(0), R(0)};
+            if (rowok) v[m] = work[((size_t)(tq + m * (T / Q)) * ny + iy) * Q + tb];
+        }
+        fft_regs<R, N, G, +1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        const R s = (R)P.scale;
+        bool bad = false;
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+        if (rowok) {
+            gmem<cpx<R>>* __restrict__ out = gp(static_cast<cpx<R>*>(P.out)) + (size_t)iy * nx;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int x = t + m * T - ox;
+                if (x >= 0 && x < nx) {
+                    const cpx<R> o = {v[m].re * s, v[m].im * s};
+                    bad |= !finite_c(o);
+                    if constexpr (PLAIN) out[x] = o;
+                    else store_post<R>(P, x, iy, o, acc0, acc1, acc2);
+                }
+            }
+        }
+        if (bad) atomicOr(P.flag, 2);
+        if (!PLAIN && (P.ops.post == POST_GS_TARGET || P.ops.post == POST_GS_PHASE))
+            accumulate4(acc0, acc1, acc2, 0.0, P.ops.sums);
+    }
+}
+
+template <typename R, int N, int V>
+__global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen(const ProbDev* __restrict__ probs,
+                                                                         int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, G = LY::G, Q = LY::B;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
+    const int t = tb + Q * tq;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[find_problem<PASS_KGEN>(probs, nprob, tile)];
+        const int y = (tile - P.base_kgen) * G + g;
+        const bool rowok = y < N;
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            v[m] = rowok ? rs_sample<R>(P.eng, t + m * T, y, N, N) : cpx<R>{R(0), R(0)};
+        fft_regs<R, N, G, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        if (rowok) {
+            gmem<cpx<R>>* __restrict__ kb = gp(static_cast<cpx<R>*>(P.kbuf));
+#pragma unroll
+            for (int m = 0; m < 16; ++m) kb[((size_t)(tq + m * (T / Q)) * N + y) * Q + tb] = v[m];
+        }
+    }
+}
+
+}  // namespace ocd
+
+namespace ocl {
+using namespace ocd;
+
+#ifdef OC_FAST_DOUBLE
+using Real = double;
+#define OC_FAST_FN(name) name##_c128
+#else
+using Real = float;
+#define OC_FAST_FN(name) name##_c64
+#endif
+
+template <int N, int V>
+static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int total, int grid, hipStream_t st) {
+    using R = Real;
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16;
+    const size_t row_lds = (size_t)LY::G * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
+    const size_t col_lds = (size_t)LY::B * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
+    const dim3 brow(LY::G * T), bcol(LY::B * T), g(grid);
+    switch (s.pass) {
+        case 0:
+            if (s.pre_kind == PRE_NONE)
+                hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_NONE>), g, brow, row_lds, st, d, np, total);
+            else if (s.pre_kind == PRE_MASK)
+                hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_MASK>), g, brow, row_lds, st, d, np, total);
+            else
+                hipLaunchKernelGGL((f_row_fwd<R, N, V, -1>), g, brow, row_lds, st, d, np, total);
+            break;
+        case 1:
+            if (s.rs)
+                hipLaunchKernelGGL((f_col<R, N, V, true, -1>), g, bcol, col_lds, st, d, np, total);
+            else if (s.col_kind == ENG_ASM)
+                hipLaunchKernelGGL((f_col<R, N, V, false, ENG_ASM>), g, bcol, col_lds, st, d, np, total);
+            else if (s.col_kind == ENG_FRESNEL)
+                hipLaunchKernelGGL((f_col<R, N, V, false, ENG_FRESNEL>), g, bcol, col_lds, st, d, np, total);
+            else
+                hipLaunchKernelGGL((f_col<R, N, V, false, -1>), g, bcol, col_lds, st, d, np, total);
+            break;
+        case 2:
+            if (s.plain_post) hipLaunchKernelGGL((f_row_inv<R, N, V, true>), g, brow, row_lds, st, d, np, total);
+            else hipLaunchKernelGGL((f_row_inv<R, N, V, false>), g, brow, row_lds, st, d, np, total);
+            break;
+        case 3: hipLaunchKernelGGL((f_rs_kgen<R, N, V>), g, brow, row_lds, st, d, np, total); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+static hipError_t allow_lds(const void* kernel) {
+    return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+template <int N, int V>
+static hipError_t attr_nv() {
+    using R = Real;
+    const void* k[] = {(const void*)f_row_fwd<R, N, V, PRE_NONE>,
+                       (const void*)f_row_fwd<R, N, V, PRE_MASK>,
+                       (const void*)f_row_fwd<R, N, V, -1>,
+                       (const void*)f_col<R, N, V, true, -1>,
+                       (const void*)f_col<R, N, V, false, ENG_ASM>,
+                       (const void*)f_col<R, N, V, false, ENG_FRESNEL>,
+                       (const void*)f_col<R, N, V, false, -1>,
+                       (const void*)f_row_inv<R, N, V, true>,
+                       (const void*)f_row_inv<R, N, V, false>,
+                       (const void*)f_rs_kgen<R, N, V>};
+    for (const void* f : k) {
+        const hipError_t e = allow_lds(f);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Sizes with one layout (64..512) and with two (the 512-thread cap only
+// changes the layout when N/16 * 16 > 512, i.e. N >= 1024).
+#ifndef OC_FAST_SIZES_1
+#define OC_FAST_SIZES_1(X) X(64) X(128) X(256) X(512)
+#define OC_FAST_SIZES_2(X) X(1024) X(2048) X(4096) X(8192)
+#endif
+
+hipError_t OC_FAST_FN(fast_launch)(const FastSel& s, const ProbDev* d, int np, int total, int grid, hipStream_t st) {
+    switch (s.n) {
+#define OC_C1(NN) \
+    case NN: return launch_nv<NN, 0>(s, d, np, total, grid, st);
+        OC_FAST_SIZES_1(OC_C1)
+#undef OC_C1
+#define OC_C2(NN) \
+    case NN: return s.v ? launch_nv<NN, 1>(s, d, np, total, grid, st) : launch_nv<NN, 0>(s, d, np, total, grid, st);
+        OC_FAST_SIZES_2(OC_C2)
+#undef OC_C2
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Workgroup layout of length n, variant v (G = B = 0 when there is no kernel).
+void OC_FAST_FN(fast_layout)(int n, int v, int* G, int* B) {
+    using R = Real;
+    switch (n) {
+#define OC_L1(NN) \
+    case NN: *G = FastLayout<R, NN, 0>::G; *B = FastLayout<R, NN, 0>::B; return;
+        OC_FAST_SIZES_1(OC_L1)
+#undef OC_L1
+#define OC_L2(NN)                                                   \
+    case NN:                                                        \
+        *G = v ? FastLayout<R, NN, 1>::G : FastLayout<R, NN, 0>::G; \
+        *B = v ? FastLayout<R, NN, 1>::B : FastLayout<R, NN, 0>::B; \
+        return;
+        OC_FAST_SIZES_2(OC_L2)
+#undef OC_L2
+        default: *G = *B = 0;
+    }
+}
+
+hipError_t OC_FAST_FN(fast_set_lds_limits)() {
+    hipError_t e;
+#define OC_A1(NN) \
+    if ((e = attr_nv<NN, 0>())) return e;
+    OC_FAST_SIZES_1(OC_A1)
+    OC_FAST_SIZES_2(OC_A1)
+#undef OC_A1
+#define OC_A2(NN) \
+    if ((e = attr_nv<NN, 1>())) return e;
+    OC_FAST_SIZES_2(OC_A2)
+#undef OC_A2
+    return hipSuccess;
+}
+
+}  // namespace ocl

@@ -16,12 +16,32 @@
 
 namespace ocd {
 
+// Aligned to its full size so that LDS and global accesses are single
+// 64/128-bit operations.
 template <typename R>
-struct cpx {
+struct alignas(2 * sizeof(R)) cpx {
     R re, im;
 };
 using c64 = cpx<float>;
 using c128 = cpx<double>;
+
+// Global-memory view of a generic pointer.  Every buffer the kernels address
+// through a descriptor lives in device (global) memory; accessing it through
+// this view emits global_* instead of flat_* instructions, which do not tie
+// up the LDS counter.
+// (The host pass of a .hip file only type-checks device code; it sees the
+// plain type.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using gmem = __attribute__((address_space(1))) T;
+#else
+template <typename T>
+using gmem = T;
+#endif
+template <typename T>
+__device__ __forceinline__ gmem<T>* gp(T* p) {
+    return (gmem<T>*)p;
+}
 
 template <typename R>
 __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
@@ -167,7 +187,7 @@ __device__ __forceinline__ void dft8(cpx<R>* v) {
 
 // Radix 16 as 4 x 4: X[q + 4s] = DFT4_r( W16^{rq} * DFT4_m(v[r + 4m]) ).
 template <int DIR, typename R>
-__device__ __forceinline__ void dft16(cpx<R>* v) {
+__device__ __forceinline__ void dft16p(cpx<R>* v) {
     const R c1 = R(0.92387953251128675612818318939679);  // cos(pi/8)
     const R s1 = R(0.38268343236508977172845998403040);  // sin(pi/8)
     const R h = R(0.70710678118654752440084436210485);
@@ -184,18 +204,24 @@ __device__ __forceinline__ void dft16(cpx<R>* v) {
     v[3 + 4] = twmul<DIR>(v[3 + 4], s1, c1);
     v[3 + 8] = twmul<DIR>(v[3 + 8], -h, h);
     v[3 + 12] = twmul<DIR>(v[3 + 12], -c1, -s1);
-    cpx<R> o[16];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        cpx<R> a0 = v[0 + 4 * q], a1 = v[1 + 4 * q], a2 = v[2 + 4 * q], a3 = v[3 + 4 * q];
-        dft4<DIR>(a0, a1, a2, a3);
-        o[q] = a0;
-        o[q + 4] = a1;
-        o[q + 8] = a2;
-        o[q + 12] = a3;
-    }
-#pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = o[i];
+    for (int q = 0; q < 4; ++q) dft4<DIR>(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    // X[q + 4s] now sits in v[4q + s]
+}
+
+// In-register transposed-output radix 16: X[q + 4s] is left in v[4q + s]
+// (see dft16p_pos).  dft16 below restores natural order.
+template <int DIR, typename R>
+__device__ __forceinline__ void dft16(cpx<R>* v) {
+    dft16p<DIR>(v);
+    cpx<R> t;
+    // transpose the 4x4 index grid in place
+    t = v[1]; v[1] = v[4]; v[4] = t;
+    t = v[2]; v[2] = v[8]; v[8] = t;
+    t = v[3]; v[3] = v[12]; v[12] = t;
+    t = v[6]; v[6] = v[9]; v[9] = t;
+    t = v[7]; v[7] = v[13]; v[13] = t;
+    t = v[11]; v[11] = v[14]; v[14] = t;
 }
 
 template <int RAD, int DIR, typename R>
@@ -225,7 +251,7 @@ struct FftPlanDev {
 
 template <int DIR, typename R>
 __device__ __forceinline__ cpx<R> tw_at(const cpx<R>* __restrict__ tw, int idx) {
-    cpx<R> w = tw[idx];
+    cpx<R> w = gp(tw)[idx];
     if (DIR > 0) w.im = -w.im;
     return w;
 }

@@ -1,0 +1,140 @@
+// Register-resident power-of-two FFTs for gfx950 with compile-time plans.
+//
+// A group of G transforms of length N runs on G*N/16 threads; thread (g, t)
+// keeps 16 elements in registers.  On entry it holds x[t + m*T] (T = N/16,
+// m < 16) and on exit the transform X[t + m*T]: natural order on both sides,
+// so callers load and store straight from/to global memory and a forward
+// transform can be followed by an element-wise op and an inverse transform
+// without any exchange in between.  Stages are radix 16 (Stockham, input
+// pattern t + r*T) with a final radix-L stage (L = N / 16^k in {2, 4, 8}) that
+// works on the thread's own registers; consecutive stages exchange through
+// LDS laid out as slot(idx)*G + g with one pad element per 16.
+#pragma once
+
+#include "fft_device.hpp"
+
+namespace ocd {
+
+template <int N>
+struct Pow2Plan {
+    static constexpr int log2n() {
+        int l = 0;
+        while ((1 << l) < N) ++l;
+        return l;
+    }
+    static constexpr int LOG = log2n();
+    static constexpr int NS16 = LOG / 4;
+    static constexpr int L = 1 << (LOG % 4);
+    static constexpr int T = N / 16;
+    static constexpr int NSTAGES = NS16 + (L > 1 ? 1 : 0);
+    static constexpr int SLOTS = N + N / 16;  // LDS elements per transform
+    static_assert((1 << LOG) == N && N >= 32, "power of two >= 32");
+};
+
+__device__ __forceinline__ int fslot(int idx) { return idx + (idx >> 4); }
+
+// Stage-output scatter to LDS for radix-16 stage with stride Ns (a power of
+// 16, a constant once the stage loop is unrolled), then the next stage's
+// gather x[t + m*T].  Both sides are written as one per-thread base plus
+// compile-time steps: with Ns = 1, slot(16 t + r) = 17 t + r; with Ns a
+// multiple of 16, slot(b + r Ns) = slot(b) + r (Ns + Ns/16); likewise
+// slot(t + m T) = slot(t) + m (T + T/16) when 16 | T.  The accesses then use
+// immediate offsets from a single address register instead of sixteen.
+template <typename R, int N, int G>
+__device__ __forceinline__ void fft_exchange16(cpx<R> (&v)[16], cpx<R>* __restrict__ lds, int g, int t, int Ns) {
+    constexpr int T = N / 16;
+    const int k = t & (Ns - 1);
+    const int base = (t - k) * 16 + k;
+    const int wstep = Ns == 1 ? 1 : Ns + Ns / 16;
+    cpx<R>* __restrict__ wp = lds + (Ns == 1 ? 17 * t : fslot(base)) * G + g;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) wp[r * wstep * G] = v[r];
+    __syncthreads();
+    if constexpr (T % 16 == 0) {
+        const cpx<R>* __restrict__ rp = lds + fslot(t) * G + g;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = rp[m * (T + T / 16) * G];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = lds[fslot(t + m * T) * G + g];
+    }
+    __syncthreads();
+}
+
+template <int DIR, typename R>
+__device__ __forceinline__ void dft_small(cpx<R>* w, int L) {
+    if (L == 2) dft2<DIR>(w[0], w[1]);
+    else if (L == 4) dft4<DIR>(w[0], w[1], w[2], w[3]);
+    else dft8<DIR>(w);
+}
+
+// G transforms of length N in place in registers (see header).  `active`
+// threads participate in the arithmetic; every thread of the block must call
+// this (barriers).
+template <typename R, int N, int G, int DIR>
+__device__ __forceinline__ void fft_regs(cpx<R> (&v)[16], cpx<R>* __restrict__ lds, int g, int t,
+                                         const cpx<R>* __restrict__ tw) {
+    using PL = Pow2Plan<N>;
+    constexpr int T = PL::T;
+#pragma unroll
+    for (int s = 0; s < PL::NS16; ++s) {
+        const int Ns = 1 << (4 * s);
+        if (s > 0) {
+            // w^r for r = 1..15 from the table entries w, w^2, w^4, w^8 with
+            // at most three extra products each.
+            const int k = t & (Ns - 1);
+            const int e = k * (N / (Ns * 16));
+            const cpx<R> w1 = tw_at<DIR>(tw, e), w2 = tw_at<DIR>(tw, 2 * e);
+            const cpx<R> w4 = tw_at<DIR>(tw, 4 * e), w8 = tw_at<DIR>(tw, 8 * e);
+            const cpx<R> w3 = cmul(w1, w2), w5 = cmul(w1, w4), w6 = cmul(w2, w4), w7 = cmul(w3, w4);
+            v[1] = cmul(v[1], w1);
+            v[2] = cmul(v[2], w2);
+            v[3] = cmul(v[3], w3);
+            v[4] = cmul(v[4], w4);
+            v[5] = cmul(v[5], w5);
+            v[6] = cmul(v[6], w6);
+            v[7] = cmul(v[7], w7);
+            v[8] = cmul(v[8], w8);
+            v[9] = cmul(v[9], cmul(w1, w8));
+            v[10] = cmul(v[10], cmul(w2, w8));
+            v[11] = cmul(v[11], cmul(w3, w8));
+            v[12] = cmul(v[12], cmul(w4, w8));
+            v[13] = cmul(v[13], cmul(w5, w8));
+            v[14] = cmul(v[14], cmul(w6, w8));
+            v[15] = cmul(v[15], cmul(w7, w8));
+        }
+        dft16<DIR>(v);
+        if (s + 1 < PL::NSTAGES) fft_exchange16<R, N, G>(v, lds, g, t, Ns);
+    }
+    if constexpr (PL::L > 1) {
+        constexpr int L = PL::L;
+        constexpr int NB = 16 / L;  // butterflies per thread
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = t + b * T;
+            cpx<R> w[L];
+#pragma unroll
+            for (int r = 0; r < L; ++r) w[r] = v[b + r * NB];
+            const cpx<R> u1 = tw_at<DIR>(tw, j);
+            w[1] = cmul(w[1], u1);
+            if constexpr (L >= 4) {
+                const cpx<R> u2 = tw_at<DIR>(tw, 2 * j);
+                w[2] = cmul(w[2], u2);
+                w[3] = cmul(w[3], cmul(u1, u2));
+                if constexpr (L == 8) {
+                    const cpx<R> u4 = tw_at<DIR>(tw, 4 * j);
+                    const cpx<R> u3 = cmul(u1, u2);
+                    w[4] = cmul(w[4], u4);
+                    w[5] = cmul(w[5], cmul(u1, u4));
+                    w[6] = cmul(w[6], cmul(u2, u4));
+                    w[7] = cmul(w[7], cmul(u3, u4));
+                }
+            }
+            dft_small<DIR>(w, L);
+#pragma unroll
+            for (int r = 0; r < L; ++r) v[b + r * NB] = w[r];
+        }
+    }
+}
+
+}  // namespace ocd

@@ -34,6 +34,29 @@ hipError_t launch_mask_apply_c128(const void* in, void* out, const ocd::MaskDev&
 hipError_t launch_check_finite(bool dbl, const void* in, size_t n, int* flag, hipStream_t st);
 hipError_t launch_copy(bool dbl, const void* in, void* out, size_t n, hipStream_t st);
 hipError_t launch_fill_int(int* p, int v, int n, hipStream_t st);
+hipError_t fast_launch_c64(const ocd::FastSel& s, const ocd::ProbDev* d, int np, int total, int grid, hipStream_t st);
+hipError_t fast_launch_c128(const ocd::FastSel& s, const ocd::ProbDev* d, int np, int total, int grid, hipStream_t st);
+void fast_layout_c64(int n, int v, int* G, int* B);
+void fast_layout_c128(int n, int v, int* G, int* B);
+hipError_t fast_set_lds_limits_c64();
+hipError_t fast_set_lds_limits_c128();
+
+// Number of layout variants of the power-of-two kernels at length n (0: none).
+int fast_variants(int n) {
+    switch (n) {
+        case 64: case 128: case 256: case 512: return 1;
+        case 1024: case 2048: case 4096: case 8192: return 2;
+        default: return 0;
+    }
+}
+
+// true when n has a specialised kernel; fills its workgroup layout.
+bool fast_supported(bool dbl, int n, int v, int* G, int* B) {
+    if (fast_variants(n) == 0) return false;
+    if (dbl) fast_layout_c128(n, v, G, B);
+    else fast_layout_c64(n, v, G, B);
+    return *G > 0;
+}
 hipError_t launch_seed(bool dbl, const void* u, const double* dmdi, void* out, size_t n, hipStream_t st);
 hipError_t launch_adj_elem(bool dbl, void* co, const void* ckpt, double* grad, const ocd::MaskDev& m, int nx, int ny,
                            hipStream_t st);
@@ -82,6 +105,15 @@ struct oc_context {
     std::vector<hipEvent_t> event_pool;
     double pass_ms[OC_NUM_PASS_KINDS] = {0};
     uint64_t pass_n[OC_NUM_PASS_KINDS] = {0};
+    bool force_generic = false;  // route every batch through the runtime-plan kernels
+    std::map<std::pair<int, int>, int> fft_variant;  // (n, double?) -> fast-kernel layout variant
+    int variant_for(int n, bool dbl) const {
+        auto it = fft_variant.find(std::make_pair(n, dbl ? 1 : 0));
+        // Untuned default: the 512-thread layout, which keeps every FP32 kernel
+        // free of register spills.
+        if (it == fft_variant.end()) return ocl::fast_variants(n) > 1 ? 1 : 0;
+        return it->second;
+    }
 };
 
 namespace {
@@ -551,6 +583,13 @@ Layout choose_layout(const FftPlanDev& prow, const FftPlanDev& pcol, size_t esz,
 
 struct Batch {
     bool dbl = false;
+    bool fast = false;  // power-of-two kernels (all problems share P_x = P_y = fast_n)
+    int fast_n = 0;
+    int fast_v = 0;
+    bool plain_post = true;  // no problem has a post-op: the lean row_inv kernel
+    int col_kind = -2;       // engine shared by every problem (-1: mixed)
+    int pre_kind = -2;       // pre-op shared by every problem (-1: mixed)
+    bool rs = false;
     std::vector<ProbDev> pd;
     std::vector<int> idx;  // index into the problem vector of each descriptor
     void* dprobs = nullptr;
@@ -596,6 +635,22 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         if (p.engine == ENG_RS) kbuf_elems += p.nyp * p.nxp;
         cand.push_back(i);
     }
+    // power-of-two fast path: one shared square padded size, uniform RS-ness
+    int fG = 0, fB = 0;
+    if (!cand.empty() && !ctx->force_generic) {
+        const size_t n0 = probs[cand[0]].nxp;
+        bool ok = n0 >= 32 && n0 <= 8192 && (n0 & (n0 - 1)) == 0;
+        const bool rs0 = probs[cand[0]].engine == ENG_RS;
+        for (int i : cand)
+            ok = ok && probs[i].nxp == n0 && probs[i].nyp == n0 && (probs[i].engine == ENG_RS) == rs0;
+        const int v = ctx->variant_for((int)n0, dbl);
+        if (ok && ocl::fast_supported(dbl, (int)n0, v, &fG, &fB)) {
+            b.fast = true;
+            b.fast_n = (int)n0;
+            b.fast_v = v;
+            b.rs = rs0;
+        }
+    }
     void *work = nullptr, *kbuf = nullptr;
     if (!cand.empty()) {
         int st = batch_alloc(ctx, b, persistent, "work", work_elems * esz, &work);
@@ -621,12 +676,22 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         st = make_plan(ctx, d.py, dbl, &d.pcol);
         if (st) return st;
         const bool rs = p.engine == ENG_RS;
-        Layout L = choose_layout(d.prow, d.pcol, esz, p.g.ny, p.nyp, rs);
+        Layout L;
+        if (b.fast) {
+            const size_t slots = (size_t)b.fast_n + b.fast_n / 16;
+            L.cnt = fG;
+            L.B = fB;
+            L.row_bytes = slots * esz;
+            L.col_bytes = slots * esz;
+            L.ok = true;
+        } else {
+            L = choose_layout(d.prow, d.pcol, esz, p.g.ny, p.nyp, rs);
+        }
         if (!L.ok) {
             p.status = fail(ctx, OC_ERR_UNSUPPORTED, "padded length exceeds the on-chip transform size");
             continue;
         }
-        const int tr = rs ? 2 : 1;
+        const int tr = (rs && !b.fast) ? 2 : 1;
         d.B = L.B;
         d.rows_cnt = L.cnt;
         d.tiles_row = (d.ny + L.cnt - 1) / L.cnt;
@@ -656,8 +721,8 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             if (st) return st;
         }
         auto rnd = [](int t) { return ((t + 63) / 64) * 64; };
-        const int brow = rnd(d.prow.threads * L.cnt);
-        const int bcol = rnd(d.pcol.threads * d.B * tr);
+        const int brow = b.fast ? (b.fast_n / 16) * L.cnt : rnd(d.prow.threads * L.cnt);
+        const int bcol = b.fast ? (b.fast_n / 16) * d.B : rnd(d.pcol.threads * d.B * tr);
         b.pc[0].block = std::max(b.pc[0].block, brow);
         b.pc[0].lds = std::max(b.pc[0].lds, L.row_bytes * L.cnt);
         b.pc[1].block = std::max(b.pc[1].block, bcol);
@@ -666,6 +731,11 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             b.pc[3].block = std::max(b.pc[3].block, brow);
             b.pc[3].lds = std::max(b.pc[3].lds, L.row_bytes * L.cnt);
         }
+        if (d.ops.post != POST_NONE) b.plain_post = false;
+        const int ek = (p.engine == ENG_ASM || p.engine == ENG_FRESNEL) ? p.engine : -1;
+        b.col_kind = b.col_kind == -2 ? ek : (b.col_kind == ek ? ek : -1);
+        const int pk = (d.ops.pre == PRE_NONE || d.ops.pre == PRE_MASK) ? d.ops.pre : -1;
+        b.pre_kind = b.pre_kind == -2 ? pk : (b.pre_kind == pk ? pk : -1);
         b.pd.push_back(d);
         b.idx.push_back(cand[a]);
     }
@@ -704,6 +774,12 @@ int launch_batch(oc_context* ctx, Batch& b, unsigned mask = kAllPasses) {
         const int grid = std::min(b.pc[pass].total, ctx->num_cu * 8);
         const PassCfg& c = b.pc[pass];
         hipError_t e = timed(ctx, pass, [&] {
+            if (b.fast) {
+                const ocd::FastSel sel{pass, b.fast_n, b.fast_v, b.rs ? 1 : 0, b.plain_post ? 1 : 0, b.col_kind,
+                                       b.pre_kind};
+                return b.dbl ? ocl::fast_launch_c128(sel, dprobs, nprob, c.total, grid, ctx->stream)
+                             : ocl::fast_launch_c64(sel, dprobs, nprob, c.total, grid, ctx->stream);
+            }
             return b.dbl ? ocl::launch_pass_c128(pass, dprobs, nprob, c.total, c.block, c.lds, grid, ctx->stream)
                          : ocl::launch_pass_c64(pass, dprobs, nprob, c.total, c.block, c.lds, grid, ctx->stream);
         });
@@ -874,6 +950,8 @@ int oc_context_create(int device, oc_context** out) {
     if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) return OC_ERR_DEVICE;
     c->stream = c->own;
     if (ocl::set_lds_limits() != hipSuccess) return OC_ERR_DEVICE;
+    if (ocl::fast_set_lds_limits_c64() != hipSuccess || ocl::fast_set_lds_limits_c128() != hipSuccess)
+        return OC_ERR_DEVICE;
     *out = c.release();
     return OC_OK;
 }
@@ -1231,7 +1309,6 @@ int gs_fft_impl(oc_context* ctx, bool dbl, uint64_t n, uint64_t R, const void* A
     if ((st = mem.alloc(nn * esz_r, &tblk))) return st;
     if ((st = mem.alloc(sizeof(double) * 4 * R, &sums))) return st;
     if ((st = mem.alloc(sizeof(int) * R, &flags))) return st;
-    HIP_TRY(ctx, ocl::launch_block_reorder(dbl, Tamp, tblk, (int)n, (int)n, L.B, ctx->stream));
     oc_band_geom g{n, n, 1.0, 1.0, 1.0};
     std::vector<Prob> it(R), me(R);
     for (uint64_t r = 0; r < R; ++r) {
@@ -1268,6 +1345,13 @@ int gs_fft_impl(oc_context* ctx, bool dbl, uint64_t n, uint64_t R, const void* A
         free_batch(ctx, bm);
     };
     if (bi.pd.size() != R || bm.pd.size() != R) { cleanup(); return it[0].status ? it[0].status : OC_ERR_UNSUPPORTED; }
+    // the target amplitude in the column pass's blocked order, with the block
+    // width of the kernels the batches were built for
+    if (bi.pd[0].B != bm.pd[0].B ||
+        ocl::launch_block_reorder(dbl, Tamp, tblk, (int)n, (int)n, bi.pd[0].B, ctx->stream) != hipSuccess) {
+        cleanup();
+        return fail(ctx, OC_ERR_DEVICE, "target reorder failed");
+    }
     const unsigned passes3 = (1u << 0) | (1u << 1) | (1u << 2);
     uint64_t done = 0;
     if (chunk > 0 && iterations >= chunk) {
@@ -1661,6 +1745,103 @@ int oc_adjoint_grid_gradient_dev_c128(oc_context* ctx, const oc_design_step* ste
     int st = adjoint_impl(ctx, steps, nsteps, *g, input, *cfg, nullptr, target, dphi_out, &m);
     if (merit_out) *merit_out = m;
     return st;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int oc_context_set_fft_variant(oc_context* ctx, uint64_t n, int dbl, int variant) {
+    if (!ctx || variant < 0 || variant >= std::max(1, ocl::fast_variants((int)n))) return OC_ERR_INVALID_ARGUMENT;
+    ctx->fft_variant[std::make_pair((int)n, dbl ? 1 : 0)] = variant;
+    return OC_OK;
+}
+
+int oc_context_fft_variant(oc_context* ctx, uint64_t n, int dbl) {
+    if (!ctx || ocl::fast_variants((int)n) == 0) return -1;
+    return ctx->variant_for((int)n, dbl != 0);
+}
+
+int oc_context_force_generic(oc_context* ctx, int enable) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    ctx->force_generic = enable != 0;
+    return OC_OK;
+}
+
+// Times every layout variant of the specialised kernels at padded length n on
+// a synthetic batch of masked ASM propagations (field n/2, padding 2) and
+// keeps the fastest.
+int oc_autotune_fft(oc_context* ctx, uint64_t n, int dbl, int* variant_out, double* ms_out) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    const int nv = ocl::fast_variants((int)n);
+    if (ms_out) ms_out[0] = ms_out[1] = -1.0;
+    if (nv == 0) {
+        if (variant_out) *variant_out = -1;
+        return OC_OK;
+    }
+    const size_t half = n / 2, esz = dbl ? 16 : 8;
+    const size_t per = half * half;
+    const size_t nb = std::max<size_t>(1, std::min<size_t>(16, ((size_t)1 << 26) / (per * 4)));
+    Scoped mem(ctx);
+    void *in, *out, *mv;
+    int st;
+    if ((st = mem.alloc(nb * per * esz, &in)) || (st = mem.alloc(nb * per * esz, &out)) ||
+        (st = mem.alloc(per * 8, &mv)))
+        return st;
+    HIP_TRY(ctx, hipMemsetAsync(in, 0, nb * per * esz, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(mv, 0, per * 8, ctx->stream));
+    oc_phase_mask m;
+    std::memset(&m, 0, sizeof m);
+    m.values = static_cast<const double*>(mv);
+    m.nx = m.ny = half;
+    m.dx = m.dy = 2e-6;
+    oc_band_geom g{half, half, 2e-6, 2e-6, 550e-9};
+    std::vector<Prob> probs;
+    for (size_t b = 0; b < nb; ++b)
+        probs.push_back(make_prob(&g, 5e-3, ENG_ASM, 1, 0, 2.0, 2.0, 1.0, &m,
+                                  static_cast<unsigned char*>(in) + b * per * esz,
+                                  static_cast<unsigned char*>(out) + b * per * esz));
+    const auto key = std::make_pair((int)n, dbl ? 1 : 0);
+    const bool had = ctx->fft_variant.count(key) != 0;
+    const int prev = had ? ctx->fft_variant[key] : 0;
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    int best = 0;
+    double best_ms = 1e300;
+    for (int v = 0; v < nv; ++v) {
+        ctx->fft_variant[key] = v;
+        double ms = 1e300;
+        for (int rep = 0; rep < 4; ++rep) {
+            std::vector<Prob> pv = probs;
+            hipEvent_t a = pool_event(ctx), e = pool_event(ctx);
+            (void)hipEventRecord(a, ctx->stream);
+            st = run_batch(ctx, dbl, pv);
+            (void)hipEventRecord(e, ctx->stream);
+            (void)hipEventSynchronize(e);
+            float f = 0.f;
+            (void)hipEventElapsedTime(&f, a, e);
+            ctx->event_pool.push_back(a);
+            ctx->event_pool.push_back(e);
+            if (st) break;
+            if (rep > 0) ms = std::min(ms, (double)f);
+        }
+        if (st) break;
+        if (ms_out) ms_out[v] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    ctx->timing = timing;
+    if (st) {
+        if (had) ctx->fft_variant[key] = prev;
+        else ctx->fft_variant.erase(key);
+        return st;
+    }
+    ctx->fft_variant[key] = best;
+    if (variant_out) *variant_out = best;
+    return OC_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,356 @@
+// Device-side element-wise operations shared by the generic and the
+// power-of-two pass kernels: phase-mask transmission, transfer functions,
+// the RS kernel sample, fused pre/post ops, accumulators and tile lookup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "prop_kernels.hpp"
+
+namespace ocd {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr double kTwoPiHi = 6.28318530717958623199592693708837032;   // nearest double to 2 pi
+constexpr double kTwoPiLo = 2.44929359829470635445213186455000e-16;  // 2 pi - kTwoPiHi
+constexpr double kInvTwoPi = 0.159154943091895335768883763372514362;
+
+// sin/cos of a phase argument.  Argument reduction modulo pi/2 is done in
+// FP64 by Cody-Waite with a three-part pi/2 (the product n*C1 and n*C2 are
+// exact for |n| < 2^31, so phases up to ~3e9 rad reduce to ~1e-16 absolute);
+// the reduced |x| <= pi/4 goes through Taylor polynomials: degree 17/18 in
+// FP64 (truncation < 5e-17) for the complex128 path, degree 9/8 in FP32
+// (truncation < 4e-7 absolute) for the FP32-bulk path.  The quadrant n mod 4
+// then rotates (sin, cos).
+constexpr double kPio2_1 = 1.57079632673412561416626;  // pi/2 rounded to 33 bits
+constexpr double kPio2_2 = 6.077100506303965976595549e-11;  // next 33 bits
+constexpr double kPio2_3 = 2.022266248795950631541144e-21;  // remainder (53 bits)
+constexpr double kInvPio2 = 6.36619772367581382433e-01;
+
+__device__ __forceinline__ double reduce_pio2(double th, int* quad) {
+    const double n = rint(th * kInvPio2);
+    double r = fma(-n, kPio2_1, th);
+    r = fma(-n, kPio2_2, r);
+    r = fma(-n, kPio2_3, r);
+    *quad = (int)(long long)n & 3;
+    return r;
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> quadrant(R s, R c, int q) {
+    switch (q) {
+        case 0: return {c, s};
+        case 1: return {-s, c};
+        case 2: return {-c, -s};
+        default: return {s, -c};
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> unit_phase(double th);
+
+// complex128 path: (cos th, sin th) to ~1 ulp.
+template <>
+__device__ __forceinline__ cpx<double> unit_phase<double>(double th) {
+    int q;
+    const double x = reduce_pio2(th, &q);
+    const double x2 = x * x;
+    // sin x = x + x^3 P(x^2),  P(u) = -1/3! + u/5! - u^2/7! + ... + u^7/17!
+    double ps = 2.81145725434552076319e-15;         // 1/17!
+    ps = fma(ps, x2, -7.64716373181981647590e-13);  // -1/15!
+    ps = fma(ps, x2, 1.60590438368216145994e-10);   // 1/13!
+    ps = fma(ps, x2, -2.50521083854417187751e-08);  // -1/11!
+    ps = fma(ps, x2, 2.75573192239858906526e-06);   // 1/9!
+    ps = fma(ps, x2, -1.98412698412698412698e-04);  // -1/7!
+    ps = fma(ps, x2, 8.33333333333333333333e-03);   // 1/5!
+    ps = fma(ps, x2, -1.66666666666666666667e-01);  // -1/3!
+    const double s = fma(x * x2, ps, x);
+    // cos x = 1 + x^2 Q(x^2),  Q(u) = -1/2 + u/4! - u^2/6! + ... - u^8/18!
+    double pc = -1.56192069685862264622e-16;        // -1/18!
+    pc = fma(pc, x2, 4.77947733238738529744e-14);   // 1/16!
+    pc = fma(pc, x2, -1.14707455977297247139e-11);  // -1/14!
+    pc = fma(pc, x2, 2.08767569878680989792e-09);   // 1/12!
+    pc = fma(pc, x2, -2.75573192239858906526e-07);  // -1/10!
+    pc = fma(pc, x2, 2.48015873015873015873e-05);   // 1/8!
+    pc = fma(pc, x2, -1.38888888888888888889e-03);  // -1/6!
+    pc = fma(pc, x2, 4.16666666666666666667e-02);   // 1/4!
+    pc = fma(pc, x2, -0.5);
+    const double c = fma(x2, pc, 1.0);
+    return quadrant<double>(s, c, q);
+}
+
+// FP32-bulk path: FP64 argument and reduction, FP32 polynomials.
+template <>
+__device__ __forceinline__ cpx<float> unit_phase<float>(double th) {
+    int q;
+    const float x = (float)reduce_pio2(th, &q);
+    const float x2 = x * x;
+    // Taylor terms through x^11 (sin) and x^10 (cos): truncation below
+    // 1.2e-10 on |x| <= pi/4, so only FP32 rounding remains.
+    float ps = -2.50521084e-08f;            // -1/11!
+    ps = fmaf(ps, x2, 2.75573192e-06f);     // 1/9!
+    ps = fmaf(ps, x2, -1.98412698e-04f);    // -1/7!
+    ps = fmaf(ps, x2, 8.33333333e-03f);     // 1/5!
+    ps = fmaf(ps, x2, -1.66666667e-01f);    // -1/3!
+    const float s = fmaf(x * x2, ps, x);
+    float pc = -2.75573192e-07f;            // -1/10!
+    pc = fmaf(pc, x2, 2.48015873e-05f);     // 1/8!
+    pc = fmaf(pc, x2, -1.38888889e-03f);    // -1/6!
+    pc = fmaf(pc, x2, 4.16666667e-02f);     // 1/4!
+    pc = fmaf(pc, x2, -0.5f);
+    const float c = fmaf(x2, pc, 1.0f);
+    return quadrant<float>(s, c, q);
+}
+
+template <typename R>
+__device__ __forceinline__ bool finite_c(cpx<R> v) {
+    return isfinite(v.re) && isfinite(v.im);
+}
+
+/* ------------------------------------------------------------------ */
+/* Phase-mask transmission (src/element.rs:1189-1207, 1443-1484).       */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ bool ap_contains(const MaskDev& m, double x, double y) {
+    switch (m.ap_kind) {
+        case 0: return true;
+        case 1: return x * x + y * y <= m.ap_a * m.ap_a;
+        case 2: {
+            const double a2 = m.ap_a * m.ap_a, b2 = m.ap_b * m.ap_b;
+            return b2 * x * x + a2 * y * y <= a2 * b2;
+        }
+        default: return fabs(x) <= m.ap_a && fabs(y) <= m.ap_b;
+    }
+}
+
+__device__ __forceinline__ double bilinear(const double* __restrict__ v, int mnx, int i0, int i1, int j0, int j1,
+                                           double tx, double ty) {
+    const gmem<const double>* g = gp(v);
+    const double v00 = g[(size_t)j0 * mnx + i0], v10 = g[(size_t)j0 * mnx + i1];
+    const double v01 = g[(size_t)j1 * mnx + i0], v11 = g[(size_t)j1 * mnx + i1];
+    const double a = v00 + (v10 - v00) * tx;
+    const double b = v01 + (v11 - v01) * tx;
+    return a + (b - a) * ty;
+}
+
+// Returns false when the sample is outside the stop (transmission 0);
+// otherwise *phi is the lattice phase (0 beyond the lattice span).
+__device__ __forceinline__ bool mask_phase(const MaskDev& m, int ix, int iy, double* phi) {
+    if (m.mode == MASK_SEPARABLE) {
+        if (!ap_contains(m, gp(m.xr)[ix], gp(m.yr)[iy])) return false;
+        if (gp(m.inx)[ix] && gp(m.iny)[iy])
+            *phi = bilinear(m.values, m.mnx, gp(m.i0x)[ix], gp(m.i1x)[ix], gp(m.j0y)[iy], gp(m.j1y)[iy],
+                            gp(m.tx)[ix], gp(m.ty)[iy]);
+        else
+            *phi = 0.0;
+        return true;
+    }
+    const double x = ((double)ix - ((double)m.fnx - 1.0) / 2.0) * m.fdx;
+    const double y = ((double)iy - ((double)m.fny - 1.0) / 2.0) * m.fdy;
+    const double xr = x - m.cx, yr = y - m.cy;
+    const double fx = m.cpsi * xr + m.spsi * yr;
+    const double fy = -m.spsi * xr + m.cpsi * yr;
+    if (!ap_contains(m, fx, fy)) return false;
+    const double gx = fx / m.mdx + ((double)m.mnx - 1.0) / 2.0;
+    const double gy = fy / m.mdy + ((double)m.mny - 1.0) / 2.0;
+    if (gx < 0.0 || gx > (double)m.mnx - 1.0 || gy < 0.0 || gy > (double)m.mny - 1.0) {
+        *phi = 0.0;
+        return true;
+    }
+    const int i0 = (int)floor(gx), j0 = (int)floor(gy);
+    const int i1 = min(i0 + 1, m.mnx - 1), j1 = min(j0 + 1, m.mny - 1);
+    *phi = bilinear(m.values, m.mnx, i0, i1, j0, j1, gx - (double)i0, gy - (double)j0);
+    return true;
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> apply_mask(const MaskDev& m, int ix, int iy, cpx<R> s) {
+    double phi;
+    if (!mask_phase(m, ix, iy, &phi)) return {R(0), R(0)};
+    return cmul(s, unit_phase<R>(phi));
+}
+
+/* ------------------------------------------------------------------ */
+/* Transfer functions (asm.rs:273-281, fresnel.rs:258-262).             */
+/* ------------------------------------------------------------------ */
+// EK: the engine when known at compile time (ENG_ASM / ENG_FRESNEL), else -1
+// and the descriptor's kind decides.
+template <typename R, int EK = -1>
+__device__ __forceinline__ cpx<R> transfer(const EngineDev& e, int kx, int ky, cpx<R> h) {
+    const gmem<const double>* ax = gp(e.ax);
+    const gmem<const double>* ay = gp(e.ay);
+    if (EK == ENG_ASM || (EK < 0 && e.kind == ENG_ASM)) {
+        if (gp(e.mx)[kx] | gp(e.my)[ky]) return {R(0), R(0)};
+        const double s = (1.0 - ax[kx]) - ay[ky];
+        if (s < 0.0) return {R(0), R(0)};
+        return cmul(h, unit_phase<R>(e.kz * sqrt(s)));
+    }
+    // Fresnel TF: phase = k z - (pi lambda z) (fx^2 + fy^2)
+    return cmul(h, unit_phase<R>(e.kz - e.c * (ax[kx] + ay[ky])));
+}
+
+// rayleigh_sommerfeld.rs:228-235 sampled at DFT-order coordinates, times dx dy.
+template <typename R>
+__device__ __forceinline__ cpx<R> rs_sample(const EngineDev& e, int ix, int iy, int px, int py) {
+    const double x = (ix <= px / 2) ? (double)ix * e.dx : ((double)ix - (double)px) * e.dx;
+    const double y = (iy <= py / 2) ? (double)iy * e.dy : ((double)iy - (double)py) * e.dy;
+    const double r = sqrt(x * x + y * y + e.z * e.z);
+    const double amp = fabs(e.z) / (2.0 * kPi * r * r);
+    const double ore = 1.0 / r, oim = -e.sgn * e.k;
+    const double are = amp * ore, aim = amp * oim;
+    const double th = e.sgn * e.k * r;
+    const double dxdy = e.dx * e.dy;
+    if constexpr (sizeof(R) == 8) {
+        const cpx<double> u = unit_phase<double>(th);
+        const double vre = are * u.re - aim * u.im, vim = are * u.im + aim * u.re;
+        return {vre * dxdy, vim * dxdy};
+    } else {
+        const cpx<float> u = unit_phase<float>(th);
+        const double vre = are * (double)u.re - aim * (double)u.im;
+        const double vim = are * (double)u.im + aim * (double)u.re;
+        return {(float)(vre * dxdy), (float)(vim * dxdy)};
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Fused element-wise ops.                                              */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ double rem_euclid(double x, double m) {
+    const double r = fmod(x, m);
+    return r < 0.0 ? r + fabs(m) : r;
+}
+__device__ __forceinline__ double wrap_pi(double v) {  // gs.rs:108-110
+    return rem_euclid(v + kPi, 2.0 * kPi) - kPi;
+}
+__device__ __forceinline__ double encode_bounds(double phi, double lo, double hi) {  // gs.rs:115-121
+    if (hi - lo >= 2.0 * kPi) return lo + rem_euclid(phi - lo, 2.0 * kPi);
+    return phi < lo ? lo : (phi > hi ? hi : phi);
+}
+
+// Sums a[0..3] over the workgroup's waves into sums[0..3] (one atomic per wave
+// and slot).  Every thread of the block must call it.
+__device__ __forceinline__ void accumulate4(double a0, double a1, double a2, double a3, double* sums) {
+    double v[4] = {a0, a1, a2, a3};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_down(v[i], off, 64);
+    }
+    if ((threadIdx.x & 63) == 0 && sums) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (v[i] != 0.0) atomicAdd(sums + i, v[i]);
+    }
+}
+
+// First-pass load of input sample (ix, iy) with the problem's pre-op.
+// PK: the pre-op when known at compile time (PRE_NONE / PRE_MASK), else -1
+// and the descriptor's op decides.
+template <typename R, int PK = -1>
+__device__ __forceinline__ cpx<R> load_pre(const ProbDev& P, int ix, int iy, bool& bad, double& acc) {
+    const size_t k = (size_t)iy * P.nx + ix;
+    const OpsDev& o = P.ops;
+    const int pre = PK >= 0 ? PK : o.pre;
+    if (pre == PRE_POLAR) {  // Complex::from_polar(A, phi) masked by the stop (gs.rs:484-492)
+        if (o.open && !gp(o.open)[k]) return {R(0), R(0)};
+        const R a = gp(static_cast<const R*>(o.amp))[k];
+        const R ph = gp(static_cast<const R*>(o.phase))[k];
+        const cpx<R> e = unit_phase<R>((double)ph);
+        return {a * e.re, a * e.im};
+    }
+    cpx<R> v = gp(static_cast<const cpx<R>*>(P.in))[k];
+    bad |= !finite_c(v);
+    if (pre == PRE_MASK) {
+        v = apply_mask(P.mask, ix, iy, v);
+    } else if (pre == PRE_SEED_MSE) {  // w = (dM/dI) U, dM/dI = 2 (I - T)/n (merit.rs:143-165)
+        const double re = v.re, im = v.im;
+        const double d = (re * re + im * im) - gp(o.seed_target)[k];
+        acc += d * d;
+        const double g = 2.0 * d / o.n_frame;
+        v = {(R)(g * re), (R)(g * im)};
+    }
+    return v;
+}
+
+// Last-pass store of output sample (ix, iy) with the problem's post-op.
+template <typename R>
+__device__ __forceinline__ void store_post(const ProbDev& P, int ix, int iy, cpx<R> o, double& acc0, double& acc1,
+                                           double& acc2) {
+    const size_t k = (size_t)iy * P.nx + ix;
+    cpx<R>* out = static_cast<cpx<R>*>(P.out);
+    const OpsDev& op = P.ops;
+    switch (op.post) {
+        case POST_GS_TARGET: {  // gs.rs:494-551
+            const double re = o.re, im = o.im;
+            const double a_t = hypot(re, im);
+            const double inten = a_t * a_t;
+            const double e = a_t - op.tamp[k];
+            acc0 += e * e;
+            const double dm = inten - op.merit_target[k];
+            acc1 += dm * dm;
+            if (op.err_out) op.err_out[k] = e;
+            const double a = (1.0 - op.beta_f) * a_t + op.beta_f * op.tamp[k];
+            const double m = hypot(re, im);
+            cpx<R> r;
+            if (m > 0.0) r = {(R)(re * (a / m)), (R)(im * (a / m))};
+            else r = {(R)a, R(0)};
+            out[k] = r;
+            return;
+        }
+        case POST_GS_PHASE: {  // gs.rs:557-567
+            R* ph = static_cast<R*>(op.phase_io);
+            const double p = ph[k];
+            const double dphi = wrap_pi(atan2((double)o.im, (double)o.re) - p);
+            ph[k] = (R)encode_bounds(p + op.beta_p * dphi, op.lo, op.hi);
+            acc2 += dphi * dphi;
+            if (op.dphi_out) op.dphi_out[k] = dphi;
+            if (out) out[k] = o;
+            return;
+        }
+        case POST_ADJ_GRAD: {  // adjoint.rs:222-236
+            const cpx<R> t = apply_mask(P.mask, ix, iy, cpx<R>{R(1), R(0)});
+            const cpx<R> v = apply_mask(P.mask, ix, iy, static_cast<const cpx<R>*>(op.ckpt)[k]);
+            const double mre = o.re, mim = o.im;
+            // -2 Im(conj(mu) V), conj(mu) = (mre, -mim)
+            const double gim = mre * (double)v.im + (-mim) * (double)v.re;
+            op.grad[k] += -2.0 * gim;
+            // co *= conj(t): re = mu.re*t.re - mu.im*(-t.im); im = mu.im*t.re + mu.re*(-t.im)
+            const R tr = t.re, ti = -t.im;
+            cpx<R> c;
+            c.re = o.re * tr - o.im * ti;
+            c.im = o.im * tr + o.re * ti;
+            out[k] = c;
+            return;
+        }
+        default:
+            out[k] = o;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Tile lookup.                                                         */
+/* ------------------------------------------------------------------ */
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup b runs on
+// XCD b % 8), each XCD behind its own L2.  Renumbering them so that the
+// workgroups of one XCD take consecutive tiles keeps neighbouring rows --
+// which share cache lines of the blocked intermediate -- in one L2.
+__device__ __forceinline__ int xcd_block() {
+    const int g = (int)gridDim.x, b = (int)blockIdx.x;
+    if (g % 8) return b;
+    return (b % 8) * (g / 8) + b / 8;
+}
+enum PassKind : int { PASS_ROW = 0, PASS_COL = 1, PASS_KGEN = 2 };
+
+template <int PASS>
+__device__ __forceinline__ int find_problem(const ProbDev* __restrict__ probs, int nprob, int tile) {
+    int lo = 0, hi = nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int b = PASS == PASS_ROW ? probs[mid].base_row : (PASS == PASS_COL ? probs[mid].base_col
+                                                                                 : probs[mid].base_kgen);
+        if (b <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace ocd

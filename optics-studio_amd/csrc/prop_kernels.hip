@@ -18,277 +18,9 @@
 
 #include "prop_kernels.hpp"
 
+#include "pass_ops.hpp"
+
 namespace ocd {
-
-constexpr double kPi = 3.14159265358979323846264338327950288;
-constexpr double kTwoPiHi = 6.28318530717958623199592693708837032;   // nearest double to 2 pi
-constexpr double kTwoPiLo = 2.44929359829470635445213186455000e-16;  // 2 pi - kTwoPiHi
-constexpr double kInvTwoPi = 0.159154943091895335768883763372514362;
-
-template <typename R>
-__device__ __forceinline__ cpx<R> unit_phase(double th);
-
-// complex128 path: libm-accuracy sincos of the full f64 argument.
-template <>
-__device__ __forceinline__ cpx<double> unit_phase<double>(double th) {
-    double s, c;
-    sincos(th, &s, &c);
-    return {c, s};
-}
-
-// FP32-bulk path: the phase argument is formed and reduced modulo 2 pi in
-// FP64 (Cody-Waite with a two-part 2 pi), only the bounded remainder goes
-// through FP32 sincos.
-template <>
-__device__ __forceinline__ cpx<float> unit_phase<float>(double th) {
-    const double nq = rint(th * kInvTwoPi);
-    double r = fma(-nq, kTwoPiHi, th);
-    r = fma(-nq, kTwoPiLo, r);
-    float s, c;
-    sincosf(static_cast<float>(r), &s, &c);
-    return {c, s};
-}
-
-template <typename R>
-__device__ __forceinline__ bool finite_c(cpx<R> v) {
-    return isfinite(v.re) && isfinite(v.im);
-}
-
-/* ------------------------------------------------------------------ */
-/* Phase-mask transmission (src/element.rs:1189-1207, 1443-1484).       */
-/* ------------------------------------------------------------------ */
-__device__ __forceinline__ bool ap_contains(const MaskDev& m, double x, double y) {
-    switch (m.ap_kind) {
-        case 0: return true;
-        case 1: return x * x + y * y <= m.ap_a * m.ap_a;
-        case 2: {
-            const double a2 = m.ap_a * m.ap_a, b2 = m.ap_b * m.ap_b;
-            return b2 * x * x + a2 * y * y <= a2 * b2;
-        }
-        default: return fabs(x) <= m.ap_a && fabs(y) <= m.ap_b;
-    }
-}
-
-__device__ __forceinline__ double bilinear(const double* __restrict__ v, int mnx, int i0, int i1, int j0, int j1,
-                                           double tx, double ty) {
-    const double v00 = v[(size_t)j0 * mnx + i0], v10 = v[(size_t)j0 * mnx + i1];
-    const double v01 = v[(size_t)j1 * mnx + i0], v11 = v[(size_t)j1 * mnx + i1];
-    const double a = v00 + (v10 - v00) * tx;
-    const double b = v01 + (v11 - v01) * tx;
-    return a + (b - a) * ty;
-}
-
-// Returns false when the sample is outside the stop (transmission 0);
-// otherwise *phi is the lattice phase (0 beyond the lattice span).
-__device__ __forceinline__ bool mask_phase(const MaskDev& m, int ix, int iy, double* phi) {
-    if (m.mode == MASK_SEPARABLE) {
-        if (!ap_contains(m, m.xr[ix], m.yr[iy])) return false;
-        if (m.inx[ix] && m.iny[iy])
-            *phi = bilinear(m.values, m.mnx, m.i0x[ix], m.i1x[ix], m.j0y[iy], m.j1y[iy], m.tx[ix], m.ty[iy]);
-        else
-            *phi = 0.0;
-        return true;
-    }
-    const double x = ((double)ix - ((double)m.fnx - 1.0) / 2.0) * m.fdx;
-    const double y = ((double)iy - ((double)m.fny - 1.0) / 2.0) * m.fdy;
-    const double xr = x - m.cx, yr = y - m.cy;
-    const double fx = m.cpsi * xr + m.spsi * yr;
-    const double fy = -m.spsi * xr + m.cpsi * yr;
-    if (!ap_contains(m, fx, fy)) return false;
-    const double gx = fx / m.mdx + ((double)m.mnx - 1.0) / 2.0;
-    const double gy = fy / m.mdy + ((double)m.mny - 1.0) / 2.0;
-    if (gx < 0.0 || gx > (double)m.mnx - 1.0 || gy < 0.0 || gy > (double)m.mny - 1.0) {
-        *phi = 0.0;
-        return true;
-    }
-    const int i0 = (int)floor(gx), j0 = (int)floor(gy);
-    const int i1 = min(i0 + 1, m.mnx - 1), j1 = min(j0 + 1, m.mny - 1);
-    *phi = bilinear(m.values, m.mnx, i0, i1, j0, j1, gx - (double)i0, gy - (double)j0);
-    return true;
-}
-
-template <typename R>
-__device__ __forceinline__ cpx<R> apply_mask(const MaskDev& m, int ix, int iy, cpx<R> s) {
-    double phi;
-    if (!mask_phase(m, ix, iy, &phi)) return {R(0), R(0)};
-    if constexpr (sizeof(R) == 8) {
-        double sn, cs;
-        sincos(phi, &sn, &cs);
-        return cmul(s, cpx<R>{cs, sn});
-    } else {
-        return cmul(s, unit_phase<float>(phi));
-    }
-}
-
-/* ------------------------------------------------------------------ */
-/* Transfer functions (asm.rs:273-281, fresnel.rs:258-262).             */
-/* ------------------------------------------------------------------ */
-template <typename R>
-__device__ __forceinline__ cpx<R> transfer(const EngineDev& e, int kx, int ky, cpx<R> h) {
-    if (e.kind == ENG_ASM) {
-        if (e.mx[kx] | e.my[ky]) return {R(0), R(0)};
-        const double s = (1.0 - e.ax[kx]) - e.ay[ky];
-        if (s < 0.0) return {R(0), R(0)};
-        return cmul(h, unit_phase<R>(e.kz * sqrt(s)));
-    }
-    // Fresnel TF: phase = k z - (pi lambda z) (fx^2 + fy^2)
-    return cmul(h, unit_phase<R>(e.kz - e.c * (e.ax[kx] + e.ay[ky])));
-}
-
-// rayleigh_sommerfeld.rs:228-235 sampled at DFT-order coordinates, times dx dy.
-template <typename R>
-__device__ __forceinline__ cpx<R> rs_sample(const EngineDev& e, int ix, int iy, int px, int py) {
-    const double x = (ix <= px / 2) ? (double)ix * e.dx : ((double)ix - (double)px) * e.dx;
-    const double y = (iy <= py / 2) ? (double)iy * e.dy : ((double)iy - (double)py) * e.dy;
-    const double r = sqrt(x * x + y * y + e.z * e.z);
-    const double amp = fabs(e.z) / (2.0 * kPi * r * r);
-    const double ore = 1.0 / r, oim = -e.sgn * e.k;
-    const double are = amp * ore, aim = amp * oim;
-    const double th = e.sgn * e.k * r;
-    const double dxdy = e.dx * e.dy;
-    if constexpr (sizeof(R) == 8) {
-        double sn, cs;
-        sincos(th, &sn, &cs);
-        const double vre = are * cs - aim * sn, vim = are * sn + aim * cs;
-        return {vre * dxdy, vim * dxdy};
-    } else {
-        const cpx<float> u = unit_phase<float>(th);
-        const double vre = are * (double)u.re - aim * (double)u.im;
-        const double vim = are * (double)u.im + aim * (double)u.re;
-        return {(float)(vre * dxdy), (float)(vim * dxdy)};
-    }
-}
-
-/* ------------------------------------------------------------------ */
-/* Fused element-wise ops.                                              */
-/* ------------------------------------------------------------------ */
-__device__ __forceinline__ double rem_euclid(double x, double m) {
-    const double r = fmod(x, m);
-    return r < 0.0 ? r + fabs(m) : r;
-}
-__device__ __forceinline__ double wrap_pi(double v) {  // gs.rs:108-110
-    return rem_euclid(v + kPi, 2.0 * kPi) - kPi;
-}
-__device__ __forceinline__ double encode_bounds(double phi, double lo, double hi) {  // gs.rs:115-121
-    if (hi - lo >= 2.0 * kPi) return lo + rem_euclid(phi - lo, 2.0 * kPi);
-    return phi < lo ? lo : (phi > hi ? hi : phi);
-}
-
-// Sums a[0..3] over the workgroup's waves into sums[0..3] (one atomic per wave
-// and slot).  Every thread of the block must call it.
-__device__ __forceinline__ void accumulate4(double a0, double a1, double a2, double a3, double* sums) {
-    double v[4] = {a0, a1, a2, a3};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_down(v[i], off, 64);
-    }
-    if ((threadIdx.x & 63) == 0 && sums) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (v[i] != 0.0) atomicAdd(sums + i, v[i]);
-    }
-}
-
-// First-pass load of input sample (ix, iy) with the problem's pre-op.
-template <typename R>
-__device__ __forceinline__ cpx<R> load_pre(const ProbDev& P, int ix, int iy, bool& bad, double& acc) {
-    const size_t k = (size_t)iy * P.nx + ix;
-    const OpsDev& o = P.ops;
-    if (o.pre == PRE_POLAR) {  // Complex::from_polar(A, phi) masked by the stop (gs.rs:484-492)
-        if (o.open && !o.open[k]) return {R(0), R(0)};
-        const R a = static_cast<const R*>(o.amp)[k];
-        const R ph = static_cast<const R*>(o.phase)[k];
-        R sn, cs;
-        if constexpr (sizeof(R) == 8) sincos(ph, &sn, &cs);
-        else sincosf(ph, &sn, &cs);
-        return {a * cs, a * sn};
-    }
-    cpx<R> v = static_cast<const cpx<R>*>(P.in)[k];
-    bad |= !finite_c(v);
-    if (o.pre == PRE_MASK) {
-        v = apply_mask(P.mask, ix, iy, v);
-    } else if (o.pre == PRE_SEED_MSE) {  // w = (dM/dI) U, dM/dI = 2 (I - T)/n (merit.rs:143-165)
-        const double re = v.re, im = v.im;
-        const double d = (re * re + im * im) - o.seed_target[k];
-        acc += d * d;
-        const double g = 2.0 * d / o.n_frame;
-        v = {(R)(g * re), (R)(g * im)};
-    }
-    return v;
-}
-
-// Last-pass store of output sample (ix, iy) with the problem's post-op.
-template <typename R>
-__device__ __forceinline__ void store_post(const ProbDev& P, int ix, int iy, cpx<R> o, double* acc) {
-    const size_t k = (size_t)iy * P.nx + ix;
-    cpx<R>* out = static_cast<cpx<R>*>(P.out);
-    const OpsDev& op = P.ops;
-    switch (op.post) {
-        case POST_GS_TARGET: {  // gs.rs:494-551
-            const double re = o.re, im = o.im;
-            const double a_t = hypot(re, im);
-            const double inten = a_t * a_t;
-            const double e = a_t - op.tamp[k];
-            acc[0] += e * e;
-            const double dm = inten - op.merit_target[k];
-            acc[1] += dm * dm;
-            if (op.err_out) op.err_out[k] = e;
-            const double a = (1.0 - op.beta_f) * a_t + op.beta_f * op.tamp[k];
-            const double m = hypot(re, im);
-            cpx<R> r;
-            if (m > 0.0) r = {(R)(re * (a / m)), (R)(im * (a / m))};
-            else r = {(R)a, R(0)};
-            out[k] = r;
-            return;
-        }
-        case POST_GS_PHASE: {  // gs.rs:557-567
-            R* ph = static_cast<R*>(op.phase_io);
-            const double p = ph[k];
-            const double dphi = wrap_pi(atan2((double)o.im, (double)o.re) - p);
-            ph[k] = (R)encode_bounds(p + op.beta_p * dphi, op.lo, op.hi);
-            acc[2] += dphi * dphi;
-            if (op.dphi_out) op.dphi_out[k] = dphi;
-            if (out) out[k] = o;
-            return;
-        }
-        case POST_ADJ_GRAD: {  // adjoint.rs:222-236
-            const cpx<R> t = apply_mask(P.mask, ix, iy, cpx<R>{R(1), R(0)});
-            const cpx<R> v = apply_mask(P.mask, ix, iy, static_cast<const cpx<R>*>(op.ckpt)[k]);
-            const double mre = o.re, mim = o.im;
-            // -2 Im(conj(mu) V), conj(mu) = (mre, -mim)
-            const double gim = mre * (double)v.im + (-mim) * (double)v.re;
-            op.grad[k] += -2.0 * gim;
-            // co *= conj(t): re = mu.re*t.re - mu.im*(-t.im); im = mu.im*t.re + mu.re*(-t.im)
-            const R tr = t.re, ti = -t.im;
-            cpx<R> c;
-            c.re = o.re * tr - o.im * ti;
-            c.im = o.im * tr + o.re * ti;
-            out[k] = c;
-            return;
-        }
-        default:
-            out[k] = o;
-    }
-}
-
-/* ------------------------------------------------------------------ */
-/* Tile lookup.                                                         */
-/* ------------------------------------------------------------------ */
-enum PassKind : int { PASS_ROW = 0, PASS_COL = 1, PASS_KGEN = 2 };
-
-template <int PASS>
-__device__ __forceinline__ int find_problem(const ProbDev* __restrict__ probs, int nprob, int tile) {
-    int lo = 0, hi = nprob - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        const int b = PASS == PASS_ROW ? probs[mid].base_row : (PASS == PASS_COL ? probs[mid].base_col
-                                                                                 : probs[mid].base_kgen);
-        if (b <= tile) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 __device__ __forceinline__ int fft_lds_stride(const FftPlanDev& pl) {
     return lds_stride(pl.n) * (pl.generic ? 2 : 1);
@@ -302,7 +34,7 @@ __global__ void k_row_fwd(const ProbDev* __restrict__ probs, int nprob, int tota
     extern __shared__ unsigned char smem_raw[];
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x, bdim = blockDim.x;
-    for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
         const int iy0 = (tile - P.base_row) * P.rows_cnt;
         const int cnt = P.rows_cnt, px = P.px, nx = P.nx, ny = P.ny, ox = P.ox;
@@ -344,7 +76,7 @@ __global__ void k_col(const ProbDev* __restrict__ probs, int nprob, int total) {
     extern __shared__ unsigned char smem_raw[];
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x, bdim = blockDim.x;
-    for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
         const int q = tile - P.base_col;
         const int B = P.B, py = P.py, ny = P.ny, oy = P.oy;
@@ -428,7 +160,7 @@ __global__ void k_row_inv(const ProbDev* __restrict__ probs, int nprob, int tota
     extern __shared__ unsigned char smem_raw[];
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x, bdim = blockDim.x;
-    for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
         const int iy0 = (tile - P.base_row) * P.rows_cnt;
         const int cnt = P.rows_cnt, px = P.px, nx = P.nx, ny = P.ny, ox = P.ox, B = P.B;
@@ -450,19 +182,19 @@ __global__ void k_row_inv(const ProbDev* __restrict__ probs, int nprob, int tota
         run_fft<+1>(lds + uu * stride, lds + uu * stride + lds_stride(px), P.prow, t, true);
         const R s = (R)P.scale;
         bool bad = false;
-        double acc[3] = {0.0, 0.0, 0.0};
+        double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
         for (int i = tid; i < cnt * nx; i += bdim) {
             const int u2 = i / nx, x = i - u2 * nx, iy = iy0 + u2;
             if (iy < ny) {
                 const cpx<R> v = lds[u2 * stride + lidx(x + ox)];
                 const cpx<R> o = {v.re * s, v.im * s};
                 bad |= !finite_c(o);
-                store_post<R>(P, x, iy, o, acc);
+                store_post<R>(P, x, iy, o, acc0, acc1, acc2);
             }
         }
         if (bad) atomicOr(P.flag, 2);
         if (P.ops.post == POST_GS_TARGET || P.ops.post == POST_GS_PHASE)
-            accumulate4(acc[0], acc[1], acc[2], 0.0, P.ops.sums);
+            accumulate4(acc0, acc1, acc2, 0.0, P.ops.sums);
         __syncthreads();
     }
 }
@@ -475,7 +207,7 @@ __global__ void k_rs_kgen(const ProbDev* __restrict__ probs, int nprob, int tota
     extern __shared__ unsigned char smem_raw[];
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x, bdim = blockDim.x;
-    for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_KGEN>(probs, nprob, tile)];
         const int y0 = (tile - P.base_kgen) * P.rows_cnt;
         const int cnt = P.rows_cnt, px = P.px, py = P.py, B = P.B;

@@ -62,6 +62,18 @@ struct EngineDev {
     double lambda, z, k, sgn, dx, dy;
 };
 
+// What a launch of the power-of-two kernels holds uniformly, selecting the
+// compile-time specialisation: pass (0 row_fwd, 1 col, 2 row_inv, 3 rs_kgen),
+// length and layout variant, and the kinds every problem shares (-1: mixed).
+struct FastSel {
+    int pass;
+    int n, v;
+    int rs;          // Rayleigh-Sommerfeld batch
+    int plain_post;  // no problem has a post-op
+    int col_kind;    // ENG_ASM / ENG_FRESNEL when uniform, else -1
+    int pre_kind;    // PRE_NONE / PRE_MASK when uniform, else -1
+};
+
 // Parameters of the fused element-wise ops (gs.rs:484-564, adjoint.rs:200-236,
 // merit.rs:143-165).  Real arrays have the transform's element type R unless
 // noted; `sums` are per-problem f64 accumulators.

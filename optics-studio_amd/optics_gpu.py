@@ -22,6 +22,13 @@ LIB_PATH = os.path.join(HERE, "liboptics_gpu.so")
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {HERE}` (hipcc, gfx950)")
 
+# One HIP runtime per process: when PyTorch (used for device buffers) is
+# present, let it load its runtime first so the library binds to the same one.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 lib = C.CDLL(LIB_PATH)
 
 vp, c_f64, c_i32, c_u64, c_sz = C.c_void_p, C.c_double, C.c_int32, C.c_uint64, C.c_size_t
@@ -38,6 +45,10 @@ EXPORTS = {
     "oc_version": ([], C.c_char_p),
     "oc_context_enable_timing": ([vp, C.c_int], C.c_int),
     "oc_context_pass_timing": ([vp, P(c_f64), P(c_u64), C.c_int], C.c_int),
+    "oc_autotune_fft": ([vp, c_u64, C.c_int, P(C.c_int), P(c_f64)], C.c_int),
+    "oc_context_set_fft_variant": ([vp, c_u64, C.c_int, C.c_int], C.c_int),
+    "oc_context_fft_variant": ([vp, c_u64, C.c_int], C.c_int),
+    "oc_context_force_generic": ([vp, C.c_int], C.c_int),
     "oc_asm_sampling_report": ([P(T.BandGeom), c_f64, P(T.AsmConfig), P(T.AsmReport)], C.c_int),
     "oc_asm_propagate": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), vp, P(T.AsmReport)], C.c_int),
     "oc_asm_propagate_dev_c128": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), P(T.PhaseMask), vp,
@@ -124,6 +135,25 @@ class Context:
         lib.oc_context_pass_timing(self.h, ms, n, 1 if reset else 0)
         names = ["row_fwd", "col", "row_inv", "rs_kgen", "gs", "other"]
         return {names[i]: (ms[i], n[i]) for i in range(6)}
+
+    def autotune(self, n, double=False):
+        """Times every fast-kernel layout variant at padded length n; keeps the best.
+        Returns (variant, [ms per variant])."""
+        v = C.c_int()
+        ms = (c_f64 * 2)()
+        st = lib.oc_autotune_fft(self.h, n, 1 if double else 0, C.byref(v), ms)
+        if st:
+            raise OcError(f"autotune failed: {T.STATUS.get(st, st)} {self.last_error()}")
+        return v.value, list(ms)
+
+    def set_fft_variant(self, n, variant, double=False):
+        return lib.oc_context_set_fft_variant(self.h, n, 1 if double else 0, variant)
+
+    def fft_variant(self, n, double=False):
+        return lib.oc_context_fft_variant(self.h, n, 1 if double else 0)
+
+    def force_generic(self, on=True):
+        return lib.oc_context_force_generic(self.h, 1 if on else 0)
 
     def last_error(self):
         return lib.oc_last_error(self.h).decode()

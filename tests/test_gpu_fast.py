@@ -1,0 +1,171 @@
+"""Compile-time-specialised power-of-two kernels (padded lengths 64..8192) in
+every workgroup layout, against the f64 oracle and against the runtime-plan
+kernels of the same library.
+
+Each case pins the variant on a private context, so the default context's
+tuning is untouched.  Bars as in the other parity tests: 1e-12 relative L2 for
+complex128, 1e-5 for the FP32-bulk complex64 form.  At P = 8192 the oracle
+would need minutes, so the fast kernels are compared with the runtime-plan
+kernels (themselves pinned to the oracle at smaller sizes) at the same bars,
+plus an ASM(+z) then ASM(-z) round trip on a band-limited field.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oc_types as T
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+C64_TOL = 1e-5
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def G():
+    import optics_gpu
+    return optics_gpu
+
+
+def field(n, seed):
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[:n, :n]
+    c = (n - 1) / 2
+    env = np.exp(-(((x - c) ** 2 + (y - c) ** 2) / (0.18 * n) ** 2))
+    return env * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+
+
+def run_dev(G, ctx, engine, u, dtype, d, lam, z, cfg, mask=None):
+    import torch
+    t = torch.as_tensor(u.astype(np.complex64 if dtype == "c64" else np.complex128)).cuda()
+    if engine == "asm":
+        st, out, _ = G.asm_propagate_dev(t, d, d, lam, z, cfg, mask=mask, ctx=ctx)
+    elif engine == "fresnel":
+        st, out, _ = G.fresnel_propagate_dev(t, d, d, lam, z, cfg, ctx=ctx)
+    else:
+        st, out, _ = G.rs_propagate_dev(t, d, d, lam, z, cfg, ctx=ctx)
+    assert st == 0, (engine, T.STATUS[st], ctx.last_error())
+    return out.cpu().numpy().astype(np.complex128)
+
+
+def variants(n):
+    return [0, 1] if n >= 1024 else [0]
+
+
+CASES = [(p, v) for p in (64, 128, 256, 512, 1024, 2048, 4096) for v in variants(p)]
+
+
+@pytest.mark.parametrize("P,v", CASES, ids=[f"P{p}-v{v}" for p, v in CASES])
+def test_fast_vs_oracle(G, orc, P, v):
+    n = P // 2
+    d, lam = 2e-6, 532e-9
+    u = field(n, P + v)
+    rng = np.random.default_rng(P)
+    phi = rng.uniform(-np.pi, np.pi, (n, n))
+    ctx = G.Context(0)
+    for dbl in (False, True):
+        assert ctx.set_fft_variant(P, v, double=dbl) == 0
+        assert ctx.fft_variant(P, double=dbl) == v
+    say("fast", P, "variant", v)
+    mask, keep = G.dev_mask(phi, d, d)
+    om = orc.Mask(phi, d, d)
+    st, um = orc.phasemask_apply(u, d, d, lam, om)
+    assert st == 0
+    z_asm = 0.3 * n * d
+    st, ref_a, _ = orc.asm_propagate(um, d, d, lam, z_asm, T.AsmConfig(2.0, 1, 0, 2.0, 1.0))
+    assert st == 0
+    fcfg = T.FresnelConfig(2.0, 0, 0, 2.0)
+    z_fr = 2.0 * n * d * d / lam
+    st, ref_f, _ = orc.fresnel_propagate(u, d, d, lam, z_fr, fcfg)
+    assert st == 0
+    rcfg = T.RsConfig(2.0, 0, 0)
+    z_rs = 0.5 * n * d
+    do_rs = P <= 2048
+    if do_rs:
+        st, ref_r, _ = orc.rs_propagate(u, d, d, lam, z_rs, rcfg)
+        assert st == 0
+    for dt, tol in (("c64", C64_TOL), ("c128", F64_TOL)):
+        out = run_dev(G, ctx, "asm", u, dt, d, lam, z_asm, T.AsmConfig(2.0, 1, 0, 2.0, 1.0), mask=mask)
+        e = rel_l2(out, ref_a)
+        say("   masked asm", dt, e)
+        assert e <= tol
+        out = run_dev(G, ctx, "fresnel", u, dt, d, lam, z_fr, fcfg)
+        e = rel_l2(out, ref_f)
+        say("   fresnel", dt, e)
+        assert e <= tol
+        if do_rs:
+            out = run_dev(G, ctx, "rs", u, dt, d, lam, z_rs, rcfg)
+            e = rel_l2(out, ref_r)
+            say("   rs", dt, e)
+            assert e <= tol
+    ctx.synchronize()
+
+
+@pytest.mark.parametrize("v", [0, 1])
+def test_fast_p8192_vs_generic(G, v):
+    P, n = 8192, 4096
+    d, lam = 2e-6, 633e-9
+    u = field(n, 77 + v)
+    fast, gen = G.Context(0), G.Context(0)
+    gen.force_generic(True)
+    for dbl in (False, True):
+        assert fast.set_fft_variant(P, v, double=dbl) == 0
+    say("P8192 variant", v)
+    z = 0.2 * n * d
+    acfg = T.AsmConfig(2.0, 1, 0, 2.0, 1.0)
+    for dt, tol in (("c64", C64_TOL), ("c128", F64_TOL)):
+        a = run_dev(G, fast, "asm", u, dt, d, lam, z, acfg)
+        b = run_dev(G, gen, "asm", u, dt, d, lam, z, acfg)
+        e = rel_l2(a, b)
+        say("   asm fast vs generic", dt, e)
+        assert e <= tol
+    zf = 1.5 * n * d * d / lam
+    a = run_dev(G, fast, "fresnel", u, "c128", d, lam, zf, T.FresnelConfig(2.0, 0, 0, 2.0))
+    b = run_dev(G, gen, "fresnel", u, "c128", d, lam, zf, T.FresnelConfig(2.0, 0, 0, 2.0))
+    e = rel_l2(a, b)
+    say("   fresnel fast vs generic c128", e)
+    assert e <= F64_TOL
+
+
+def test_fast_asm_round_trip_p8192(G):
+    """A field band-limited well inside the propagating disc and the Matsushima
+    limit, with zero border, returns to itself under ASM(+z) then ASM(-z)."""
+    import torch
+    n, d, lam = 4096, 2e-6, 633e-9
+    y, x = np.mgrid[:n, :n]
+    c = (n - 1) / 2
+    u = np.exp(-(((x - c) ** 2 + (y - c) ** 2) / (0.08 * n) ** 2)) * np.exp(1j * 0.002 * (x + 2 * y))
+    ctx = G.Context(0)
+    cfg = T.AsmConfig(2.0, 1, 0, 2.0, 1.0)
+    z = 40 * d
+    t = torch.as_tensor(u).cuda()
+    st, a, _ = G.asm_propagate_dev(t, d, d, lam, z, cfg, ctx=ctx)
+    assert st == 0
+    st, b, _ = G.asm_propagate_dev(a, d, d, lam, -z, cfg, ctx=ctx)
+    assert st == 0
+    e = rel_l2(b.cpu().numpy(), u)
+    say("round trip", e)
+    assert e <= 1e-10
+
+
+def test_autotune_picks_a_variant(G):
+    ctx = G.Context(0)
+    for P, dbl in ((2048, False), (4096, False), (1024, True)):
+        v, ms = ctx.autotune(P, double=dbl)
+        say("autotune", P, dbl, v, ms)
+        assert v in (0, 1)
+        assert ms[v] > 0 and ms[v] == min(m for m in ms if m > 0)
+        assert ctx.fft_variant(P, double=dbl) == v
+    v, ms = ctx.autotune(96)
+    assert v == -1 and ms == [-1.0, -1.0]
+    assert ctx.set_fft_variant(512, 1) != 0  # one layout only below 1024

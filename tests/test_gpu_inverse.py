@@ -132,7 +132,8 @@ def test_gs_fft_graph_equals_eager(G):
     st, e2 = G.gs_fft_run_dev(a, t, p2, its, 2)  # three replays of a 2-iteration graph
     assert st == 0
     assert torch.equal(p1, p2)
-    assert np.array_equal(e1, e2)
+    # the error sums are wave-reduced f64 atomics: order-dependent in the last bits
+    assert np.allclose(e1, e2, rtol=1e-12, atol=0.0)
 
 
 def design(G, orc, n, d, z, seeds=(11, 12)):
@@ -149,16 +150,19 @@ def test_adjoint_push_back_matches_oracle(G, orc):
     dmdi = rng.standard_normal((n, n))
     cfg = T.AsmConfig.default()
     ap = (T.APERTURE_CIRCLE, 100e-6, 0.0)
+    # element 2 has no propagation after it (the z == 0 element kernel); a
+    # last element after it keeps its gradient away from zero (a detector
+    # right behind a phase-only element with a real dmdi has dphi == 0)
     steps_o = [(None, z / 2, 1.0), (orc.Mask(p1, d, d, aperture=ap), z, 1.0), (orc.Mask(p2, d, d), z, 1.3),
-               (orc.Mask(p1, d, d, orientation=0.4), 0.0, 1.0)]
+               (orc.Mask(p1, d, d, orientation=0.4), 0.0, 1.0), (orc.Mask(p2, d, d, orientation=-0.3), z / 3, 1.0)]
     steps_g = [(None, z / 2, 1.0), (G.Mask(p1, d, d, aperture=ap), z, 1.0), (G.Mask(p2, d, d), z, 1.3),
-               (G.Mask(p1, d, d, orientation=0.4), 0.0, 1.0)]
+               (G.Mask(p1, d, d, orientation=0.4), 0.0, 1.0), (G.Mask(p2, d, d, orientation=-0.3), z / 3, 1.0)]
     say("push_back")
     st, ref = orc.adjoint_push_back(steps_o, u, d, d, lam, cfg, dmdi)
     assert st == 0
     st, got = G.adjoint_push_back(steps_g, u, d, d, lam, cfg, dmdi)
     assert st == 0, T.STATUS[st]
-    for e in range(3):
+    for e in range(4):
         rel = np.linalg.norm(got[e] - ref[e]) / np.linalg.norm(ref[e])
         say("  element", e, rel)
         assert rel < 1e-10
