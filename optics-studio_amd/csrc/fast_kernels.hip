@@ -72,10 +72,10 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
             accumulate4(acc, 0.0, 0.0, 0.0, P.ops.sums ? P.ops.sums + 3 : nullptr);
         fft_regs<R, N, G, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         if (rowok) {
-            gmem<cpx<R>>* __restrict__ work = gp(static_cast<cpx<R>*>(P.work));
-            const int ny = P.ny;
+            cpx<R>* work = static_cast<cpx<R>*>(P.work);
+            const unsigned ny = P.ny, b0 = ((unsigned)tq * ny + iy) * Q + tb;
 #pragma unroll
-            for (int m = 0; m < 16; ++m) work[((size_t)(tq + m * (T / Q)) * ny + iy) * Q + tb] = v[m];
+            for (int m = 0; m < 16; ++m) at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q) = v[m];
         }
     }
 }
@@ -96,33 +96,37 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::B * (N / 16))) void f_col(con
         const int q = tile - P.base_col;
         const int ny = P.ny, oy = P.oy;
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
-        gmem<cpx<R>>* __restrict__ work = gp(static_cast<cpx<R>*>(P.work));
+        cpx<R>* work = static_cast<cpx<R>*>(P.work);
+        // element (q, y - oy, g) of the slab: b0 + y*B
+        const unsigned b0 = ((unsigned)q * ny - oy) * B + g;
         cpx<R> v[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
             v[m] = {R(0), R(0)};
-            if (y >= oy && y < oy + ny) v[m] = work[((size_t)q * ny + (y - oy)) * B + g];
+            if (y >= oy && y < oy + ny) v[m] = at32(work, b0 + (unsigned)y * B);
         }
         fft_regs<R, N, B, -1>(v, lds, g, t, tw);
         const int kx = q * B + g;
         if constexpr (RS) {
-            const gmem<const cpx<R>>* __restrict__ kb = gp(static_cast<const cpx<R>*>(P.kbuf));
+            const cpx<R>* kb = static_cast<const cpx<R>*>(P.kbuf);
+            const unsigned k0 = ((unsigned)q * N + t) * B + g;
             cpx<R> w[16];
 #pragma unroll
-            for (int m = 0; m < 16; ++m) w[m] = kb[((size_t)q * N + (t + m * T)) * B + g];
+            for (int m = 0; m < 16; ++m) w[m] = at32(kb, k0 + (unsigned)(m * T * B));
             fft_regs<R, N, B, -1>(w, lds, g, t, tw);
 #pragma unroll
             for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], w[m]);
         } else {
             const int ek = EK >= 0 ? EK : P.eng.kind;
             if (ek == ENG_MEASURE) {
-                const gmem<const R>* tb = gp(static_cast<const R*>(P.ops.tamp_blk));
+                const R* tb = static_cast<const R*>(P.ops.tamp_blk);
+                const unsigned k0 = ((unsigned)q * N + t) * B + g;
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
                 for (int m = 0; m < 16; ++m) {
                     const double in2 = (double)v[m].re * v[m].re + (double)v[m].im * v[m].im;
-                    const double ta = tb[((size_t)q * N + (t + m * T)) * B + g];
+                    const double ta = at32(tb, k0 + (unsigned)(m * T * B));
                     s0 += in2;
                     s1 += in2 * in2;
                     s2 += in2 * (ta * ta);
@@ -130,14 +134,19 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::B * (N / 16))) void f_col(con
                 accumulate4(s0, s1, s2, 0.0, P.ops.sums);
                 continue;
             } else if (ek == ENG_GS_PROJECT) {
-                const gmem<const R>* tb = gp(static_cast<const R*>(P.ops.tamp_blk));
+                const R* tb = static_cast<const R*>(P.ops.tamp_blk);
+                const unsigned k0 = ((unsigned)q * N + t) * B + g;
 #pragma unroll
                 for (int m = 0; m < 16; ++m) {
-                    const R a = tb[((size_t)q * N + (t + m * T)) * B + g];
+                    const R a = at32(tb, k0 + (unsigned)(m * T * B));
                     const R mg = hypot(v[m].re, v[m].im);
                     v[m] = mg > R(0) ? cpx<R>{v[m].re * (a / mg), v[m].im * (a / mg)} : cpx<R>{a, R(0)};
                 }
             } else if (ek != ENG_NONE) {
+                // keep the table loads below the forward transform: hoisted
+                // into it they would hold 32 more registers across its
+                // last stage
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int m = 0; m < 16; ++m) v[m] = transfer<R, EK>(P.eng, kx, t + m * T, v[m]);
             }
@@ -146,7 +155,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::B * (N / 16))) void f_col(con
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
-            if (y >= oy && y < oy + ny) work[((size_t)q * ny + (y - oy)) * B + g] = v[m];
+            if (y >= oy && y < oy + ny) at32(work, b0 + (unsigned)y * B) = v[m];
         }
     }
 }
@@ -168,26 +177,28 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
         const int iy = (tile - P.base_row) * G + g;
         const bool rowok = iy < P.ny;
         const int ny = P.ny, nx = P.nx, ox = P.ox;
-        const gmem<const cpx<R>>* __restrict__ work = gp(static_cast<const cpx<R>*>(P.work));
+        const cpx<R>* work = static_cast<const cpx<R>*>(P.work);
+        const unsigned b0 = ((unsigned)tq * ny + iy) * Q + tb;
         cpx<R> v[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             v[m] = {R(0), R(0)};
-            if (rowok) v[m] = work[((size_t)(tq + m * (T / Q)) * ny + iy) * Q + tb];
+            if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
         fft_regs<R, N, G, +1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         const R s = (R)P.scale;
         bool bad = false;
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
         if (rowok) {
-            gmem<cpx<R>>* __restrict__ out = gp(static_cast<cpx<R>*>(P.out)) + (size_t)iy * nx;
+            cpx<R>* out = static_cast<cpx<R>*>(P.out);
+            const unsigned o0 = (unsigned)iy * nx;
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
                 const int x = t + m * T - ox;
                 if (x >= 0 && x < nx) {
                     const cpx<R> o = {v[m].re * s, v[m].im * s};
                     bad |= !finite_c(o);
-                    if constexpr (PLAIN) out[x] = o;
+                    if constexpr (PLAIN) at32(out, o0 + x) = o;
                     else store_post<R>(P, x, iy, o, acc0, acc1, acc2);
                 }
             }
@@ -218,9 +229,10 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
             v[m] = rowok ? rs_sample<R>(P.eng, t + m * T, y, N, N) : cpx<R>{R(0), R(0)};
         fft_regs<R, N, G, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         if (rowok) {
-            gmem<cpx<R>>* __restrict__ kb = gp(static_cast<cpx<R>*>(P.kbuf));
+            cpx<R>* kb = static_cast<cpx<R>*>(P.kbuf);
+            const unsigned k0 = ((unsigned)tq * N + y) * Q + tb;
 #pragma unroll
-            for (int m = 0; m < 16; ++m) kb[((size_t)(tq + m * (T / Q)) * N + y) * Q + tb] = v[m];
+            for (int m = 0; m < 16; ++m) at32(kb, k0 + (unsigned)(m * (T / Q)) * N * Q) = v[m];
         }
     }
 }

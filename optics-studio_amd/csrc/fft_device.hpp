@@ -43,6 +43,15 @@ __device__ __forceinline__ gmem<T>* gp(T* p) {
     return (gmem<T>*)p;
 }
 
+// Element idx of a device array addressed with a 32-bit byte offset from a
+// wave-uniform base (one offset register per access instead of a 64-bit
+// address pair).  For arrays below 4 GiB.
+template <typename T>
+__device__ __forceinline__ gmem<T>& at32(T* base, unsigned idx) {
+    const gmem<const char>* b = (const gmem<const char>*)gp(base);
+    return *(gmem<T>*)(b + idx * (unsigned)sizeof(T));
+}
+
 template <typename R>
 __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};

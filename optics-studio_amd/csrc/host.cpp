@@ -318,7 +318,35 @@ std::string fmtkey(const char* tag, std::initializer_list<double> vals) {
 }
 
 // Builds (or reuses) the per-axis engine tables of one problem.
-int engine_tables(oc_context* ctx, const Prob& p, EngineDev* e) {
+// Per-axis factors of the Fresnel transfer function for complex64 launches
+// (see EngineDev::hx): phases k z - c fx^2 and -c fy^2 in f64, sin/cos in
+// long double, rounded to complex64.
+int fresnel_factors(oc_context* ctx, const Prob& p, EngineDev* e) {
+    const size_t nxp = p.nxp, nyp = p.nyp;
+    std::string key = fmtkey("frh", {(double)nxp, (double)nyp, p.g.dx, p.g.dy, p.g.lambda, p.z});
+    auto it = ctx->tables.find(key);
+    if (it == ctx->tables.end()) {
+        std::vector<float> h(2 * (nxp + nyp));
+        for (size_t k = 0; k < nxp + nyp; ++k) {
+            const bool isx = k < nxp;
+            const double f = isx ? och::fftfreq_at(k, nxp, p.g.dx) : och::fftfreq_at(k - nxp, nyp, p.g.dy);
+            const double ph = isx ? e->kz - e->c * (f * f) : -(e->c * (f * f));
+            h[2 * k] = (float)std::cos((long double)ph);
+            h[2 * k + 1] = (float)std::sin((long double)ph);
+        }
+        TableEntry te;
+        const size_t bytes = h.size() * sizeof(float);
+        HIP_TRY(ctx, hipMalloc(&te.dev, bytes));
+        HIP_TRY(ctx, hipMemcpy(te.dev, h.data(), bytes, hipMemcpyHostToDevice));
+        te.bytes = bytes;
+        it = ctx->tables.emplace(key, te).first;
+    }
+    e->hx = it->second.dev;
+    e->hy = static_cast<const float*>(it->second.dev) + 2 * nxp;
+    return OC_OK;
+}
+
+int engine_tables(oc_context* ctx, const Prob& p, bool dbl, EngineDev* e) {
     const size_t nxp = p.nxp, nyp = p.nyp;
     std::memset(e, 0, sizeof *e);
     e->kind = p.engine;
@@ -385,6 +413,7 @@ int engine_tables(oc_context* ctx, const Prob& p, EngineDev* e) {
     e->ay = e->ax + nxp;
     e->mx = d + (nxp + nyp) * 8;
     e->my = e->mx + nxp;
+    if (p.engine == ENG_FRESNEL && !dbl) return fresnel_factors(ctx, p, e);
     return OC_OK;
 }
 
@@ -711,7 +740,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         d.flag = dflags + cand[a];
         d.ops = p.ops;
         if (p.engine == ENG_ASM || p.engine == ENG_FRESNEL || p.engine == ENG_RS) {
-            st = engine_tables(ctx, p, &d.eng);
+            st = engine_tables(ctx, p, dbl, &d.eng);
             if (st) return st;
         } else {
             d.eng.kind = p.engine;

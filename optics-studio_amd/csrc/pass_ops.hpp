@@ -184,7 +184,15 @@ __device__ __forceinline__ cpx<R> transfer(const EngineDev& e, int kx, int ky, c
         return cmul(h, unit_phase<R>(e.kz * sqrt(s)));
     }
     // Fresnel TF: phase = k z - (pi lambda z) (fx^2 + fy^2)
-    return cmul(h, unit_phase<R>(e.kz - e.c * (ax[kx] + ay[ky])));
+    if constexpr (sizeof(R) == 4) {
+        // complex64: the factored form (the host always provides hx, hy)
+        cpx<R> a, b;
+        a = at32(static_cast<const cpx<R>*>(e.hx), kx);
+        b = at32(static_cast<const cpx<R>*>(e.hy), ky);
+        return cmul(h, cmul(a, b));
+    } else {
+        return cmul(h, unit_phase<R>(e.kz - e.c * (ax[kx] + ay[ky])));
+    }
 }
 
 // rayleigh_sommerfeld.rs:228-235 sampled at DFT-order coordinates, times dx dy.

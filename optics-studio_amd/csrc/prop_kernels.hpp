@@ -58,6 +58,11 @@ struct EngineDev {
     const double* ay;
     const unsigned char* mx;  // ASM band-limit mask per axis (1 = masked)
     const unsigned char* my;
+    // Fresnel, complex64 launches: the transfer function factored per axis,
+    // H = hx[kx] hy[ky] with hx = exp(i (k z - c fx^2)), hy = exp(-i c fy^2)
+    // (phases in f64 on the host, factors rounded to complex64).
+    const void* hx;
+    const void* hy;
     // Rayleigh-Sommerfeld kernel sampling
     double lambda, z, k, sgn, dx, dy;
 };
