@@ -61,11 +61,51 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
         bool bad = false;
         double acc = 0.0;
         cpx<R> v[16];
+        if constexpr (PK == PRE_NONE || PK == PRE_MASK) {
+            // the row's 16 samples as one batch of loads, then the checks,
+            // then the mask with its mode decided once for the row
+            const cpx<R>* in = static_cast<const cpx<R>*>(P.in);
+            const unsigned r0 = (unsigned)iy * nx - ox;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const int x = t + m * T;
-            v[m] = {R(0), R(0)};
-            if (rowok && x >= ox && x < ox + nx) v[m] = load_pre<R, PK>(P, x - ox, iy, bad, acc);
+            for (int m = 0; m < 16; ++m) {
+                const int x = t + m * T;
+                v[m] = {R(0), R(0)};
+                if (rowok && x >= ox && x < ox + nx) v[m] = at32(in, r0 + x);
+            }
+#pragma unroll
+            for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
+            if constexpr (PK == PRE_MASK) {
+                const MaskDev& mk = P.mask;
+                if (mk.mode == MASK_SEPARABLE) {
+                    MaskAxis Y;
+                    Y = at32(mk.sy, rowok ? iy : 0);
+#pragma unroll
+                    for (int m = 0; m < 16; ++m) {
+                        const int x = t + m * T;
+                        if (rowok && x >= ox && x < ox + nx) {
+                            MaskAxis X;
+                            X = at32(mk.sx, x - ox);
+                            const double lv = bilinear(mk.values, mk.mnx, X.i0, X.i1, Y.i0, Y.i1, X.t, Y.t);
+                            const double phi = (X.in & Y.in) ? lv : 0.0;
+                            const cpx<R> o = cmul(v[m], unit_phase<R>(phi));
+                            v[m] = ap_contains(mk, X.r, Y.r) ? o : cpx<R>{R(0), R(0)};
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 16; ++m) {
+                        const int x = t + m * T;
+                        if (rowok && x >= ox && x < ox + nx) v[m] = apply_mask(mk, x - ox, iy, v[m]);
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int x = t + m * T;
+                v[m] = {R(0), R(0)};
+                if (rowok && x >= ox && x < ox + nx) v[m] = load_pre<R, PK>(P, x - ox, iy, bad, acc);
+            }
         }
         if (bad) atomicOr(P.flag, 1);
         if (PK < 0 && P.ops.pre == PRE_SEED_MSE)
@@ -142,11 +182,13 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::B * (N / 16))) void f_col(con
                     const R mg = hypot(v[m].re, v[m].im);
                     v[m] = mg > R(0) ? cpx<R>{v[m].re * (a / mg), v[m].im * (a / mg)} : cpx<R>{a, R(0)};
                 }
+            } else if (EK == ENG_ASM) {
+                // the column's table entries once, the 16 rows' as a batch
+                const EngineDev& e = P.eng;
+                const double axk = at32(e.ax, kx);
+#pragma unroll
+                for (int m = 0; m < 16; ++m) v[m] = asm_transfer<R>(e.kz, axk, at32(e.ay, t + m * T), v[m]);
             } else if (ek != ENG_NONE) {
-                // keep the table loads below the forward transform: hoisted
-                // into it they would hold 32 more registers across its
-                // last stage
-                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int m = 0; m < 16; ++m) v[m] = transfer<R, EK>(P.eng, kx, t + m * T, v[m]);
             }

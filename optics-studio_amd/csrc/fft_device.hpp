@@ -52,9 +52,15 @@ __device__ __forceinline__ gmem<T>& at32(T* base, unsigned idx) {
     return *(gmem<T>*)(b + idx * (unsigned)sizeof(T));
 }
 
+// Fused multiply-add.  The transforms and field products use it explicitly
+// (the files are built without automatic contraction so that f64 phase
+// arguments round exactly as written).
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename R>
 __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
-    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return {fmadd(a.re, b.re, -(a.im * b.im)), fmadd(a.re, b.im, a.im * b.re)};
 }
 template <typename R>
 __device__ __forceinline__ cpx<R> cadd(cpx<R> a, cpx<R> b) {
@@ -75,7 +81,7 @@ __device__ __forceinline__ cpx<R> rot(cpx<R> a) {
 template <int DIR, typename R>
 __device__ __forceinline__ cpx<R> twmul(cpx<R> a, R c, R s) {
     const R si = DIR < 0 ? -s : s;
-    return {a.re * c - a.im * si, a.re * si + a.im * c};
+    return {fmadd(a.re, c, -(a.im * si)), fmadd(a.re, si, a.im * c)};
 }
 
 // LDS index with a one-element pad every 32 elements (spreads strided accesses

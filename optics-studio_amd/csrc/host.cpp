@@ -375,31 +375,21 @@ int engine_tables(oc_context* ctx, const Prob& p, bool dbl, EngineDev* e) {
     std::string key = fmtkey(p.engine == ENG_ASM ? "asm" : "fr",
                              {(double)nxp, (double)nyp, p.g.dx, p.g.dy, lam_n, (double)p.band_limited, lim_x, lim_y});
     auto it = ctx->tables.find(key);
-    const size_t bytes = (nxp + nyp) * 9 + 64;
+    const size_t bytes = (nxp + nyp) * 8;
     if (it == ctx->tables.end()) {
-        std::vector<unsigned char> h(bytes, 0);
-        double* ax = reinterpret_cast<double*>(h.data());
-        double* ay = ax + nxp;
-        unsigned char* mx = h.data() + (nxp + nyp) * 8;
-        unsigned char* my = mx + nxp;
-        for (size_t k = 0; k < nxp; ++k) {
-            const double f = och::fftfreq_at(k, nxp, p.g.dx);
+        // ASM: (lam_n f)^2, or +inf where the band limit masks the frequency
+        // (then 1 - ax - ay = -inf < 0 and the sample is zeroed like an
+        // evanescent one).  Fresnel: f^2.
+        std::vector<double> h(nxp + nyp);
+        for (size_t k = 0; k < nxp + nyp; ++k) {
+            const bool isx = k < nxp;
+            const double f = isx ? och::fftfreq_at(k, nxp, p.g.dx) : och::fftfreq_at(k - nxp, nyp, p.g.dy);
             if (p.engine == ENG_ASM) {
                 const double a = lam_n * f;
-                ax[k] = a * a;
-                mx[k] = (p.band_limited && std::fabs(f) > lim_x) ? 1 : 0;
+                const bool masked = p.band_limited && std::fabs(f) > (isx ? lim_x : lim_y);
+                h[k] = masked ? HUGE_VAL : a * a;
             } else {
-                ax[k] = f * f;
-            }
-        }
-        for (size_t k = 0; k < nyp; ++k) {
-            const double f = och::fftfreq_at(k, nyp, p.g.dy);
-            if (p.engine == ENG_ASM) {
-                const double a = lam_n * f;
-                ay[k] = a * a;
-                my[k] = (p.band_limited && std::fabs(f) > lim_y) ? 1 : 0;
-            } else {
-                ay[k] = f * f;
+                h[k] = f * f;
             }
         }
         TableEntry te;
@@ -408,11 +398,8 @@ int engine_tables(oc_context* ctx, const Prob& p, bool dbl, EngineDev* e) {
         te.bytes = bytes;
         it = ctx->tables.emplace(key, te).first;
     }
-    unsigned char* d = static_cast<unsigned char*>(it->second.dev);
-    e->ax = reinterpret_cast<const double*>(d);
+    e->ax = static_cast<const double*>(it->second.dev);
     e->ay = e->ax + nxp;
-    e->mx = d + (nxp + nyp) * 8;
-    e->my = e->mx + nxp;
     if (p.engine == ENG_FRESNEL && !dbl) return fresnel_factors(ctx, p, e);
     return OC_OK;
 }
@@ -466,51 +453,36 @@ int mask_tables(oc_context* ctx, const oc_band_geom& g, const oc_phase_mask& m, 
     std::string key = fmtkey("mask", {(double)nx, (double)ny, g.dx, g.dy, (double)m.nx, (double)m.ny, m.dx, m.dy,
                                       m.cx, m.cy});
     auto it = ctx->tables.find(key);
-    // layout: i0x,i1x (int) tx,xr (double) inx (u8) | same for y
-    const size_t bx = nx * (4 + 4 + 8 + 8 + 1), by = ny * (4 + 4 + 8 + 8 + 1);
     if (it == ctx->tables.end()) {
-        std::vector<unsigned char> h(bx + by + 64, 0);
-        auto fill = [&](unsigned char* base, size_t n, double d, size_t mn, double md_, double c) {
-            int* i0 = reinterpret_cast<int*>(base);
-            int* i1 = i0 + n;
-            double* t = reinterpret_cast<double*>(base + n * 8);
-            double* xr = t + n;
-            unsigned char* in = base + n * 24;
+        std::vector<MaskAxis> h(nx + ny);
+        auto fill = [](MaskAxis* out, size_t n, double d, size_t mn, double md_, double c) {
             for (size_t i = 0; i < n; ++i) {
                 const double x = ((double)i - ((double)n - 1.0) / 2.0) * d;
-                const double r = x - c;
-                xr[i] = r;
-                const double f = r / md_ + ((double)mn - 1.0) / 2.0;
+                MaskAxis a{};
+                a.r = x - c;
+                const double f = a.r / md_ + ((double)mn - 1.0) / 2.0;
                 const bool inside = !(f < 0.0 || f > (double)mn - 1.0);
-                in[i] = inside ? 1 : 0;
+                a.in = inside ? 1 : 0;
                 if (inside) {
-                    const size_t a = (size_t)std::floor(f);
-                    i0[i] = (int)a;
-                    i1[i] = (int)std::min(a + 1, mn - 1);
-                    t[i] = f - (double)a;
+                    const size_t k = (size_t)std::floor(f);
+                    a.i0 = (int)k;
+                    a.i1 = (int)std::min(k + 1, mn - 1);
+                    a.t = f - (double)k;
                 }
+                out[i] = a;
             }
         };
         fill(h.data(), nx, g.dx, m.nx, m.dx, m.cx);
-        fill(h.data() + bx, ny, g.dy, m.ny, m.dy, m.cy);
+        fill(h.data() + nx, ny, g.dy, m.ny, m.dy, m.cy);
         TableEntry te;
-        HIP_TRY(ctx, hipMalloc(&te.dev, h.size()));
-        HIP_TRY(ctx, hipMemcpy(te.dev, h.data(), h.size(), hipMemcpyHostToDevice));
-        te.bytes = h.size();
+        const size_t bytes = h.size() * sizeof(MaskAxis);
+        HIP_TRY(ctx, hipMalloc(&te.dev, bytes));
+        HIP_TRY(ctx, hipMemcpy(te.dev, h.data(), bytes, hipMemcpyHostToDevice));
+        te.bytes = bytes;
         it = ctx->tables.emplace(key, te).first;
     }
-    unsigned char* d = static_cast<unsigned char*>(it->second.dev);
-    md->i0x = reinterpret_cast<const int*>(d);
-    md->i1x = md->i0x + nx;
-    md->tx = reinterpret_cast<const double*>(d + nx * 8);
-    md->xr = md->tx + nx;
-    md->inx = d + nx * 24;
-    unsigned char* e = d + bx;
-    md->j0y = reinterpret_cast<const int*>(e);
-    md->j1y = md->j0y + ny;
-    md->ty = reinterpret_cast<const double*>(e + ny * 8);
-    md->yr = md->ty + ny;
-    md->iny = e + ny * 24;
+    md->sx = static_cast<const MaskAxis*>(it->second.dev);
+    md->sy = md->sx + nx;
     return OC_OK;
 }
 

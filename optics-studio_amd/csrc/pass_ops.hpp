@@ -18,9 +18,9 @@ constexpr double kInvTwoPi = 0.159154943091895335768883763372514362;
 // FP64 by Cody-Waite with a three-part pi/2 (the product n*C1 and n*C2 are
 // exact for |n| < 2^31, so phases up to ~3e9 rad reduce to ~1e-16 absolute);
 // the reduced |x| <= pi/4 goes through Taylor polynomials: degree 17/18 in
-// FP64 (truncation < 5e-17) for the complex128 path, degree 9/8 in FP32
-// (truncation < 4e-7 absolute) for the FP32-bulk path.  The quadrant n mod 4
-// then rotates (sin, cos).
+// FP64 (truncation < 5e-17) for the complex128 path, degree 11/10 in FP32
+// (truncation < 1.2e-10, so FP32 rounding dominates) for the FP32-bulk path.
+// The quadrant n mod 4 then rotates (sin, cos).
 constexpr double kPio2_1 = 1.57079632673412561416626;  // pi/2 rounded to 33 bits
 constexpr double kPio2_2 = 6.077100506303965976595549e-11;  // next 33 bits
 constexpr double kPio2_3 = 2.022266248795950631541144e-21;  // remainder (53 bits)
@@ -31,18 +31,17 @@ __device__ __forceinline__ double reduce_pio2(double th, int* quad) {
     double r = fma(-n, kPio2_1, th);
     r = fma(-n, kPio2_2, r);
     r = fma(-n, kPio2_3, r);
-    *quad = (int)(long long)n & 3;
+    *quad = (int)n & 3;  // |n| < 2^31 (see above)
     return r;
 }
 
+// (cos, sin) of x + q pi/2 from (sin x, cos x), with selects (no branches):
+// q = 0: (c, s); 1: (-s, c); 2: (-c, -s); 3: (s, -c).
 template <typename R>
 __device__ __forceinline__ cpx<R> quadrant(R s, R c, int q) {
-    switch (q) {
-        case 0: return {c, s};
-        case 1: return {-s, c};
-        case 2: return {-c, -s};
-        default: return {s, -c};
-    }
+    const R a = (q & 1) ? s : c;
+    const R b = (q & 1) ? c : s;
+    return {((q + 1) & 2) ? -a : a, (q & 2) ? -b : b};
 }
 
 template <typename R>
@@ -135,13 +134,14 @@ __device__ __forceinline__ double bilinear(const double* __restrict__ v, int mnx
 // otherwise *phi is the lattice phase (0 beyond the lattice span).
 __device__ __forceinline__ bool mask_phase(const MaskDev& m, int ix, int iy, double* phi) {
     if (m.mode == MASK_SEPARABLE) {
-        if (!ap_contains(m, gp(m.xr)[ix], gp(m.yr)[iy])) return false;
-        if (gp(m.inx)[ix] && gp(m.iny)[iy])
-            *phi = bilinear(m.values, m.mnx, gp(m.i0x)[ix], gp(m.i1x)[ix], gp(m.j0y)[iy], gp(m.j1y)[iy],
-                            gp(m.tx)[ix], gp(m.ty)[iy]);
-        else
-            *phi = 0.0;
-        return true;
+        // lattice reads stay in bounds (indices 0 outside the span), so
+        // they are issued unconditionally and selected afterwards
+        MaskAxis X, Y;
+        X = at32(m.sx, ix);
+        Y = at32(m.sy, iy);
+        const double v = bilinear(m.values, m.mnx, X.i0, X.i1, Y.i0, Y.i1, X.t, Y.t);
+        *phi = (X.in & Y.in) ? v : 0.0;
+        return ap_contains(m, X.r, Y.r);
     }
     const double x = ((double)ix - ((double)m.fnx - 1.0) / 2.0) * m.fdx;
     const double y = ((double)iy - ((double)m.fny - 1.0) / 2.0) * m.fdy;
@@ -171,18 +171,42 @@ __device__ __forceinline__ cpx<R> apply_mask(const MaskDev& m, int ix, int iy, c
 /* ------------------------------------------------------------------ */
 /* Transfer functions (asm.rs:273-281, fresnel.rs:258-262).             */
 /* ------------------------------------------------------------------ */
+// sqrt of an ASM radicand for the phase k z sqrt(s).  complex128: IEEE sqrt
+// (the phase argument rounds as the reference's).  complex64: hardware
+// reciprocal square root and one Newton step (relative error ~1e-14, a phase
+// error below 1e-8 rad at |k z| < 1e6), clamped so that s = 0 gives ~0.
+template <typename R>
+__device__ __forceinline__ double asm_sqrt(double s) {
+    if constexpr (sizeof(R) == 8) {
+        return sqrt(s);
+    } else {
+        const double x = fmax(s, 1e-300);
+        const double r = __builtin_amdgcn_rsq(x);
+        const double y = x * r;
+        return fma(fma(-y, y, x), 0.5 * r, y);
+    }
+}
+
+// ASM transfer for one sample with its axis table entries (asm.rs:273-281):
+// zero when band-limited (an entry is +inf) or evanescent, else
+// h exp(i kz sqrt(s)).  Written without branches: every lane computes the
+// phase and a select keeps it.
+template <typename R>
+__device__ __forceinline__ cpx<R> asm_transfer(double kz, double axk, double ayk, cpx<R> h) {
+    const double s = (1.0 - axk) - ayk;
+    const bool dead = s < 0.0;
+    const cpx<R> t = cmul(h, unit_phase<R>(kz * asm_sqrt<R>(dead ? 0.0 : s)));
+    return dead ? cpx<R>{R(0), R(0)} : t;
+}
+
 // EK: the engine when known at compile time (ENG_ASM / ENG_FRESNEL), else -1
 // and the descriptor's kind decides.
 template <typename R, int EK = -1>
 __device__ __forceinline__ cpx<R> transfer(const EngineDev& e, int kx, int ky, cpx<R> h) {
     const gmem<const double>* ax = gp(e.ax);
     const gmem<const double>* ay = gp(e.ay);
-    if (EK == ENG_ASM || (EK < 0 && e.kind == ENG_ASM)) {
-        if (gp(e.mx)[kx] | gp(e.my)[ky]) return {R(0), R(0)};
-        const double s = (1.0 - ax[kx]) - ay[ky];
-        if (s < 0.0) return {R(0), R(0)};
-        return cmul(h, unit_phase<R>(e.kz * sqrt(s)));
-    }
+    if (EK == ENG_ASM || (EK < 0 && e.kind == ENG_ASM))
+        return asm_transfer<R>(e.kz, ax[kx], ay[ky], h);
     // Fresnel TF: phase = k z - (pi lambda z) (fx^2 + fy^2)
     if constexpr (sizeof(R) == 4) {
         // complex64: the factored form (the host always provides hx, hy)

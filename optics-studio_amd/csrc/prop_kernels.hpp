@@ -24,6 +24,15 @@ enum MaskMode : int { MASK_NONE = 0, MASK_SEPARABLE = 1, MASK_GENERAL = 2 };
 // PhaseMask::apply ingredients (src/element.rs:1443-1484).  The separable form
 // is used when psi == 0: element-frame coordinates are then exactly the
 // centre-relative field coordinates, so lattice indices/weights are per axis.
+// One field sample's position on one axis of an unrotated phase mask.
+struct MaskAxis {
+    double t;    // fractional lattice coordinate f - i0
+    double r;    // field coordinate relative to the mask centre (stop test)
+    int i0, i1;  // bracketing lattice indices (0, 0 outside the span)
+    int in;      // 1 when the coordinate lies within the lattice span
+    int pad;
+};
+
 struct MaskDev {
     int mode;
     int ap_kind;
@@ -37,16 +46,8 @@ struct MaskDev {
     int fnx, fny;
     double fdx, fdy;
     // separable form, per field axis
-    const int* i0x;
-    const int* i1x;
-    const double* tx;
-    const double* xr;
-    const unsigned char* inx;
-    const int* j0y;
-    const int* j1y;
-    const double* ty;
-    const double* yr;
-    const unsigned char* iny;
+    const MaskAxis* sx;
+    const MaskAxis* sy;
 };
 
 struct EngineDev {
@@ -54,10 +55,8 @@ struct EngineDev {
     int _pad;
     double kz;  // ASM: (2 pi/lam_n)*z ; Fresnel: (2 pi/lambda)*z
     double c;   // Fresnel: (pi*lambda)*z
-    const double* ax;  // ASM: (lam_n*fx)^2 ; Fresnel: fx*fx
+    const double* ax;  // ASM: (lam_n*fx)^2, +inf where band-limited ; Fresnel: fx*fx
     const double* ay;
-    const unsigned char* mx;  // ASM band-limit mask per axis (1 = masked)
-    const unsigned char* my;
     // Fresnel, complex64 launches: the transfer function factored per axis,
     // H = hx[kx] hy[ky] with hx = exp(i (k z - c fx^2)), hy = exp(-i c fy^2)
     // (phases in f64 on the host, factors rounded to complex64).
