@@ -207,16 +207,17 @@ int oc_context_pass_timing(oc_context* ctx, double* ms, uint64_t* launches,
                            int reset);
 
 /* Autotuner (no reference counterpart).  Power-of-two padded lengths
-   64..8192 have compile-time-specialised kernels in up to two workgroup
-   layouts (variant 0: up to 1024 threads and wider contiguous segments;
-   variant 1: up to 512 threads and twice the registers per thread).
-   oc_autotune_fft times a full masked-ASM propagation round of every variant
-   at length n on a synthetic batch and keeps the fastest for this context;
-   ms_out (optional, 2 entries) receives the per-variant times (negative for
-   variants that do not exist).  oc_context_set_fft_variant forces a choice;
+   64..8192 have compile-time-specialised kernels in up to OC_FFT_VARIANTS
+   workgroup layouts (rows per row-pass workgroup, width of the blocked
+   intermediate, columns per column-pass workgroup; 1024- or 512-thread
+   caps).  oc_autotune_fft times a masked-ASM plus Fresnel propagation round
+   of every variant at length n on a synthetic batch and keeps the fastest for
+   this context; ms_out (optional, OC_FFT_VARIANTS entries) receives the
+   per-variant times (negative for variants that do not exist).  oc_context_set_fft_variant forces a choice;
    oc_context_fft_variant reports the current one (-1: no specialised kernel).
    oc_context_force_generic(ctx, 1) routes every batch through the
    runtime-plan kernels instead. */
+#define OC_FFT_VARIANTS 4
 int oc_autotune_fft(oc_context* ctx, uint64_t n, int double_precision,
                     int* variant_out, double* ms_out);
 int oc_context_set_fft_variant(oc_context* ctx, uint64_t n,

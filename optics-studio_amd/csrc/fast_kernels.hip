@@ -25,21 +25,33 @@
 
 namespace ocd {
 
-// Rows (and columns) per workgroup for a given element type and length:
-// at most MAXT threads, LDS within the 140 KB budget, at most 16.
-// Variant V = 0 allows 1024-thread workgroups (more rows/columns per
-// workgroup, wider contiguous segments, 128 VGPRs); V = 1 caps them at 512
-// threads (256 VGPRs, no spills at large N).  The autotuner picks per size.
+// Workgroup layout of variant V at length N:
+//   G  rows per row-pass workgroup (G*T threads),
+//   B  width of the blocked intermediate [P/B][ny][B] (a row workgroup
+//      writes runs of G*B contiguous elements; a column workgroup reads
+//      slabs of B*ny),
+//   CB columns per column-pass workgroup (CB*T threads, CB | B).
+// Thread caps per variant (rows / cols) and the run width:
+//   V0: 1024 / 1024, B = G    -- fewest, widest workgroups
+//   V1:  512 /  512, B = G    -- more registers per thread
+//   V2: 1024 /  512, B = G    -- wide row passes, two column workgroups/CU
+//   V3:  512 /  512, B = 2G   -- narrow row passes with V0's run width
+// each within the 140 KB LDS budget and at most 16 rows/columns.  The
+// autotuner picks per size; N < 1024 has V0 only.
 template <typename R, int N, int V = 0>
 struct FastLayout {
     static constexpr int T = N / 16;
     static constexpr int ESZ = (int)sizeof(cpx<R>);
     static constexpr int BYTES = Pow2Plan<N>::SLOTS * ESZ;
-    static constexpr int MAXT = V == 0 ? 1024 : 512;
-    static constexpr int G0 = (MAXT / T) < (140 * 1024 / BYTES) ? (MAXT / T) : (140 * 1024 / BYTES);
-    static constexpr int G = G0 < 16 ? G0 : 16;
-    static constexpr int B = G < T ? G : T;
-    static_assert(G >= 1 && B >= 1 && (T % B) == 0, "layout");
+    static constexpr int LG = 140 * 1024 / BYTES;
+    static constexpr int ROWCAP = (V == 1 || V == 3) ? 512 : 1024;
+    static constexpr int COLCAP = V == 0 ? 1024 : 512;
+    static constexpr int BM = V == 3 ? 2 : 1;
+    static constexpr int min3(int a, int b, int c) { return a < b ? (a < c ? a : c) : (b < c ? b : c); }
+    static constexpr int G = min3(ROWCAP / T, LG, 16);
+    static constexpr int B = min3(G * BM, T, 16);
+    static constexpr int CB = min3(B, COLCAP / T, LG);
+    static_assert(G >= 1 && B >= 1 && CB >= 1 && (T % B) == 0 && (B % CB) == 0, "layout");
 };
 
 // PK: pre-op (PRE_NONE, PRE_MASK, or -1 for the descriptor's).
@@ -123,22 +135,23 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
 // EK: engine (ENG_ASM, ENG_FRESNEL, or -1 for the descriptor's); RS: the
 // Rayleigh-Sommerfeld product with the kernel spectrum instead.
 template <typename R, int N, int V, bool RS, int EK>
-__global__ __launch_bounds__((FastLayout<R, N, V>::B * (N / 16))) void f_col(const ProbDev* __restrict__ probs, int nprob,
-                                                                     int total) {
+__global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(const ProbDev* __restrict__ probs,
+                                                                      int nprob, int total) {
     using LY = FastLayout<R, N, V>;
-    constexpr int T = N / 16, B = LY::B;
+    constexpr int T = N / 16, B = LY::B, CB = LY::CB;
     extern __shared__ unsigned char smem_raw[];
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x;
-    const int g = tid % B, t = tid / B;
+    const int g = tid % CB, t = tid / CB;
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
-        const int q = tile - P.base_col;
+        const int kx = (tile - P.base_col) * CB + g;  // this thread's column
+        const int q = kx / B, bq = kx % B;          // its block and place in it
         const int ny = P.ny, oy = P.oy;
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
         cpx<R>* work = static_cast<cpx<R>*>(P.work);
-        // element (q, y - oy, g) of the slab: b0 + y*B
-        const unsigned b0 = ((unsigned)q * ny - oy) * B + g;
+        // element (q, y - oy, bq) of the slab: b0 + y*B
+        const unsigned b0 = ((unsigned)q * ny - oy) * B + bq;
         cpx<R> v[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
@@ -146,22 +159,21 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::B * (N / 16))) void f_col(con
             v[m] = {R(0), R(0)};
             if (y >= oy && y < oy + ny) v[m] = at32(work, b0 + (unsigned)y * B);
         }
-        fft_regs<R, N, B, -1>(v, lds, g, t, tw);
-        const int kx = q * B + g;
+        fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
         if constexpr (RS) {
             const cpx<R>* kb = static_cast<const cpx<R>*>(P.kbuf);
-            const unsigned k0 = ((unsigned)q * N + t) * B + g;
+            const unsigned k0 = ((unsigned)q * N + t) * B + bq;
             cpx<R> w[16];
 #pragma unroll
             for (int m = 0; m < 16; ++m) w[m] = at32(kb, k0 + (unsigned)(m * T * B));
-            fft_regs<R, N, B, -1>(w, lds, g, t, tw);
+            fft_regs<R, N, CB, -1>(w, lds, g, t, tw);
 #pragma unroll
             for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], w[m]);
         } else {
             const int ek = EK >= 0 ? EK : P.eng.kind;
             if (ek == ENG_MEASURE) {
                 const R* tb = static_cast<const R*>(P.ops.tamp_blk);
-                const unsigned k0 = ((unsigned)q * N + t) * B + g;
+                const unsigned k0 = ((unsigned)q * N + t) * B + bq;
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
                 for (int m = 0; m < 16; ++m) {
@@ -175,7 +187,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::B * (N / 16))) void f_col(con
                 continue;
             } else if (ek == ENG_GS_PROJECT) {
                 const R* tb = static_cast<const R*>(P.ops.tamp_blk);
-                const unsigned k0 = ((unsigned)q * N + t) * B + g;
+                const unsigned k0 = ((unsigned)q * N + t) * B + bq;
 #pragma unroll
                 for (int m = 0; m < 16; ++m) {
                     const R a = at32(tb, k0 + (unsigned)(m * T * B));
@@ -193,7 +205,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::B * (N / 16))) void f_col(con
                 for (int m = 0; m < 16; ++m) v[m] = transfer<R, EK>(P.eng, kx, t + m * T, v[m]);
             }
         }
-        fft_regs<R, N, B, +1>(v, lds, g, t, tw);
+        fft_regs<R, N, CB, +1>(v, lds, g, t, tw);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
@@ -298,8 +310,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
     using LY = FastLayout<R, N, V>;
     constexpr int T = N / 16;
     const size_t row_lds = (size_t)LY::G * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
-    const size_t col_lds = (size_t)LY::B * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
-    const dim3 brow(LY::G * T), bcol(LY::B * T), g(grid);
+    const size_t col_lds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
+    const dim3 brow(LY::G * T), bcol(LY::CB * T), g(grid);
     switch (s.pass) {
         case 0:
             if (s.pre_kind == PRE_NONE)
@@ -366,30 +378,47 @@ hipError_t OC_FAST_FN(fast_launch)(const FastSel& s, const ProbDev* d, int np, i
     case NN: return launch_nv<NN, 0>(s, d, np, total, grid, st);
         OC_FAST_SIZES_1(OC_C1)
 #undef OC_C1
-#define OC_C2(NN) \
-    case NN: return s.v ? launch_nv<NN, 1>(s, d, np, total, grid, st) : launch_nv<NN, 0>(s, d, np, total, grid, st);
+#define OC_C2(NN)                                                  \
+    case NN:                                                       \
+        switch (s.v) {                                             \
+            case 1: return launch_nv<NN, 1>(s, d, np, total, grid, st); \
+            case 2: return launch_nv<NN, 2>(s, d, np, total, grid, st); \
+            case 3: return launch_nv<NN, 3>(s, d, np, total, grid, st); \
+            default: return launch_nv<NN, 0>(s, d, np, total, grid, st); \
+        }
         OC_FAST_SIZES_2(OC_C2)
 #undef OC_C2
         default: return hipErrorInvalidValue;
     }
 }
 
-// Workgroup layout of length n, variant v (G = B = 0 when there is no kernel).
-void OC_FAST_FN(fast_layout)(int n, int v, int* G, int* B) {
-    using R = Real;
+template <int N, int V>
+static void layout_nv(int* G, int* B, int* CB) {
+    using LY = FastLayout<Real, N, V>;
+    *G = LY::G;
+    *B = LY::B;
+    *CB = LY::CB;
+}
+
+// Workgroup layout of length n, variant v (all 0 when there is no kernel).
+void OC_FAST_FN(fast_layout)(int n, int v, int* G, int* B, int* CB) {
+    *G = *B = *CB = 0;
     switch (n) {
 #define OC_L1(NN) \
-    case NN: *G = FastLayout<R, NN, 0>::G; *B = FastLayout<R, NN, 0>::B; return;
+    case NN: layout_nv<NN, 0>(G, B, CB); return;
         OC_FAST_SIZES_1(OC_L1)
 #undef OC_L1
-#define OC_L2(NN)                                                   \
-    case NN:                                                        \
-        *G = v ? FastLayout<R, NN, 1>::G : FastLayout<R, NN, 0>::G; \
-        *B = v ? FastLayout<R, NN, 1>::B : FastLayout<R, NN, 0>::B; \
-        return;
+#define OC_L2(NN)                                      \
+    case NN:                                           \
+        switch (v) {                                   \
+            case 1: layout_nv<NN, 1>(G, B, CB); return; \
+            case 2: layout_nv<NN, 2>(G, B, CB); return; \
+            case 3: layout_nv<NN, 3>(G, B, CB); return; \
+            default: layout_nv<NN, 0>(G, B, CB); return; \
+        }
         OC_FAST_SIZES_2(OC_L2)
 #undef OC_L2
-        default: *G = *B = 0;
+        default: return;
     }
 }
 
@@ -400,8 +429,8 @@ hipError_t OC_FAST_FN(fast_set_lds_limits)() {
     OC_FAST_SIZES_1(OC_A1)
     OC_FAST_SIZES_2(OC_A1)
 #undef OC_A1
-#define OC_A2(NN) \
-    if ((e = attr_nv<NN, 1>())) return e;
+#define OC_A2(NN)                                                                        \
+    if ((e = attr_nv<NN, 1>()) || (e = attr_nv<NN, 2>()) || (e = attr_nv<NN, 3>())) return e;
     OC_FAST_SIZES_2(OC_A2)
 #undef OC_A2
     return hipSuccess;

@@ -36,8 +36,8 @@ hipError_t launch_copy(bool dbl, const void* in, void* out, size_t n, hipStream_
 hipError_t launch_fill_int(int* p, int v, int n, hipStream_t st);
 hipError_t fast_launch_c64(const ocd::FastSel& s, const ocd::ProbDev* d, int np, int total, int grid, hipStream_t st);
 hipError_t fast_launch_c128(const ocd::FastSel& s, const ocd::ProbDev* d, int np, int total, int grid, hipStream_t st);
-void fast_layout_c64(int n, int v, int* G, int* B);
-void fast_layout_c128(int n, int v, int* G, int* B);
+void fast_layout_c64(int n, int v, int* G, int* B, int* CB);
+void fast_layout_c128(int n, int v, int* G, int* B, int* CB);
 hipError_t fast_set_lds_limits_c64();
 hipError_t fast_set_lds_limits_c128();
 
@@ -45,16 +45,16 @@ hipError_t fast_set_lds_limits_c128();
 int fast_variants(int n) {
     switch (n) {
         case 64: case 128: case 256: case 512: return 1;
-        case 1024: case 2048: case 4096: case 8192: return 2;
+        case 1024: case 2048: case 4096: case 8192: return 4;
         default: return 0;
     }
 }
 
 // true when n has a specialised kernel; fills its workgroup layout.
-bool fast_supported(bool dbl, int n, int v, int* G, int* B) {
+bool fast_supported(bool dbl, int n, int v, int* G, int* B, int* CB) {
     if (fast_variants(n) == 0) return false;
-    if (dbl) fast_layout_c128(n, v, G, B);
-    else fast_layout_c64(n, v, G, B);
+    if (dbl) fast_layout_c128(n, v, G, B, CB);
+    else fast_layout_c64(n, v, G, B, CB);
     return *G > 0;
 }
 hipError_t launch_seed(bool dbl, const void* u, const double* dmdi, void* out, size_t n, hipStream_t st);
@@ -109,9 +109,7 @@ struct oc_context {
     std::map<std::pair<int, int>, int> fft_variant;  // (n, double?) -> fast-kernel layout variant
     int variant_for(int n, bool dbl) const {
         auto it = fft_variant.find(std::make_pair(n, dbl ? 1 : 0));
-        // Untuned default: the 512-thread layout, which keeps every FP32 kernel
-        // free of register spills.
-        if (it == fft_variant.end()) return ocl::fast_variants(n) > 1 ? 1 : 0;
+        if (it == fft_variant.end()) return 0;
         return it->second;
     }
 };
@@ -561,6 +559,7 @@ struct PassCfg {
 // the column block width B of the blocked intermediate.
 struct Layout {
     int cnt = 1, B = 1;
+    int ccnt = 1;  // columns per column-pass workgroup (divides B)
     size_t row_bytes = 0, col_bytes = 0;
     bool ok = false;
 };
@@ -578,6 +577,7 @@ Layout choose_layout(const FftPlanDev& prow, const FftPlanDev& pcol, size_t esz,
     bcap = std::max(1, std::min(bcap, 16));
     L.cnt = cnt;
     L.B = pick_pow2_divisor((size_t)prow.n, bcap);
+    L.ccnt = L.B;
     L.ok = true;
     return L;
 }
@@ -637,7 +637,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         cand.push_back(i);
     }
     // power-of-two fast path: one shared square padded size, uniform RS-ness
-    int fG = 0, fB = 0;
+    int fG = 0, fB = 0, fC = 0;
     if (!cand.empty() && !ctx->force_generic) {
         const size_t n0 = probs[cand[0]].nxp;
         bool ok = n0 >= 32 && n0 <= 8192 && (n0 & (n0 - 1)) == 0;
@@ -645,7 +645,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         for (int i : cand)
             ok = ok && probs[i].nxp == n0 && probs[i].nyp == n0 && (probs[i].engine == ENG_RS) == rs0;
         const int v = ctx->variant_for((int)n0, dbl);
-        if (ok && ocl::fast_supported(dbl, (int)n0, v, &fG, &fB)) {
+        if (ok && ocl::fast_supported(dbl, (int)n0, v, &fG, &fB, &fC)) {
             b.fast = true;
             b.fast_n = (int)n0;
             b.fast_v = v;
@@ -682,6 +682,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             const size_t slots = (size_t)b.fast_n + b.fast_n / 16;
             L.cnt = fG;
             L.B = fB;
+            L.ccnt = fC;
             L.row_bytes = slots * esz;
             L.col_bytes = slots * esz;
             L.ok = true;
@@ -696,7 +697,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         d.B = L.B;
         d.rows_cnt = L.cnt;
         d.tiles_row = (d.ny + L.cnt - 1) / L.cnt;
-        d.tiles_col = d.px / d.B;
+        d.tiles_col = d.px / L.ccnt;
         d.tiles_kgen = rs ? (d.py + L.cnt - 1) / L.cnt : 0;
         d.base_row = base_row;
         d.base_col = base_col;
@@ -723,11 +724,11 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         }
         auto rnd = [](int t) { return ((t + 63) / 64) * 64; };
         const int brow = b.fast ? (b.fast_n / 16) * L.cnt : rnd(d.prow.threads * L.cnt);
-        const int bcol = b.fast ? (b.fast_n / 16) * d.B : rnd(d.pcol.threads * d.B * tr);
+        const int bcol = b.fast ? (b.fast_n / 16) * L.ccnt : rnd(d.pcol.threads * d.B * tr);
         b.pc[0].block = std::max(b.pc[0].block, brow);
         b.pc[0].lds = std::max(b.pc[0].lds, L.row_bytes * L.cnt);
         b.pc[1].block = std::max(b.pc[1].block, bcol);
-        b.pc[1].lds = std::max(b.pc[1].lds, L.col_bytes * d.B * tr);
+        b.pc[1].lds = std::max(b.pc[1].lds, L.col_bytes * L.ccnt * tr);
         if (rs) {
             b.pc[3].block = std::max(b.pc[3].block, brow);
             b.pc[3].lds = std::max(b.pc[3].lds, L.row_bytes * L.cnt);
@@ -1770,20 +1771,21 @@ int oc_context_force_generic(oc_context* ctx, int enable) {
 }
 
 // Times every layout variant of the specialised kernels at padded length n on
-// a synthetic batch of masked ASM propagations (field n/2, padding 2) and
-// keeps the fastest.
+// a synthetic round of masked ASM then Fresnel propagations (up to 16 fields
+// of n/2 x n/2, padding 2) and keeps the fastest.
 int oc_autotune_fft(oc_context* ctx, uint64_t n, int dbl, int* variant_out, double* ms_out) {
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     const int nv = ocl::fast_variants((int)n);
-    if (ms_out) ms_out[0] = ms_out[1] = -1.0;
+    if (ms_out)
+        for (int v = 0; v < OC_FFT_VARIANTS; ++v) ms_out[v] = -1.0;
     if (nv == 0) {
         if (variant_out) *variant_out = -1;
         return OC_OK;
     }
     const size_t half = n / 2, esz = dbl ? 16 : 8;
     const size_t per = half * half;
-    const size_t nb = std::max<size_t>(1, std::min<size_t>(16, ((size_t)1 << 26) / (per * 4)));
+    const size_t nb = std::max<size_t>(1, std::min<size_t>(16, ((size_t)1 << 27) / per));
     Scoped mem(ctx);
     void *in, *out, *mv;
     int st;
@@ -1798,11 +1800,13 @@ int oc_autotune_fft(oc_context* ctx, uint64_t n, int dbl, int* variant_out, doub
     m.nx = m.ny = half;
     m.dx = m.dy = 2e-6;
     oc_band_geom g{half, half, 2e-6, 2e-6, 550e-9};
-    std::vector<Prob> probs;
-    for (size_t b = 0; b < nb; ++b)
-        probs.push_back(make_prob(&g, 5e-3, ENG_ASM, 1, 0, 2.0, 2.0, 1.0, &m,
-                                  static_cast<unsigned char*>(in) + b * per * esz,
-                                  static_cast<unsigned char*>(out) + b * per * esz));
+    std::vector<Prob> probs, probs_f;
+    for (size_t b = 0; b < nb; ++b) {
+        unsigned char* bi = static_cast<unsigned char*>(in) + b * per * esz;
+        unsigned char* bo = static_cast<unsigned char*>(out) + b * per * esz;
+        probs.push_back(make_prob(&g, 5e-3, ENG_ASM, 1, 0, 2.0, 2.0, 1.0, &m, bi, bo));
+        probs_f.push_back(make_prob(&g, 0.2, ENG_FRESNEL, 0, 0, 2.0, 2.0, 1.0, nullptr, bo, bi));
+    }
     const auto key = std::make_pair((int)n, dbl ? 1 : 0);
     const bool had = ctx->fft_variant.count(key) != 0;
     const int prev = had ? ctx->fft_variant[key] : 0;
@@ -1814,10 +1818,11 @@ int oc_autotune_fft(oc_context* ctx, uint64_t n, int dbl, int* variant_out, doub
         ctx->fft_variant[key] = v;
         double ms = 1e300;
         for (int rep = 0; rep < 4; ++rep) {
-            std::vector<Prob> pv = probs;
+            std::vector<Prob> pv = probs, pf = probs_f;
             hipEvent_t a = pool_event(ctx), e = pool_event(ctx);
             (void)hipEventRecord(a, ctx->stream);
             st = run_batch(ctx, dbl, pv);
+            if (!st) st = run_batch(ctx, dbl, pf);
             (void)hipEventRecord(e, ctx->stream);
             (void)hipEventSynchronize(e);
             float f = 0.f;

@@ -97,6 +97,9 @@ class OcError(RuntimeError):
     pass
 
 
+FFT_VARIANTS = 4  # OC_FFT_VARIANTS
+
+
 class Context:
     """oc_context: device, stream, scratch arena, plan caches."""
 
@@ -140,7 +143,7 @@ class Context:
         """Times every fast-kernel layout variant at padded length n; keeps the best.
         Returns (variant, [ms per variant])."""
         v = C.c_int()
-        ms = (c_f64 * 2)()
+        ms = (c_f64 * FFT_VARIANTS)()
         st = lib.oc_autotune_fft(self.h, n, 1 if double else 0, C.byref(v), ms)
         if st:
             raise OcError(f"autotune failed: {T.STATUS.get(st, st)} {self.last_error()}")

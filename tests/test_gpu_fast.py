@@ -59,7 +59,7 @@ def run_dev(G, ctx, engine, u, dtype, d, lam, z, cfg, mask=None):
 
 
 def variants(n):
-    return [0, 1] if n >= 1024 else [0]
+    return [0, 1, 2, 3] if n >= 1024 else [0]
 
 
 CASES = [(p, v) for p in (64, 128, 256, 512, 1024, 2048, 4096) for v in variants(p)]
@@ -111,7 +111,7 @@ def test_fast_vs_oracle(G, orc, P, v):
     ctx.synchronize()
 
 
-@pytest.mark.parametrize("v", [0, 1])
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
 def test_fast_p8192_vs_generic(G, v):
     P, n = 8192, 4096
     d, lam = 2e-6, 633e-9
@@ -163,9 +163,9 @@ def test_autotune_picks_a_variant(G):
     for P, dbl in ((2048, False), (4096, False), (1024, True)):
         v, ms = ctx.autotune(P, double=dbl)
         say("autotune", P, dbl, v, ms)
-        assert v in (0, 1)
+        assert v in (0, 1, 2, 3)
         assert ms[v] > 0 and ms[v] == min(m for m in ms if m > 0)
         assert ctx.fft_variant(P, double=dbl) == v
     v, ms = ctx.autotune(96)
-    assert v == -1 and ms == [-1.0, -1.0]
+    assert v == -1 and ms == [-1.0] * G.FFT_VARIANTS
     assert ctx.set_fft_variant(512, 1) != 0  # one layout only below 1024
