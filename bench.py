@@ -151,11 +151,15 @@ def main():
     log(f"c64: {ms_step:.3f} ms/step, {value:.1f} propagations/s; passes {timing}")
 
     # Algorithmic bytes per launch (minimal-sweep model, complex64 = 8 B):
-    # row_fwd reads N^2 and writes N P; col reads and writes N P; row_inv
-    # reads N P and writes N^2 (per band; 16 bands per launch).
+    # three-pass form (the masked ASM batch): row_fwd reads N^2 and writes
+    # N P; col reads and writes N P; row_inv reads N P and writes N^2.
+    # Two-pass separable form (the complex64 Fresnel batch): each pass reads
+    # and writes N^2.  Per band; 16 bands per launch.
     B = 8
     P = 2 * n
-    alg = {"row_fwd": nb * (n * n + n * P) * B, "col": nb * (2 * n * P) * B, "row_inv": nb * (n * P + n * n) * B}
+    alg = {"row_fwd": nb * (n * n + n * P) * B, "col": nb * (2 * n * P) * B, "row_inv": nb * (n * P + n * n) * B,
+           "sep_col": nb * (2 * n * n) * B, "sep_row": nb * (2 * n * n) * B}
+    alg = {k: v for k, v in alg.items() if timing[k][1]}
     dom = max(alg, key=lambda k: timing[k][0])
     avg_ms = timing[dom][0] / max(timing[dom][1], 1)
     achieved = alg[dom] / (avg_ms * 1e-3) / 1e9
@@ -163,7 +167,7 @@ def main():
     passes = {k: {"avg_ms": timing[k][0] / max(timing[k][1], 1), "launches": timing[k][1],
                   "alg_GBps": alg[k] / (timing[k][0] / max(timing[k][1], 1) * 1e-3) / 1e9}
               for k in alg if timing[k][1]}
-    step_bytes = 2 * nb * (2 * n * n + 4 * n * P) * B + nb * n * n * 8  # + f64 mask lattice read per ASM band
+    step_bytes = sum(alg[k] * timing[k][1] for k in alg) / args.steps + nb * n * n * 8  # + f64 mask lattice per band
     extra = {"passes": passes, "fft_variant": variant, "autotune_ms": variant_ms, "step_alg_GB": step_bytes / 1e9,
              "step_alg_GBps": step_bytes / (elapsed / args.steps) / 1e9}
 

@@ -200,8 +200,10 @@ const char* oc_version(void);
    When enabled, every pass launch is bracketed by HIP events recorded on the
    context stream; oc_context_pass_timing returns the accumulated milliseconds
    and launch counts per pass: [0] row forward, [1] column, [2] row inverse,
-   [3] RS kernel generation, [4] GS fused passes, [5] other kernels. */
-#define OC_NUM_PASS_KINDS 6
+   [3] RS kernel generation, [4] GS fused passes, [5] other kernels,
+   [6] column and [7] row pass of the two-pass (separable) complex64
+   Fresnel form. */
+#define OC_NUM_PASS_KINDS 8
 int oc_context_enable_timing(oc_context* ctx, int enable);
 int oc_context_pass_timing(oc_context* ctx, double* ms, uint64_t* launches,
                            int reset);

@@ -291,6 +291,131 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Separable complex64 Fresnel: two passes instead of three.            */
+/* ------------------------------------------------------------------ */
+// The Fresnel transfer function factors per axis (H = hx[kx] hy[ky]), and
+// embedding, transforms and cropping act on each axis separately, so
+//   out = crop_y IFFT_y hy FFT_y embed_y  .  in  .  (same along x)^T.
+// Pass A applies the y factor column by column and keeps the ny cropped
+// rows; pass B applies the x factor row by row and writes the output rows.
+// 4N transforms of length P and 4 N^2 elements of traffic per band, against
+// 6N and 2N^2 + 4NP for the three-pass form.  The intermediate is the
+// cropped N_y x N_x field, blocked [ceil(N_x/B)][N_y][B].
+
+// Pass A: CB input columns per tile, read from the natural-layout input
+// (pre-op PK: none or phase mask), written to the blocked intermediate.
+template <typename R, int N, int V, int PK>
+__global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_col(const ProbDev* __restrict__ probs,
+                                                                          int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, B = LY::B, CB = LY::CB;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int g = tid % CB, t = tid / CB;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
+        const int kx = (tile - P.base_col) * CB + g;  // input column
+        const int nx = P.nx, ny = P.ny, oy = P.oy;
+        const bool colok = kx < nx;
+        const cpx<R>* in = static_cast<const cpx<R>*>(P.in);
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int y = t + m * T;
+            v[m] = {R(0), R(0)};
+            if (colok && y >= oy && y < oy + ny) v[m] = at32(in, (unsigned)(y - oy) * nx + kx);
+        }
+        bool bad = false;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
+        if (bad) atomicOr(P.flag, 1);
+        if constexpr (PK == PRE_MASK) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int y = t + m * T;
+                if (colok && y >= oy && y < oy + ny) v[m] = apply_mask(P.mask, kx, y - oy, v[m]);
+            }
+        }
+        const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
+        fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
+        const cpx<R>* hy = static_cast<const cpx<R>*>(P.eng.hy);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            cpx<R> h;
+            h = at32(hy, t + m * T);
+            v[m] = cmul(v[m], h);
+        }
+        fft_regs<R, N, CB, +1>(v, lds, g, t, tw);
+        if (colok) {
+            cpx<R>* work = static_cast<cpx<R>*>(P.work);
+            const unsigned b0 = ((unsigned)(kx / B) * ny - oy) * B + kx % B;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int y = t + m * T;
+                if (y >= oy && y < oy + ny) at32(work, b0 + (unsigned)y * B) = v[m];
+            }
+        }
+    }
+}
+
+// Pass B: G rows per tile from the blocked intermediate (ox is a multiple
+// of B), x factor, crop, 1/(P_x P_y), natural-layout output.
+template <typename R, int N, int V>
+__global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row(const ProbDev* __restrict__ probs,
+                                                                         int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, G = LY::G, Q = LY::B;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
+    const int t = tb + Q * tq;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
+        const int iy = (tile - P.base_row) * G + g;
+        const bool rowok = iy < P.ny;
+        const int ny = P.ny, nx = P.nx, ox = P.ox;
+        // sample x = t + m T is column ix = x - ox: block (x - ox)/Q, lane tb
+        const cpx<R>* work = static_cast<const cpx<R>*>(P.work);
+        const unsigned b0 = ((unsigned)tq * ny + iy) * Q + tb - (unsigned)(ox / Q) * ny * Q;
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int x = t + m * T;
+            v[m] = {R(0), R(0)};
+            if (rowok && x >= ox && x < ox + nx) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
+        }
+        const cpx<R>* tw = static_cast<const cpx<R>*>(P.prow.tw);
+        fft_regs<R, N, G, -1>(v, lds, g, t, tw);
+        const cpx<R>* hx = static_cast<const cpx<R>*>(P.eng.hx);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            cpx<R> h;
+            h = at32(hx, t + m * T);
+            v[m] = cmul(v[m], h);
+        }
+        fft_regs<R, N, G, +1>(v, lds, g, t, tw);
+        const R s = (R)P.scale;
+        bool bad = false;
+        if (rowok) {
+            cpx<R>* out = static_cast<cpx<R>*>(P.out);
+            const unsigned o0 = (unsigned)iy * nx;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int x = t + m * T - ox;
+                if (x >= 0 && x < nx) {
+                    const cpx<R> o = {v[m].re * s, v[m].im * s};
+                    bad |= !finite_c(o);
+                    at32(out, o0 + x) = o;
+                }
+            }
+        }
+        if (bad) atomicOr(P.flag, 2);
+    }
+}
+
 }  // namespace ocd
 
 namespace ocl {
@@ -312,6 +437,22 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
     const size_t row_lds = (size_t)LY::G * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
     const size_t col_lds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
     const dim3 brow(LY::G * T), bcol(LY::CB * T), g(grid);
+    if (s.sep) {  // separable complex64 Fresnel (pass 1: columns, pass 2: rows)
+        if constexpr (sizeof(R) == 4) {
+            if (s.pass == 1) {
+                if (s.pre_kind == PRE_MASK)
+                    hipLaunchKernelGGL((f_sep_col<R, N, V, PRE_MASK>), g, bcol, col_lds, st, d, np, total);
+                else
+                    hipLaunchKernelGGL((f_sep_col<R, N, V, PRE_NONE>), g, bcol, col_lds, st, d, np, total);
+                return hipGetLastError();
+            }
+            if (s.pass == 2) {
+                hipLaunchKernelGGL((f_sep_row<R, N, V>), g, brow, row_lds, st, d, np, total);
+                return hipGetLastError();
+            }
+        }
+        return hipErrorInvalidValue;
+    }
     switch (s.pass) {
         case 0:
             if (s.pre_kind == PRE_NONE)
@@ -361,6 +502,14 @@ static hipError_t attr_nv() {
     for (const void* f : k) {
         const hipError_t e = allow_lds(f);
         if (e != hipSuccess) return e;
+    }
+    if constexpr (sizeof(R) == 4) {
+        const void* ks[] = {(const void*)f_sep_col<R, N, V, PRE_NONE>, (const void*)f_sep_col<R, N, V, PRE_MASK>,
+                            (const void*)f_sep_row<R, N, V>};
+        for (const void* f : ks) {
+            const hipError_t e = allow_lds(f);
+            if (e != hipSuccess) return e;
+        }
     }
     return hipSuccess;
 }

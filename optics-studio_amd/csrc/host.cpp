@@ -590,6 +590,7 @@ struct Batch {
     bool plain_post = true;  // no problem has a post-op: the lean row_inv kernel
     int col_kind = -2;       // engine shared by every problem (-1: mixed)
     int pre_kind = -2;       // pre-op shared by every problem (-1: mixed)
+    bool sep = false;        // separable complex64 Fresnel: two passes
     bool rs = false;
     std::vector<ProbDev> pd;
     std::vector<int> idx;  // index into the problem vector of each descriptor
@@ -652,6 +653,18 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             b.rs = rs0;
         }
     }
+    // complex64 Fresnel batches without post-ops factor per axis: two passes
+    // (the x embedding offset must fall on a block boundary of the layout)
+    if (b.fast && !dbl && !b.rs) {
+        bool sep = true;
+        for (int i : cand) {
+            const Prob& p = probs[i];
+            const size_t ox = (p.nxp - p.g.nx) / 2;
+            sep = sep && p.engine == ENG_FRESNEL && (p.ops.pre == PRE_NONE || p.ops.pre == PRE_MASK) &&
+                  p.ops.post == POST_NONE && ox % (size_t)fB == 0;
+        }
+        b.sep = sep;
+    }
     void *work = nullptr, *kbuf = nullptr;
     if (!cand.empty()) {
         int st = batch_alloc(ctx, b, persistent, "work", work_elems * esz, &work);
@@ -697,7 +710,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         d.B = L.B;
         d.rows_cnt = L.cnt;
         d.tiles_row = (d.ny + L.cnt - 1) / L.cnt;
-        d.tiles_col = d.px / L.ccnt;
+        d.tiles_col = b.sep ? (d.nx + L.ccnt - 1) / L.ccnt : d.px / L.ccnt;
         d.tiles_kgen = rs ? (d.py + L.cnt - 1) / L.cnt : 0;
         d.base_row = base_row;
         d.base_col = base_col;
@@ -746,6 +759,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     b.pc[1].total = base_col;
     b.pc[3].total = base_kgen;
     b.pc[2] = b.pc[0];
+    if (b.sep) b.pc[0].total = 0;  // pass 1: columns, pass 2: rows
     const size_t bytes = b.pd.size() * sizeof(ProbDev);
     int st = batch_alloc(ctx, b, persistent, "probs", bytes, &b.dprobs);
     if (st) return st;
@@ -762,6 +776,8 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
 }
 
 constexpr unsigned kAllPasses = 0xF;
+// timing slots of the separable Fresnel passes (see OC_NUM_PASS_KINDS)
+constexpr int PASS_T_SEP_COL = 6, PASS_T_SEP_ROW = 7;
 
 // Enqueues the passes of `b` selected by `mask` (bit i = pass i), in the
 // order kernel generation, row forward, column, row inverse.
@@ -775,10 +791,11 @@ int launch_batch(oc_context* ctx, Batch& b, unsigned mask = kAllPasses) {
         if (!(mask & (1u << pass)) || b.pc[pass].total <= 0) continue;
         const int grid = std::min(b.pc[pass].total, ctx->num_cu * 8);
         const PassCfg& c = b.pc[pass];
-        hipError_t e = timed(ctx, pass, [&] {
+        const int tpass = b.sep ? (pass == 1 ? PASS_T_SEP_COL : PASS_T_SEP_ROW) : pass;
+        hipError_t e = timed(ctx, tpass, [&] {
             if (b.fast) {
-                const ocd::FastSel sel{pass, b.fast_n, b.fast_v, b.rs ? 1 : 0, b.plain_post ? 1 : 0, b.col_kind,
-                                       b.pre_kind};
+                const ocd::FastSel sel{pass,         b.fast_n,   b.fast_v,   b.rs ? 1 : 0, b.plain_post ? 1 : 0,
+                                       b.col_kind,   b.pre_kind, b.sep ? 1 : 0};
                 return b.dbl ? ocl::fast_launch_c128(sel, dprobs, nprob, c.total, grid, ctx->stream)
                              : ocl::fast_launch_c64(sel, dprobs, nprob, c.total, grid, ctx->stream);
             }

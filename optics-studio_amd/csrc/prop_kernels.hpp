@@ -76,6 +76,7 @@ struct FastSel {
     int plain_post;  // no problem has a post-op
     int col_kind;    // ENG_ASM / ENG_FRESNEL when uniform, else -1
     int pre_kind;    // PRE_NONE / PRE_MASK when uniform, else -1
+    int sep;         // separable complex64 Fresnel (two passes)
 };
 
 // Parameters of the fused element-wise ops (gs.rs:484-564, adjoint.rs:200-236,
@@ -118,6 +119,7 @@ struct ProbDev {
     const void* in;
     void* out;
     void* work;  // N_y x P_x intermediate, blocked [P_x/B][N_y][B]
+                 // (separable Fresnel: N_y x N_x, blocked [ceil(N_x/B)][N_y][B])
     void* kbuf;  // RS kernel row spectra, blocked [P_x/B][P_y][B]
     double scale;  // 1/(P_x P_y)
     MaskDev mask;

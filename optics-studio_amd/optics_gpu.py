@@ -132,12 +132,12 @@ class Context:
         lib.oc_context_enable_timing(self.h, 1 if on else 0)
 
     def pass_timing(self, reset=True):
-        """{pass: (total_ms, launches)} for row_fwd, col, row_inv, rs_kgen, gs, other."""
-        ms = (c_f64 * 6)()
-        n = (c_u64 * 6)()
+        """{pass: (total_ms, launches)} for row_fwd, col, row_inv, rs_kgen, gs, other, sep_col, sep_row."""
+        ms = (c_f64 * 8)()
+        n = (c_u64 * 8)()
         lib.oc_context_pass_timing(self.h, ms, n, 1 if reset else 0)
-        names = ["row_fwd", "col", "row_inv", "rs_kgen", "gs", "other"]
-        return {names[i]: (ms[i], n[i]) for i in range(6)}
+        names = ["row_fwd", "col", "row_inv", "rs_kgen", "gs", "other", "sep_col", "sep_row"]
+        return {names[i]: (ms[i], n[i]) for i in range(8)}
 
     def autotune(self, n, double=False):
         """Times every fast-kernel layout variant at padded length n; keeps the best.

@@ -169,3 +169,53 @@ def test_autotune_picks_a_variant(G):
     v, ms = ctx.autotune(96)
     assert v == -1 and ms == [-1.0] * G.FFT_VARIANTS
     assert ctx.set_fft_variant(512, 1) != 0  # one layout only below 1024
+
+
+SEP_CASES = [(1024, 2.0), (1000, 2.048), (600, 3.4133), (777, 2.6357)]
+
+
+@pytest.mark.parametrize("n,pad", SEP_CASES, ids=[f"n{n}-pad{p}" for n, p in SEP_CASES])
+def test_fresnel_c64_two_pass_and_fallback(G, orc, n, pad):
+    """complex64 Fresnel runs the two-pass separable form when the x offset
+    of the embedding falls on a block boundary of the layout (n = 1024:
+    offset 512, always) and the three-pass form otherwise (n = 777: odd
+    offset 635, always; 1000 and 600 depend on the layout's block width);
+    both forms, masked and not, in every layout, against the oracle."""
+    import math
+    d, lam = 3e-6, 600e-9
+    P = math.ceil(n * pad)
+    assert P == 2048
+    u = field(n, n)
+    rng = np.random.default_rng(n + 1)
+    phi = rng.uniform(0, 2 * np.pi, (n // 2, n // 2))
+    z = 1.2 * n * d * d / lam
+    cfg = T.FresnelConfig(pad, 0, 0, pad)
+    st, ref, _ = orc.fresnel_propagate(u, d, d, lam, z, cfg)
+    assert st == 0
+    om = orc.Mask(phi, 2 * d, 2 * d)
+    st, um = orc.phasemask_apply(u, d, d, lam, om)
+    assert st == 0
+    st, ref_m, _ = orc.fresnel_propagate(um, d, d, lam, z, cfg)
+    assert st == 0
+    mask, keep = G.dev_mask(phi, 2 * d, 2 * d)
+    import torch
+    t = torch.as_tensor(u.astype(np.complex64)).cuda()
+    for v in variants(P):
+        ctx = G.Context(0)
+        assert ctx.set_fft_variant(P, v) == 0
+        ctx.enable_timing(True)
+        st, out, _ = G.fresnel_propagate_dev(t, d, d, lam, z, cfg, ctx=ctx)
+        assert st == 0
+        st, out_m, _ = G.fresnel_propagate_dev(t, d, d, lam, z, cfg, mask=mask, ctx=ctx)
+        assert st == 0
+        timing = ctx.pass_timing()
+        e = rel_l2(out.cpu().numpy().astype(np.complex128), ref)
+        em = rel_l2(out_m.cpu().numpy().astype(np.complex128), ref_m)
+        two = timing["sep_col"][1] > 0
+        say("   variant", v, "two-pass" if two else "three-pass", e, em)
+        assert e <= C64_TOL and em <= C64_TOL
+        if n == 777:
+            assert not two
+        if n == 1024:
+            assert two
+        ctx.close()
