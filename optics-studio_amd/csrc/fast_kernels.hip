@@ -54,7 +54,7 @@ struct FastLayout {
     static_assert(G >= 1 && B >= 1 && CB >= 1 && (T % B) == 0 && (B % CB) == 0, "layout");
 };
 
-// PK: pre-op (PRE_NONE, PRE_MASK, or -1 for the descriptor's).
+// PK: pre-op (PRE_NONE, PRE_MASK, PRE_TMUL, or -1 for the descriptor's).
 template <typename R, int N, int V, int PK>
 __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd(const ProbDev* __restrict__ probs,
                                                                          int nprob, int total) {
@@ -73,7 +73,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
         bool bad = false;
         double acc = 0.0;
         cpx<R> v[16];
-        if constexpr (PK == PRE_NONE || PK == PRE_MASK) {
+        if constexpr (PK == PRE_NONE || PK == PRE_MASK || PK == PRE_TMUL) {
             // the row's 16 samples as one batch of loads, then the checks,
             // then the mask with its mode decided once for the row
             const cpx<R>* in = static_cast<const cpx<R>*>(P.in);
@@ -86,6 +86,18 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
             }
 #pragma unroll
             for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
+            if constexpr (PK == PRE_TMUL) {
+                const cpx<R>* tr = static_cast<const cpx<R>*>(P.ops.trans);
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                    const int x = t + m * T;
+                    if (rowok && x >= ox && x < ox + nx) {
+                        cpx<R> w;
+                        w = at32(tr, r0 + x);
+                        v[m] = cmul(v[m], w);
+                    }
+                }
+            }
             if constexpr (PK == PRE_MASK) {
                 const MaskDev& mk = P.mask;
                 if (mk.mode == MASK_SEPARABLE) {
@@ -338,6 +350,18 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
                 if (colok && y >= oy && y < oy + ny) v[m] = apply_mask(P.mask, kx, y - oy, v[m]);
             }
         }
+        if constexpr (PK == PRE_TMUL) {
+            const cpx<R>* tr = static_cast<const cpx<R>*>(P.ops.trans);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int y = t + m * T;
+                if (colok && y >= oy && y < oy + ny) {
+                    cpx<R> w;
+                    w = at32(tr, (unsigned)(y - oy) * nx + kx);
+                    v[m] = cmul(v[m], w);
+                }
+            }
+        }
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
         fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
         const cpx<R>* hy = static_cast<const cpx<R>*>(P.eng.hy);
@@ -442,6 +466,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
             if (s.pass == 1) {
                 if (s.pre_kind == PRE_MASK)
                     hipLaunchKernelGGL((f_sep_col<R, N, V, PRE_MASK>), g, bcol, col_lds, st, d, np, total);
+                else if (s.pre_kind == PRE_TMUL)
+                    hipLaunchKernelGGL((f_sep_col<R, N, V, PRE_TMUL>), g, bcol, col_lds, st, d, np, total);
                 else
                     hipLaunchKernelGGL((f_sep_col<R, N, V, PRE_NONE>), g, bcol, col_lds, st, d, np, total);
                 return hipGetLastError();
@@ -459,6 +485,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_NONE>), g, brow, row_lds, st, d, np, total);
             else if (s.pre_kind == PRE_MASK)
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_MASK>), g, brow, row_lds, st, d, np, total);
+            else if (s.pre_kind == PRE_TMUL)
+                hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_TMUL>), g, brow, row_lds, st, d, np, total);
             else
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, -1>), g, brow, row_lds, st, d, np, total);
             break;
@@ -491,6 +519,7 @@ static hipError_t attr_nv() {
     using R = Real;
     const void* k[] = {(const void*)f_row_fwd<R, N, V, PRE_NONE>,
                        (const void*)f_row_fwd<R, N, V, PRE_MASK>,
+                       (const void*)f_row_fwd<R, N, V, PRE_TMUL>,
                        (const void*)f_row_fwd<R, N, V, -1>,
                        (const void*)f_col<R, N, V, true, -1>,
                        (const void*)f_col<R, N, V, false, ENG_ASM>,
@@ -505,7 +534,7 @@ static hipError_t attr_nv() {
     }
     if constexpr (sizeof(R) == 4) {
         const void* ks[] = {(const void*)f_sep_col<R, N, V, PRE_NONE>, (const void*)f_sep_col<R, N, V, PRE_MASK>,
-                            (const void*)f_sep_row<R, N, V>};
+                            (const void*)f_sep_col<R, N, V, PRE_TMUL>, (const void*)f_sep_row<R, N, V>};
         for (const void* f : ks) {
             const hipError_t e = allow_lds(f);
             if (e != hipSuccess) return e;

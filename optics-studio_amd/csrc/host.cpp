@@ -29,6 +29,7 @@ hipError_t launch_pass_c128(int pass, const ocd::ProbDev* d, int np, int total, 
 hipError_t set_lds_limits();
 hipError_t launch_mask_apply_c64(const void* in, void* out, const ocd::MaskDev& m, int nx, int ny, int* flag,
                                  hipStream_t st);
+hipError_t launch_mask_trans(bool dbl, void* out, const ocd::MaskDev& m, int nx, int ny, hipStream_t st);
 hipError_t launch_mask_apply_c128(const void* in, void* out, const ocd::MaskDev& m, int nx, int ny, int* flag,
                                   hipStream_t st);
 hipError_t launch_check_finite(bool dbl, const void* in, size_t n, int* flag, hipStream_t st);
@@ -674,6 +675,19 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             if (st) return st;
         }
     }
+    // Phase masks shared by several problems (the bands of one field): the
+    // transmission does not depend on the wavelength, so it is evaluated
+    // once into a buffer and the first pass multiplies by it.
+    std::map<std::string, int> mask_uses;
+    auto mask_key = [](const Prob& p) {
+        const oc_phase_mask& m = *p.mask;
+        return fmtkey("tm", {(double)(uintptr_t)m.values, (double)m.nx, (double)m.ny, m.dx, m.dy, m.orientation,
+                             (double)m.aperture.kind, m.aperture.a, m.aperture.b, m.cx, m.cy, (double)p.g.nx,
+                             (double)p.g.ny, p.g.dx, p.g.dy});
+    };
+    for (int i : cand)
+        if (probs[i].mask && probs[i].ops.pre == PRE_MASK) mask_uses[mask_key(probs[i])] += 1;
+    std::map<std::string, void*> trans;
     int base_row = 0, base_col = 0, base_kgen = 0;
     for (size_t a = 0; a < cand.size(); ++a) {
         Prob& p = probs[cand[a]];
@@ -734,6 +748,25 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         if (p.mask) {
             st = mask_tables(ctx, p.g, *p.mask, p.mask->values, &d.mask);
             if (st) return st;
+            if (d.ops.pre == PRE_MASK) {
+                const std::string key = mask_key(p);
+                if (mask_uses[key] > 1) {
+                    auto it = trans.find(key);
+                    if (it == trans.end()) {
+                        void* tb = nullptr;
+                        const std::string name = "trans" + std::to_string(trans.size());
+                        st = batch_alloc(ctx, b, persistent, name.c_str(), p.g.nx * p.g.ny * esz, &tb);
+                        if (st) return st;
+                        HIP_TRY(ctx, timed(ctx, 5, [&] {
+                                    return ocl::launch_mask_trans(dbl, tb, d.mask, (int)p.g.nx, (int)p.g.ny,
+                                                                  ctx->stream);
+                                }));
+                        it = trans.emplace(key, tb).first;
+                    }
+                    d.ops.pre = PRE_TMUL;
+                    d.ops.trans = it->second;
+                }
+            }
         }
         auto rnd = [](int t) { return ((t + 63) / 64) * 64; };
         const int brow = b.fast ? (b.fast_n / 16) * L.cnt : rnd(d.prow.threads * L.cnt);
@@ -749,7 +782,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         if (d.ops.post != POST_NONE) b.plain_post = false;
         const int ek = (p.engine == ENG_ASM || p.engine == ENG_FRESNEL) ? p.engine : -1;
         b.col_kind = b.col_kind == -2 ? ek : (b.col_kind == ek ? ek : -1);
-        const int pk = (d.ops.pre == PRE_NONE || d.ops.pre == PRE_MASK) ? d.ops.pre : -1;
+        const int pk = (d.ops.pre == PRE_NONE || d.ops.pre == PRE_MASK || d.ops.pre == PRE_TMUL) ? d.ops.pre : -1;
         b.pre_kind = b.pre_kind == -2 ? pk : (b.pre_kind == pk ? pk : -1);
         b.pd.push_back(d);
         b.idx.push_back(cand[a]);

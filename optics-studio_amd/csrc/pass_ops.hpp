@@ -77,11 +77,14 @@ __device__ __forceinline__ cpx<double> unit_phase<double>(double th) {
     return quadrant<double>(s, c, q);
 }
 
-// FP32-bulk path: FP64 argument and reduction, FP32 polynomials.
+// FP32-bulk path: FP64 argument and reduction, FP32 polynomials.  The
+// reduction keeps two of the three parts of pi/2: the third contributes
+// n * 2.0e-21 rad, below 1e-14 rad for |th| < 1e7.
 template <>
 __device__ __forceinline__ cpx<float> unit_phase<float>(double th) {
-    int q;
-    const float x = (float)reduce_pio2(th, &q);
+    const double n = rint(th * kInvPio2);
+    const int q = (int)n & 3;
+    const float x = (float)fma(-n, kPio2_2, fma(-n, kPio2_1, th));
     const float x2 = x * x;
     // Taylor terms through x^11 (sin) and x^10 (cos): truncation below
     // 1.2e-10 on |x| <= pi/4, so only FP32 rounding remains.
@@ -274,8 +277,8 @@ __device__ __forceinline__ void accumulate4(double a0, double a1, double a2, dou
 }
 
 // First-pass load of input sample (ix, iy) with the problem's pre-op.
-// PK: the pre-op when known at compile time (PRE_NONE / PRE_MASK), else -1
-// and the descriptor's op decides.
+// PK: the pre-op when known at compile time (PRE_NONE / PRE_MASK /
+// PRE_TMUL), else -1 and the descriptor's op decides.
 template <typename R, int PK = -1>
 __device__ __forceinline__ cpx<R> load_pre(const ProbDev& P, int ix, int iy, bool& bad, double& acc) {
     const size_t k = (size_t)iy * P.nx + ix;
@@ -292,6 +295,10 @@ __device__ __forceinline__ cpx<R> load_pre(const ProbDev& P, int ix, int iy, boo
     bad |= !finite_c(v);
     if (pre == PRE_MASK) {
         v = apply_mask(P.mask, ix, iy, v);
+    } else if (pre == PRE_TMUL) {
+        cpx<R> t;
+        t = gp(static_cast<const cpx<R>*>(o.trans))[k];
+        v = cmul(v, t);
     } else if (pre == PRE_SEED_MSE) {  // w = (dM/dI) U, dM/dI = 2 (I - T)/n (merit.rs:143-165)
         const double re = v.re, im = v.im;
         const double d = (re * re + im * im) - gp(o.seed_target)[k];

@@ -250,6 +250,16 @@ __global__ void k_mask_apply(const cpx<R>* __restrict__ in, cpx<R>* __restrict__
     }
 }
 
+// Transmission of a phase mask over the field grid (the mask applied to 1).
+template <typename R>
+__global__ void k_mask_trans(cpx<R>* __restrict__ out, MaskDev m, int nx, int ny) {
+    const size_t n = (size_t)nx * ny;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const int iy = (int)(i / nx), ix = (int)(i - (size_t)iy * nx);
+        out[i] = apply_mask(m, ix, iy, cpx<R>{R(1), R(0)});
+    }
+}
+
 template <typename R>
 __global__ void k_check_finite(const cpx<R>* __restrict__ in, size_t n, int* flag) {
     bool bad = false;
@@ -388,6 +398,19 @@ hipError_t launch_mask_apply_c64(const void* in, void* out, const MaskDev& m, in
                        static_cast<cpx<float>*>(out), m, nx, ny, flag);
     return hipGetLastError();
 }
+hipError_t launch_mask_trans(bool dbl, void* out, const MaskDev& m, int nx, int ny, hipStream_t st) {
+    const size_t n = (size_t)nx * ny;
+    int grid = (int)std::min<size_t>((n + 255) / 256, 65536);
+    if (grid < 1) grid = 1;
+    if (dbl)
+        hipLaunchKernelGGL(k_mask_trans<double>, dim3(grid), dim3(256), 0, st, static_cast<cpx<double>*>(out), m, nx,
+                           ny);
+    else
+        hipLaunchKernelGGL(k_mask_trans<float>, dim3(grid), dim3(256), 0, st, static_cast<cpx<float>*>(out), m, nx,
+                           ny);
+    return hipGetLastError();
+}
+
 hipError_t launch_mask_apply_c128(const void* in, void* out, const MaskDev& m, int nx, int ny, int* flag,
                                   hipStream_t st) {
     const size_t n = (size_t)nx * ny;

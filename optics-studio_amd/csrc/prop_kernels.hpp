@@ -16,7 +16,10 @@ enum EngineKind : int {
     ENG_MEASURE = 5      // forward only: accumulate intensity statistics
 };
 // Element-wise op fused into the first pass's loads.
-enum PreOp : int { PRE_NONE = 0, PRE_MASK = 1, PRE_POLAR = 2, PRE_SEED_MSE = 3 };
+// PRE_TMUL multiplies by a precomputed transmission (a phase mask shared by
+// several problems of a launch, evaluated once: it does not depend on the
+// wavelength).
+enum PreOp : int { PRE_NONE = 0, PRE_MASK = 1, PRE_POLAR = 2, PRE_SEED_MSE = 3, PRE_TMUL = 4 };
 // Element-wise op fused into the last pass's (cropped) stores.
 enum PostOp : int { POST_NONE = 0, POST_GS_TARGET = 1, POST_GS_PHASE = 2, POST_ADJ_GRAD = 3 };
 enum MaskMode : int { MASK_NONE = 0, MASK_SEPARABLE = 1, MASK_GENERAL = 2 };
@@ -75,7 +78,7 @@ struct FastSel {
     int rs;          // Rayleigh-Sommerfeld batch
     int plain_post;  // no problem has a post-op
     int col_kind;    // ENG_ASM / ENG_FRESNEL when uniform, else -1
-    int pre_kind;    // PRE_NONE / PRE_MASK when uniform, else -1
+    int pre_kind;    // PRE_NONE / PRE_MASK / PRE_TMUL when uniform, else -1
     int sep;         // separable complex64 Fresnel (two passes)
 };
 
@@ -88,6 +91,8 @@ struct OpsDev {
     const void* amp;
     const void* phase;
     const unsigned char* open;
+    // PRE_TMUL: v = u * trans[k]   (element type R, field layout)
+    const void* trans;
     // PRE_SEED_MSE: v = (2 (|u|^2 - target)/n) u   (target f64, n = frame size)
     const double* seed_target;
     double n_frame;  // frame size n as f64 (divisor)

@@ -219,3 +219,43 @@ def test_fresnel_c64_two_pass_and_fallback(G, orc, n, pad):
         if n == 1024:
             assert two
         ctx.close()
+
+
+@pytest.mark.parametrize("engine", [T.ENGINE_ASM, T.ENGINE_FRESNEL])
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_shared_mask_batch_equals_single(G, engine, dtype):
+    """Bands of one field share its phase mask: the batch evaluates the
+    transmission once and multiplies by it.  That product is the same
+    arithmetic as the per-problem fused mask, so every band must equal its
+    single-problem result bit for bit."""
+    import torch
+    n, d = 512, 2e-6
+    lams = [450e-9, 532e-9, 633e-9]
+    rng = np.random.default_rng(5)
+    phi = rng.uniform(0, 2 * np.pi, (n, n))
+    mask, keep = G.dev_mask(phi, d, d, aperture=(T.APERTURE_CIRCLE, 0.45 * n * d, 0.0))
+    cdt = torch.complex64 if dtype == "c64" else torch.complex128
+    u = torch.as_tensor(field(n, 9)).to(cdt).cuda()
+    outs = [torch.empty_like(u) for _ in lams]
+    arr = (T.PropDesc * len(lams))()
+    for b, lam in enumerate(lams):
+        x = arr[b]
+        x.geom = T.geom(n, n, d, d, lam)
+        x.engine = engine
+        x.z, x.padding, x.band_limited, x.strict, x.max_padding, x.medium = 2e-3, 2.0, 1, 0, 2.0, 1.0
+        x.mask = T.C.pointer(mask)
+        x.in_ = u.data_ptr()
+        x.out = outs[b].data_ptr()
+    ctx = G.Context(0)
+    ctx.enable_timing(True)
+    assert G.propagate_batch_dev(arr, double=dtype == "c128", ctx=ctx) == 0
+    timing = ctx.pass_timing()
+    assert timing["other"][1] == 1  # one transmission evaluation for the three bands
+    for b, lam in enumerate(lams):
+        cfg = T.AsmConfig(2.0, 1, 0, 2.0, 1.0) if engine == T.ENGINE_ASM else T.FresnelConfig(2.0, 0, 0, 2.0)
+        fn = G.asm_propagate_dev if engine == T.ENGINE_ASM else G.fresnel_propagate_dev
+        st, single, _ = fn(u, d, d, lam, 2e-3, cfg, mask=mask, ctx=ctx)
+        assert st == 0
+        say("   band", b, dtype, "max |diff|", float((outs[b] - single).abs().max()))
+        assert torch.equal(outs[b], single)
+    ctx.close()
