@@ -61,6 +61,139 @@ def copy_calibration():
                       "launches": 8}), flush=True)
 
 
+def bench_gs(G, W, torch, np):
+    """Workload 3: FFT-form GS at 1024^2, complex64, 32 restarts x 200
+    iterations in one batched launch sequence, hipGraph-captured in chunks."""
+    c = W.CFG3
+    amp, tgt, ph = W.workload3()
+    tgt = np.fft.ifftshift(tgt)  # Fourier plane in DFT order
+    a = torch.as_tensor(amp.astype(np.float32)).cuda()
+    t = torch.as_tensor(tgt.astype(np.float32)).cuda()
+    p0 = torch.as_tensor(ph.astype(np.float32)).cuda()
+    chunk = 20
+    st, _ = G.gs_fft_run_dev(a, t, p0.clone(), chunk, chunk)  # warm-up (plans, graph)
+    if st:
+        raise RuntimeError(f"gs warm-up failed: {st}")
+    p = p0.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, err = G.gs_fft_run_dev(a, t, p, c["iterations"], chunk)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    if st:
+        raise RuntimeError(f"gs failed: {st}")
+    its = c["restarts"] * c["iterations"]
+    r = {"workload": f"GS FFT form {c['n']}^2 complex64, {c['restarts']} restarts x {c['iterations']} iterations, "
+                     f"graph chunk {chunk}",
+         "iterations_per_s": its / (t1 - t0), "seconds": t1 - t0,
+         "final_error_mean": float(np.mean(err)), "final_error_best": float(np.min(err))}
+    log(f"gs: {r['iterations_per_s']:.0f} restart-iterations/s, error mean {r['final_error_mean']:.4f} "
+        f"best {r['final_error_best']:.4f}")
+    return r
+
+
+def bench_adjoint(G, W, T, torch, np, cpu=True):
+    """Workload 4: adjoint gradient of the intensity MSE w.r.t. both DOEs of
+    DOE1 -> ASM 10 mm -> DOE2 -> ASM 10 mm at 1024^2 (f64, per-plane
+    checkpoints), against central FD (1e-4 rad) on 256 seeded pixels and,
+    with the CPU leg, against the f64 oracle adjoint."""
+    c = W.CFG4
+    n, d, lam, z = c["n"], c["d"], c["lam"], c["z"]
+    u, p1, p2, tgt = W.workload4()
+    cfg = T.AsmConfig.default()
+    m1, k1 = G.dev_mask(p1, d, d)
+    m2, k2 = G.dev_mask(p2, d, d)
+    ud = torch.as_tensor(u).cuda()
+    td = torch.as_tensor(tgt).cuda()
+    grads = torch.zeros((2, n, n), dtype=torch.float64, device="cuda")
+    steps = [(m1, z, 1.0), (m2, z, 1.0)]
+    st, merit = G.adjoint_grid_gradient_dev(steps, ud, d, d, lam, cfg, td, grads)
+    if st:
+        raise RuntimeError(f"adjoint failed: {st}")
+    k = 10
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        G.adjoint_grid_gradient_dev(steps, ud, d, d, lam, cfg, td, grads)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    g = grads.cpu().numpy()
+    # central finite differences on 2 x 128 seeded pixels (merit-only calls)
+    h = c["fd_step"]
+    px1, px2 = W.workload4_fd_pixels()
+    fd, ad = [], []
+    f0 = time.perf_counter()
+    for plane, pix in ((0, px1), (1, px2)):
+        base = torch.as_tensor(p1 if plane == 0 else p2).cuda()
+        for kk in pix:
+            iy, ix = divmod(int(kk), n)
+            mp = []
+            for sgn in (1.0, -1.0):
+                vv = base.clone()
+                vv[iy, ix] += sgn * h
+                mm = G.dev_mask_from_tensor(vv, d, d)
+                st_ = [(mm, z, 1.0), (m2, z, 1.0)] if plane == 0 else [(m1, z, 1.0), (mm, z, 1.0)]
+                st, mv = G.adjoint_grid_gradient_dev(st_, ud, d, d, lam, cfg, td, None)
+                mp.append(mv)
+            fd.append((mp[0] - mp[1]) / (2 * h))
+            ad.append(g[plane, iy, ix])
+    f1 = time.perf_counter()
+    fd, ad = np.array(fd), np.array(ad)
+    r = {"workload": f"adjoint grid gradient {n}^2 f64, 2 DOEs + 2 ASM {z * 1e3:.0f} mm, MSE to a 16-spot target",
+         "gradients_per_s": k / (t1 - t0), "ms_per_gradient": 1e3 * (t1 - t0) / k, "merit": merit,
+         "fd_rel_l2": float(np.linalg.norm(fd - ad) / np.linalg.norm(fd)),
+         "fd_max_rel": float(np.max(np.abs(fd - ad) / np.maximum(np.abs(fd), 1e-30))),
+         "fd_pixels": int(fd.size), "fd_seconds": f1 - f0}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle
+        so = [(oracle.Mask(p1, d, d), z, 1.0), (oracle.Mask(p2, d, d), z, 1.0)]
+        c0 = time.perf_counter()
+        st, det = oracle.trace(so, u, d, d, lam, cfg)
+        inten = det.real ** 2 + det.imag ** 2
+        st2, ref = oracle.adjoint_push_back(so, u, d, d, lam, cfg, 2.0 * (inten - tgt) / inten.size)
+        c1 = time.perf_counter()
+        r["vs_f64_oracle_rel_l2"] = [float(np.linalg.norm(g[e] - ref[e]) / np.linalg.norm(ref[e])) for e in range(2)]
+        r["cpu_baseline"] = {"gradients_per_s": 1.0 / (c1 - c0), "cores": 1, "kind": "port",
+                             "sample": f"one gradient (trace + push_back) with the f64 oracle, {c1 - c0:.1f} s"}
+    log(f"adjoint: {r['gradients_per_s']:.1f} gradients/s, FD rel L2 {r['fd_rel_l2']:.2e}"
+        + (f", vs oracle {r['vs_f64_oracle_rel_l2']}" if cpu else ""))
+    return r
+
+
+def bench_rs(G, W, T, torch, np):
+    """Workload 5: 64 uneven Rayleigh-Sommerfeld problems (sizes 256..2048,
+    padding 2) in one batched call, complex64."""
+    ps = W.workload5_params()
+    ins = [torch.as_tensor(W.workload5_field(p, index=i).astype(np.complex64)).cuda() for i, p in enumerate(ps)]
+    outs = [torch.empty_like(x) for x in ins]
+    arr = (T.PropDesc * len(ps))()
+    for i, p in enumerate(ps):
+        arr[i].geom = T.geom(p["n"], p["n"], p["d"], p["d"], p["lam"])
+        arr[i].z = p["z"]
+        arr[i].engine = T.ENGINE_RS
+        arr[i].padding = 2.0
+        arr[i].strict = 0
+        arr[i].in_ = ins[i].data_ptr()
+        arr[i].out = outs[i].data_ptr()
+    st = G.propagate_batch_dev(arr)
+    if st:
+        raise RuntimeError(f"rs batch failed: {st}")
+    k = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        G.propagate_batch_dev(arr)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    px = sum(p["n"] ** 2 for p in ps)
+    r = {"workload": "64 uneven RS problems (256..2048, padding 2), complex64, one call",
+         "problems_per_s": len(ps) * k / (t1 - t0), "ms_per_batch": 1e3 * (t1 - t0) / k,
+         "field_samples": px, "sizes": sorted({p["n"] for p in ps})}
+    log(f"rs batch: {r['ms_per_batch']:.2f} ms per 64 problems")
+    return r
+
+
 def read_traffic():
     """HBM bytes per launch of the column pass from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
@@ -224,6 +357,10 @@ def main():
         extra["f64_host_dropin_propagations_per_s"] = 3 / (h1 - h0)
         log(f"c128 device {extra['c128_device_propagations_per_s']:.1f}/s, f64 host drop-in "
             f"{extra['f64_host_dropin_propagations_per_s']:.2f}/s")
+        del u_in, u_asm, u_fr
+        extra["gs"] = bench_gs(G, W, torch, np)
+        extra["adjoint"] = bench_adjoint(G, W, T, torch, np, cpu=not args.no_cpu)
+        extra["rs_batch"] = bench_rs(G, W, T, torch, np)
 
     out = {
         "metric": METRIC,

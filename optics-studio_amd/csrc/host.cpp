@@ -96,6 +96,7 @@ struct oc_context {
     size_t tables_bytes = 0;
     void* pinned = nullptr;  // host staging for descriptors / flags
     size_t pinned_cap = 0;
+    hipEvent_t pinned_done = nullptr;  // recorded after the last copy out of `pinned`
     // per-pass timing
     bool timing = false;
     struct Pending {
@@ -619,13 +620,21 @@ int batch_alloc(oc_context* ctx, Batch& b, bool persistent, const std::string& n
 // Descriptors for every prepared problem with launch == true.  A persistent
 // batch owns its intermediates and descriptor array (reusable across calls,
 // capturable in a graph); a transient one uses the context's shared buffers.
-int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags, Batch& b, bool persistent) {
+// `subset` (optional) restricts the batch to those problem indices; `tag`
+// names the transient buffers so that several batches can be in flight.
+int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags, Batch& b, bool persistent,
+                const std::vector<int>* subset = nullptr, const std::string& tag = "") {
     const size_t esz = dbl ? 16 : 8;
     b.dbl = dbl;
     size_t work_elems = 0, kbuf_elems = 0;
     std::vector<size_t> woff, koff;
     std::vector<int> cand;
-    for (int i = 0; i < (int)probs.size(); ++i) {
+    std::vector<int> all;
+    if (!subset) {
+        for (int i = 0; i < (int)probs.size(); ++i) all.push_back(i);
+        subset = &all;
+    }
+    for (int i : *subset) {
         Prob& p = probs[i];
         if (p.status != OC_OK || !p.launch) continue;
         if (p.nxp > (1u << 20) || p.nyp > (1u << 20) || p.g.nx > (1u << 20) || p.g.ny > (1u << 20)) {
@@ -668,10 +677,10 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     }
     void *work = nullptr, *kbuf = nullptr;
     if (!cand.empty()) {
-        int st = batch_alloc(ctx, b, persistent, "work", work_elems * esz, &work);
+        int st = batch_alloc(ctx, b, persistent, "work" + tag, work_elems * esz, &work);
         if (st) return st;
         if (kbuf_elems) {
-            st = batch_alloc(ctx, b, persistent, "kbuf", kbuf_elems * esz, &kbuf);
+            st = batch_alloc(ctx, b, persistent, "kbuf" + tag, kbuf_elems * esz, &kbuf);
             if (st) return st;
         }
     }
@@ -754,7 +763,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
                     auto it = trans.find(key);
                     if (it == trans.end()) {
                         void* tb = nullptr;
-                        const std::string name = "trans" + std::to_string(trans.size());
+                        const std::string name = "trans" + tag + "_" + std::to_string(trans.size());
                         st = batch_alloc(ctx, b, persistent, name.c_str(), p.g.nx * p.g.ny * esz, &tb);
                         if (st) return st;
                         HIP_TRY(ctx, timed(ctx, 5, [&] {
@@ -794,16 +803,20 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     b.pc[2] = b.pc[0];
     if (b.sep) b.pc[0].total = 0;  // pass 1: columns, pass 2: rows
     const size_t bytes = b.pd.size() * sizeof(ProbDev);
-    int st = batch_alloc(ctx, b, persistent, "probs", bytes, &b.dprobs);
+    int st = batch_alloc(ctx, b, persistent, "probs" + tag, bytes, &b.dprobs);
     if (st) return st;
     if (persistent) {
         HIP_TRY(ctx, hipMemcpy(b.dprobs, b.pd.data(), bytes, hipMemcpyHostToDevice));
     } else {
         st = ensure_pinned(ctx, bytes);
         if (st) return st;
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));  // staging reuse
+        // the staging buffer is free once the previous copy out of it ran
+        // (not the kernels queued after that copy)
+        if (ctx->pinned_done) HIP_TRY(ctx, hipEventSynchronize(ctx->pinned_done));
+        else HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->pinned_done, hipEventDisableTiming));
         std::memcpy(ctx->pinned, b.pd.data(), bytes);
         HIP_TRY(ctx, hipMemcpyAsync(b.dprobs, ctx->pinned, bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->pinned_done, ctx->stream));
     }
     return OC_OK;
 }
@@ -875,11 +888,24 @@ int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs) {
         HIP_TRY(ctx, ocl::launch_check_finite(dbl, p.in, n, dflags + i, ctx->stream));
         if (p.pending == OC_OK && p.out != p.in) HIP_TRY(ctx, ocl::launch_copy(dbl, p.in, p.out, n, ctx->stream));
     }
-    Batch b;
-    int st = build_batch(ctx, dbl, probs, dflags, b, false);
-    if (st) return st;
-    st = launch_batch(ctx, b);
-    if (st) return st;
+    // One batch per padded size that has specialised kernels (uneven grids
+    // then each get the power-of-two kernels), one for all other sizes.
+    std::map<std::pair<size_t, size_t>, std::vector<int>> groups;
+    for (int i = 0; i < np; ++i) {
+        const Prob& p = probs[i];
+        if (p.status != OC_OK || !p.launch) continue;
+        const bool fast = !ctx->force_generic && p.nxp == p.nyp && ocl::fast_variants((int)p.nxp) > 0;
+        groups[fast ? std::make_pair(p.nxp, p.nyp) : std::make_pair((size_t)0, (size_t)0)].push_back(i);
+    }
+    int gi = 0;
+    for (auto& kv : groups) {
+        Batch b;
+        int st = build_batch(ctx, dbl, probs, dflags, b, false, &kv.second, groups.size() > 1 ? std::to_string(gi) : "");
+        if (st) return st;
+        st = launch_batch(ctx, b);
+        if (st) return st;
+        ++gi;
+    }
     std::vector<int> hflags(std::max(np, 1), 0);
     HIP_TRY(ctx, hipMemcpyAsync(hflags.data(), dflags, sizeof(int) * np, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -1019,6 +1045,7 @@ void oc_context_destroy(oc_context* ctx) {
     collect_timing(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->pinned_done) (void)hipEventDestroy(ctx->pinned_done);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
 }

@@ -281,6 +281,20 @@ def dev_mask(values, dx, dy, orientation=0.0, aperture=(T.APERTURE_FIELD, 0.0, 0
     return s, v
 
 
+def dev_mask_from_tensor(values_dev, dx, dy, orientation=0.0, aperture=(T.APERTURE_FIELD, 0.0, 0.0),
+                         center=(0.0, 0.0)):
+    """PhaseMask over a lattice already on the device (contiguous float64
+    CUDA tensor, (ny, nx)); the caller keeps the tensor alive while the
+    struct is in use."""
+    torch = _torch()
+    if values_dev.dtype != torch.float64 or not values_dev.is_cuda or not values_dev.is_contiguous():
+        raise ValueError("lattice must be a contiguous float64 CUDA tensor")
+    ny, nx = values_dev.shape
+    kind, a, b = aperture
+    return T.PhaseMask(values_dev.data_ptr(), nx, ny, dx, dy, orientation, T.ClearAperture(kind, 0, a, b),
+                       center[0], center[1])
+
+
 def _dev_call(fn, u_dev, out_dev, geom, z, cfg, rep, mask_struct, ctx):
     torch = _torch()
     torch.cuda.synchronize()
