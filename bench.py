@@ -71,6 +71,7 @@ def bench_gs(G, W, torch, np):
     t = torch.as_tensor(tgt.astype(np.float32)).cuda()
     p0 = torch.as_tensor(ph.astype(np.float32)).cuda()
     chunk = 20
+    variant, _ = G.context().autotune(c["n"], double=False)
     st, _ = G.gs_fft_run_dev(a, t, p0.clone(), chunk, chunk)  # warm-up (plans, graph)
     if st:
         raise RuntimeError(f"gs warm-up failed: {st}")
@@ -85,7 +86,7 @@ def bench_gs(G, W, torch, np):
     its = c["restarts"] * c["iterations"]
     r = {"workload": f"GS FFT form {c['n']}^2 complex64, {c['restarts']} restarts x {c['iterations']} iterations, "
                      f"graph chunk {chunk}",
-         "iterations_per_s": its / (t1 - t0), "seconds": t1 - t0,
+         "iterations_per_s": its / (t1 - t0), "seconds": t1 - t0, "fft_variant": variant,
          "final_error_mean": float(np.mean(err)), "final_error_best": float(np.min(err))}
     log(f"gs: {r['iterations_per_s']:.0f} restart-iterations/s, error mean {r['final_error_mean']:.4f} "
         f"best {r['final_error_best']:.4f}")

@@ -54,7 +54,7 @@ struct FastLayout {
     static_assert(G >= 1 && B >= 1 && CB >= 1 && (T % B) == 0 && (B % CB) == 0, "layout");
 };
 
-// PK: pre-op (PRE_NONE, PRE_MASK, PRE_TMUL, or -1 for the descriptor's).
+// PK: pre-op (PRE_NONE, PRE_MASK, PRE_TMUL, PRE_POLAR, or -1 for the descriptor's).
 template <typename R, int N, int V, int PK>
 __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd(const ProbDev* __restrict__ probs,
                                                                          int nprob, int total) {
@@ -122,6 +122,30 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
                         if (rowok && x >= ox && x < ox + nx) v[m] = apply_mask(mk, x - ox, iy, v[m]);
                     }
                 }
+            }
+        } else if constexpr (PK == PRE_POLAR) {
+            // GS source plane: amplitude and phase of the 16 samples as one
+            // batch of loads, then from_polar (masked by the stop)
+            const R* amp = static_cast<const R*>(P.ops.amp);
+            const R* phs = static_cast<const R*>(P.ops.phase);
+            const unsigned char* open = P.ops.open;
+            const unsigned r0 = (unsigned)iy * nx - ox;
+            R a[16], ph[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int x = t + m * T;
+                a[m] = R(0);
+                ph[m] = R(0);
+                if (rowok && x >= ox && x < ox + nx) {
+                    a[m] = at32(amp, r0 + x);
+                    ph[m] = at32(phs, r0 + x);
+                    if (open && !at32(open, r0 + x)) a[m] = R(0);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const cpx<R> e = unit_phase<R>((double)ph[m]);
+                v[m] = {a[m] * e.re, a[m] * e.im};
             }
         } else {
 #pragma unroll
@@ -203,8 +227,15 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
 #pragma unroll
                 for (int m = 0; m < 16; ++m) {
                     const R a = at32(tb, k0 + (unsigned)(m * T * B));
-                    const R mg = hypot(v[m].re, v[m].im);
-                    v[m] = mg > R(0) ? cpx<R>{v[m].re * (a / mg), v[m].im * (a / mg)} : cpx<R>{a, R(0)};
+                    if constexpr (sizeof(R) == 4) {
+                        // complex64: a U / |U| through the hardware rsqrt
+                        const float r2 = fmadd(v[m].re, v[m].re, v[m].im * v[m].im);
+                        const float sc = a * __builtin_amdgcn_rsqf(r2);
+                        v[m] = r2 > 0.0f ? cpx<R>{v[m].re * sc, v[m].im * sc} : cpx<R>{a, R(0)};
+                    } else {
+                        const R mg = hypot(v[m].re, v[m].im);
+                        v[m] = mg > R(0) ? cpx<R>{v[m].re * (a / mg), v[m].im * (a / mg)} : cpx<R>{a, R(0)};
+                    }
                 }
             } else if (EK == ENG_ASM) {
                 // the column's table entries once, the 16 rows' as a batch
@@ -226,9 +257,10 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
     }
 }
 
-// PLAIN: every problem of the launch stores the cropped, scaled field as is
-// (no post-op).
-template <typename R, int N, int V, bool PLAIN>
+// QK: the post-op every problem of the launch has (POST_NONE: store the
+// cropped, scaled field; POST_GS_PHASE: the GS phase update), or -1 for the
+// descriptor's.
+template <typename R, int N, int V, int QK>
 __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv(const ProbDev* __restrict__ probs,
                                                                          int nprob, int total) {
     using LY = FastLayout<R, N, V>;
@@ -264,13 +296,34 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
                 if (x >= 0 && x < nx) {
                     const cpx<R> o = {v[m].re * s, v[m].im * s};
                     bad |= !finite_c(o);
-                    if constexpr (PLAIN) at32(out, o0 + x) = o;
-                    else store_post<R>(P, x, iy, o, acc0, acc1, acc2);
+                    if constexpr (QK == POST_NONE) {
+                        at32(out, o0 + x) = o;
+                    } else if constexpr (QK == POST_GS_PHASE) {
+                        R* ph = static_cast<R*>(P.ops.phase_io);
+                        const unsigned k = o0 + x;
+                        if constexpr (sizeof(R) == 4) {
+                            const float p = at32(ph, k);
+                            const float dphi = wrap_pi_f(atan2f(o.im, o.re) - p);
+                            at32(ph, k) =
+                                encode_bounds_f(p + (float)P.ops.beta_p * dphi, (float)P.ops.lo, (float)P.ops.hi);
+                            acc2 += (double)dphi * dphi;
+                            if (P.ops.dphi_out) at32(P.ops.dphi_out, k) = dphi;
+                        } else {
+                            const double p = at32(ph, k);
+                            const double dphi = wrap_pi(atan2((double)o.im, (double)o.re) - p);
+                            at32(ph, k) = (R)encode_bounds(p + P.ops.beta_p * dphi, P.ops.lo, P.ops.hi);
+                            acc2 += dphi * dphi;
+                            if (P.ops.dphi_out) at32(P.ops.dphi_out, k) = dphi;
+                        }
+                        if (P.out) at32(out, k) = o;
+                    } else {
+                        store_post<R>(P, x, iy, o, acc0, acc1, acc2);
+                    }
                 }
             }
         }
         if (bad) atomicOr(P.flag, 2);
-        if (!PLAIN && (P.ops.post == POST_GS_TARGET || P.ops.post == POST_GS_PHASE))
+        if (QK == POST_GS_PHASE || (QK < 0 && (P.ops.post == POST_GS_TARGET || P.ops.post == POST_GS_PHASE)))
             accumulate4(acc0, acc1, acc2, 0.0, P.ops.sums);
     }
 }
@@ -487,6 +540,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_MASK>), g, brow, row_lds, st, d, np, total);
             else if (s.pre_kind == PRE_TMUL)
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_TMUL>), g, brow, row_lds, st, d, np, total);
+            else if (s.pre_kind == PRE_POLAR)
+                hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_POLAR>), g, brow, row_lds, st, d, np, total);
             else
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, -1>), g, brow, row_lds, st, d, np, total);
             break;
@@ -501,8 +556,12 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                 hipLaunchKernelGGL((f_col<R, N, V, false, -1>), g, bcol, col_lds, st, d, np, total);
             break;
         case 2:
-            if (s.plain_post) hipLaunchKernelGGL((f_row_inv<R, N, V, true>), g, brow, row_lds, st, d, np, total);
-            else hipLaunchKernelGGL((f_row_inv<R, N, V, false>), g, brow, row_lds, st, d, np, total);
+            if (s.post_kind == POST_NONE)
+                hipLaunchKernelGGL((f_row_inv<R, N, V, POST_NONE>), g, brow, row_lds, st, d, np, total);
+            else if (s.post_kind == POST_GS_PHASE)
+                hipLaunchKernelGGL((f_row_inv<R, N, V, POST_GS_PHASE>), g, brow, row_lds, st, d, np, total);
+            else
+                hipLaunchKernelGGL((f_row_inv<R, N, V, -1>), g, brow, row_lds, st, d, np, total);
             break;
         case 3: hipLaunchKernelGGL((f_rs_kgen<R, N, V>), g, brow, row_lds, st, d, np, total); break;
         default: return hipErrorInvalidValue;
@@ -520,13 +579,15 @@ static hipError_t attr_nv() {
     const void* k[] = {(const void*)f_row_fwd<R, N, V, PRE_NONE>,
                        (const void*)f_row_fwd<R, N, V, PRE_MASK>,
                        (const void*)f_row_fwd<R, N, V, PRE_TMUL>,
+                       (const void*)f_row_fwd<R, N, V, PRE_POLAR>,
                        (const void*)f_row_fwd<R, N, V, -1>,
                        (const void*)f_col<R, N, V, true, -1>,
                        (const void*)f_col<R, N, V, false, ENG_ASM>,
                        (const void*)f_col<R, N, V, false, ENG_FRESNEL>,
                        (const void*)f_col<R, N, V, false, -1>,
-                       (const void*)f_row_inv<R, N, V, true>,
-                       (const void*)f_row_inv<R, N, V, false>,
+                       (const void*)f_row_inv<R, N, V, POST_NONE>,
+                       (const void*)f_row_inv<R, N, V, POST_GS_PHASE>,
+                       (const void*)f_row_inv<R, N, V, -1>,
                        (const void*)f_rs_kgen<R, N, V>};
     for (const void* f : k) {
         const hipError_t e = allow_lds(f);

@@ -589,7 +589,7 @@ struct Batch {
     bool fast = false;  // power-of-two kernels (all problems share P_x = P_y = fast_n)
     int fast_n = 0;
     int fast_v = 0;
-    bool plain_post = true;  // no problem has a post-op: the lean row_inv kernel
+    int post_kind = -2;      // post-op shared by every problem (-1: mixed)
     int col_kind = -2;       // engine shared by every problem (-1: mixed)
     int pre_kind = -2;       // pre-op shared by every problem (-1: mixed)
     bool sep = false;        // separable complex64 Fresnel: two passes
@@ -788,10 +788,14 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             b.pc[3].block = std::max(b.pc[3].block, brow);
             b.pc[3].lds = std::max(b.pc[3].lds, L.row_bytes * L.cnt);
         }
-        if (d.ops.post != POST_NONE) b.plain_post = false;
+        const int qk = (d.ops.post == POST_NONE || d.ops.post == POST_GS_PHASE) ? d.ops.post : -1;
+        b.post_kind = b.post_kind == -2 ? qk : (b.post_kind == qk ? qk : -1);
         const int ek = (p.engine == ENG_ASM || p.engine == ENG_FRESNEL) ? p.engine : -1;
         b.col_kind = b.col_kind == -2 ? ek : (b.col_kind == ek ? ek : -1);
-        const int pk = (d.ops.pre == PRE_NONE || d.ops.pre == PRE_MASK || d.ops.pre == PRE_TMUL) ? d.ops.pre : -1;
+        const int pk =
+            (d.ops.pre == PRE_NONE || d.ops.pre == PRE_MASK || d.ops.pre == PRE_TMUL || d.ops.pre == PRE_POLAR)
+                ? d.ops.pre
+                : -1;
         b.pre_kind = b.pre_kind == -2 ? pk : (b.pre_kind == pk ? pk : -1);
         b.pd.push_back(d);
         b.idx.push_back(cand[a]);
@@ -840,8 +844,8 @@ int launch_batch(oc_context* ctx, Batch& b, unsigned mask = kAllPasses) {
         const int tpass = b.sep ? (pass == 1 ? PASS_T_SEP_COL : PASS_T_SEP_ROW) : pass;
         hipError_t e = timed(ctx, tpass, [&] {
             if (b.fast) {
-                const ocd::FastSel sel{pass,         b.fast_n,   b.fast_v,   b.rs ? 1 : 0, b.plain_post ? 1 : 0,
-                                       b.col_kind,   b.pre_kind, b.sep ? 1 : 0};
+                const ocd::FastSel sel{pass,       b.fast_n,   b.fast_v,     b.rs ? 1 : 0, b.post_kind,
+                                       b.col_kind, b.pre_kind, b.sep ? 1 : 0};
                 return b.dbl ? ocl::fast_launch_c128(sel, dprobs, nprob, c.total, grid, ctx->stream)
                              : ocl::fast_launch_c64(sel, dprobs, nprob, c.total, grid, ctx->stream);
             }

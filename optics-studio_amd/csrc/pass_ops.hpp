@@ -260,6 +260,22 @@ __device__ __forceinline__ double encode_bounds(double phi, double lo, double hi
     return phi < lo ? lo : (phi > hi ? hi : phi);
 }
 
+// FP32 forms of wrap_pi / encode_bounds for the complex64 GS update
+// (x - m floor(x/m), folded back into [0, m) when the product rounds up).
+__device__ __forceinline__ float rem_euclid_f(float x, float m) {
+    float r = x - m * floorf(x / m);
+    return r >= m ? r - m : (r < 0.0f ? r + m : r);
+}
+__device__ __forceinline__ float wrap_pi_f(float v) {
+    const float pi = 3.14159265358979f, two_pi = 6.28318530717959f;
+    return rem_euclid_f(v + pi, two_pi) - pi;
+}
+__device__ __forceinline__ float encode_bounds_f(float phi, float lo, float hi) {
+    const float two_pi = 6.28318530717959f;
+    if (hi - lo >= two_pi) return lo + rem_euclid_f(phi - lo, two_pi);
+    return phi < lo ? lo : (phi > hi ? hi : phi);
+}
+
 // Sums a[0..3] over the workgroup's waves into sums[0..3] (one atomic per wave
 // and slot).  Every thread of the block must call it.
 __device__ __forceinline__ void accumulate4(double a0, double a1, double a2, double a3, double* sums) {
@@ -336,11 +352,20 @@ __device__ __forceinline__ void store_post(const ProbDev& P, int ix, int iy, cpx
         }
         case POST_GS_PHASE: {  // gs.rs:557-567
             R* ph = static_cast<R*>(op.phase_io);
-            const double p = ph[k];
-            const double dphi = wrap_pi(atan2((double)o.im, (double)o.re) - p);
-            ph[k] = (R)encode_bounds(p + op.beta_p * dphi, op.lo, op.hi);
-            acc2 += dphi * dphi;
-            if (op.dphi_out) op.dphi_out[k] = dphi;
+            if constexpr (sizeof(R) == 4) {
+                // complex64 form: the same update in FP32
+                const float p = ph[k];
+                const float dphi = wrap_pi_f(atan2f(o.im, o.re) - p);
+                ph[k] = encode_bounds_f(p + (float)op.beta_p * dphi, (float)op.lo, (float)op.hi);
+                acc2 += (double)dphi * dphi;
+                if (op.dphi_out) op.dphi_out[k] = dphi;
+            } else {
+                const double p = ph[k];
+                const double dphi = wrap_pi(atan2((double)o.im, (double)o.re) - p);
+                ph[k] = (R)encode_bounds(p + op.beta_p * dphi, op.lo, op.hi);
+                acc2 += dphi * dphi;
+                if (op.dphi_out) op.dphi_out[k] = dphi;
+            }
             if (out) out[k] = o;
             return;
         }

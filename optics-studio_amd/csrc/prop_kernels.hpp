@@ -76,9 +76,9 @@ struct FastSel {
     int pass;
     int n, v;
     int rs;          // Rayleigh-Sommerfeld batch
-    int plain_post;  // no problem has a post-op
+    int post_kind;   // POST_NONE / POST_GS_PHASE when uniform, else -1
     int col_kind;    // ENG_ASM / ENG_FRESNEL when uniform, else -1
-    int pre_kind;    // PRE_NONE / PRE_MASK / PRE_TMUL when uniform, else -1
+    int pre_kind;    // PRE_NONE / PRE_MASK / PRE_TMUL / PRE_POLAR when uniform, else -1
     int sep;         // separable complex64 Fresnel (two passes)
 };
 
