@@ -865,9 +865,10 @@ void evict_tables(oc_context* ctx) {
     }
 }
 
-// Runs all problems.  Device pointers throughout.  Synchronises once at the
-// end to read the per-problem finiteness flags.
-int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs) {
+// Queues all problems on the context stream (host-side validation, the
+// standalone finiteness checks, then one batch per padded-size group) with
+// their finiteness flags in dflags[0..n).  No synchronisation.
+int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags) {
     evict_tables(ctx);
     for (auto& p : probs) {
         prepare(p);
@@ -877,14 +878,7 @@ int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs) {
         }
     }
     const int np = (int)probs.size();
-    int* dflags = nullptr;
-    {
-        void* f;
-        int st = ensure_buf(ctx, "flags", sizeof(int) * std::max(np, 1), &f);
-        if (st) return st;
-        dflags = static_cast<int*>(f);
-        HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, np, ctx->stream));
-    }
+    HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, np, ctx->stream));
     for (int i = 0; i < np; ++i) {
         Prob& p = probs[i];
         if (p.status != OC_OK || !p.check_only) continue;
@@ -910,8 +904,15 @@ int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs) {
         if (st) return st;
         ++gi;
     }
+    return OC_OK;
+}
+
+// Waits for the queued work and turns the flags of launch_problems into
+// per-problem statuses; returns the first non-OK one.
+int finish_problems(oc_context* ctx, std::vector<Prob>& probs, const int* dflags) {
+    const int np = (int)probs.size();
     std::vector<int> hflags(std::max(np, 1), 0);
-    HIP_TRY(ctx, hipMemcpyAsync(hflags.data(), dflags, sizeof(int) * np, hipMemcpyDeviceToHost, ctx->stream));
+    if (np) HIP_TRY(ctx, hipMemcpyAsync(hflags.data(), dflags, sizeof(int) * np, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     collect_timing(ctx);
     int first = OC_OK;
@@ -924,6 +925,18 @@ int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs) {
         if (first == OC_OK && p.status != OC_OK) first = p.status;
     }
     return first;
+}
+
+// Runs all problems.  Device pointers throughout.  Synchronises once at the
+// end to read the per-problem finiteness flags.
+int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs) {
+    void* f;
+    int st = ensure_buf(ctx, "flags", sizeof(int) * std::max<size_t>(probs.size(), 1), &f);
+    if (st) return st;
+    int* dflags = static_cast<int*>(f);
+    st = launch_problems(ctx, dbl, probs, dflags);
+    if (st) return st;
+    return finish_problems(ctx, probs, dflags);
 }
 
 Prob make_prob(const oc_band_geom* g, double z, int engine, int band_limited, int strict, double padding,
@@ -1708,10 +1721,23 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
     size_t ne = 0;
     for (size_t s = 0; s < nsteps; ++s) ne += steps[s].has_element ? 1 : 0;
     Scoped mem(ctx);
-    void *cur, *nxt, *ckpt = nullptr, *sums;
+    void *cur, *nxt, *ckpt = nullptr, *sums, *flags;
     int st;
-    if ((st = mem.alloc(n * 16, &cur)) || (st = mem.alloc(n * 16, &nxt)) || (st = mem.alloc(64, &sums))) return st;
+    if ((st = mem.alloc(n * 16, &cur)) || (st = mem.alloc(n * 16, &nxt)) || (st = mem.alloc(64, &sums)) ||
+        (st = mem.alloc(sizeof(int) * (2 * nsteps + 1), &flags)))
+        return st;
     if (ne && (st = mem.alloc(ne * n * 16, &ckpt))) return st;
+    // Every step is queued without waiting; the finiteness flags of all
+    // propagations and the merit are read back once at the end.
+    std::vector<std::vector<Prob>> queued;
+    queued.reserve(2 * nsteps);
+    int* dflags = static_cast<int*>(flags);
+    auto propagate = [&](Prob p) -> int {
+        queued.push_back({p});
+        int s2 = launch_problems(ctx, true, queued.back(), dflags + (queued.size() - 1));
+        if (s2 == OC_OK) s2 = queued.back()[0].status;
+        return s2;
+    };
     HIP_TRY(ctx, hipMemcpyAsync(cur, d_input, n * 16, hipMemcpyDeviceToDevice, ctx->stream));
     // forward: store each element's incoming field, fuse the mask into the propagation
     size_t e = 0;
@@ -1723,10 +1749,10 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
             ++e;
         }
         if (stp.z_after != 0.0) {
-            std::vector<Prob> v{make_prob(&g, stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
-                                          cfg.max_padding, stp.medium, stp.has_element ? &stp.mask : nullptr, cur,
-                                          nxt)};
-            if (run_batch(ctx, true, v) != OC_OK) return fail(ctx, OC_INV_ADJOINT, "forward propagation failed");
+            if (propagate(make_prob(&g, stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
+                                    cfg.max_padding, stp.medium, stp.has_element ? &stp.mask : nullptr, cur,
+                                    nxt)) != OC_OK)
+                return fail(ctx, OC_INV_ADJOINT, "forward propagation failed");
             std::swap(cur, nxt);
         } else if (stp.has_element) {
             const int fs = mask_footprint(g, stp.mask);
@@ -1742,13 +1768,21 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
     if (dmdi) HIP_TRY(ctx, ocl::launch_seed(true, cur, dmdi, nxt, n, ctx->stream));
     else HIP_TRY(ctx, ocl::launch_seed_mse(true, cur, target, nxt, n, static_cast<double*>(sums), ctx->stream));
     std::swap(cur, nxt);
-    if (merit) {
+    auto finish = [&]() -> int {
         double hs[4];
+        std::vector<int> hf(queued.size() + 1, 0);
         HIP_TRY(ctx, hipMemcpyAsync(hs, sums, 32, hipMemcpyDeviceToHost, ctx->stream));
+        if (!queued.empty())
+            HIP_TRY(ctx, hipMemcpyAsync(hf.data(), dflags, sizeof(int) * queued.size(), hipMemcpyDeviceToHost,
+                                        ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        *merit = hs[0] / (double)n;
-    }
-    if (!grads) return OC_OK;
+        collect_timing(ctx);
+        if (merit) *merit = hs[0] / (double)n;
+        for (size_t i = 0; i < queued.size(); ++i)
+            if (hf[i]) return fail(ctx, OC_INV_ADJOINT, "non-finite samples in a propagation");
+        return OC_OK;
+    };
+    if (!grads) return finish();
     HIP_TRY(ctx, hipMemsetAsync(grads, 0, ne * n * 8, ctx->stream));
     // reverse sweep
     e = ne;
@@ -1758,16 +1792,16 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
         void* ck = stp.has_element ? static_cast<unsigned char*>(ckpt) + e * n * 16 : nullptr;
         double* gr = stp.has_element ? grads + e * n : nullptr;
         if (stp.z_after != 0.0) {
-            std::vector<Prob> v{make_prob(&g, -stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
-                                          cfg.max_padding, stp.medium, nullptr, cur, nxt)};
+            Prob pb = make_prob(&g, -stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding, cfg.max_padding,
+                                stp.medium, nullptr, cur, nxt);
             if (stp.has_element) {
-                v[0].mask = &stp.mask;  // mask tables for the epilogue (no pre-op)
-                v[0].ops.pre = PRE_NONE;
-                v[0].ops.post = POST_ADJ_GRAD;
-                v[0].ops.ckpt = ck;
-                v[0].ops.grad = gr;
+                pb.mask = &stp.mask;  // mask tables for the epilogue (no pre-op)
+                pb.ops.pre = PRE_NONE;
+                pb.ops.post = POST_ADJ_GRAD;
+                pb.ops.ckpt = ck;
+                pb.ops.grad = gr;
             }
-            if (run_batch(ctx, true, v) != OC_OK) return fail(ctx, OC_INV_ADJOINT, "adjoint propagation failed");
+            if (propagate(pb) != OC_OK) return fail(ctx, OC_INV_ADJOINT, "adjoint propagation failed");
             std::swap(cur, nxt);
         } else if (stp.has_element) {
             MaskDev md;
@@ -1775,8 +1809,7 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
             HIP_TRY(ctx, ocl::launch_adj_elem(true, cur, ck, gr, md, (int)g.nx, (int)g.ny, ctx->stream));
         }
     }
-    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    return OC_OK;
+    return finish();
 }
 
 }  // namespace

@@ -215,7 +215,10 @@ __global__ void k_rs_kgen(const ProbDev* __restrict__ probs, int nprob, int tota
         for (int i = tid; i < cnt * px; i += bdim) {
             const int u = i / px, x = i - u * px, y = y0 + u;
             cpx<R> v = {R(0), R(0)};
-            if (y < py) v = rs_sample<R>(P.eng, x, y, px, py);
+            if (y < py) {
+                if constexpr (sizeof(R) == 4) v = rs_sample_fast(P.eng, x, y, px, py);
+                else v = rs_sample<R>(P.eng, x, y, px, py);
+            }
             lds[u * stride + lidx(x)] = v;
         }
         __syncthreads();
