@@ -48,6 +48,8 @@ typedef enum oc_status {
     OC_EMPTY_GRID = 4,
     OC_NON_FINITE_SAMPLE = 5,
     OC_ALIASING_RISK = 6,
+    /* FraunhoferError::RegimeViolation (src/propagation/fraunhofer.rs:246-252) */
+    OC_REGIME_VIOLATION = 7,
     /* OpticsError (src/element.rs:60-170) */
     OC_INVALID_APERTURE = 20,
     OC_OUTSIDE_FIELD = 21,
@@ -134,6 +136,34 @@ typedef struct oc_rs_config {
     int32_t strict;
     int32_t _pad0;
 } oc_rs_config;
+
+/* FraunhoferConfig (src/propagation/fraunhofer.rs:117-170): the Option
+   fields are a flag plus value. */
+typedef struct oc_fraunhofer_config {
+    int32_t strict;
+    int32_t has_source_pos;       /* 0: axis at the grid origin */
+    double max_n_f;
+    double source_x, source_y;    /* optical axis (m), when has_source_pos */
+    int32_t has_aperture_extent;  /* 0: conservative window extent */
+    int32_t _pad0;
+    double aperture_extent;       /* declared source half-extent (m) */
+} oc_fraunhofer_config;
+
+/* FraunhoferReport (src/propagation/fraunhofer.rs:178-222).  The output
+   grid has the input's sample counts and pitch dx_out x dy_out. */
+typedef struct oc_fraunhofer_report {
+    double lambda, z;
+    uint64_t nx, ny;
+    double dx, dy;
+    double dx_out, dy_out;
+    double n_f_x, n_f_y;
+    double source_x, source_y;
+    double extent_used;
+    int32_t extent_is_aperture;
+    int32_t regime_ok;
+    double quad_phase_error_rad;
+    double max_n_f;
+} oc_fraunhofer_report;
 
 /* RayleighSommerfeldSamplingReport (rayleigh_sommerfeld.rs:123-189). */
 typedef struct oc_rs_report {
@@ -281,6 +311,31 @@ int oc_fresnel_propagate_dev_c64(oc_context* ctx, const oc_band_geom* geom,
                                  const oc_fresnel_config* cfg,
                                  const oc_phase_mask* mask, float* out,
                                  oc_fresnel_report* report);
+
+/* ------------------------------------------------------------------------ */
+/* Fraunhofer far field (src/propagation/fraunhofer.rs).                     */
+/* ------------------------------------------------------------------------ */
+
+/* Replaces fraunhofer::regime_report (src/propagation/fraunhofer.rs:289-334). */
+int oc_fraunhofer_regime_report(const oc_band_geom* geom, double z,
+                                const oc_fraunhofer_config* cfg,
+                                oc_fraunhofer_report* report);
+/* Replaces fraunhofer::propagate (src/propagation/fraunhofer.rs:351-443):
+   the output field has the input's dims on the far-field grid
+   (report->dx_out, report->dy_out). */
+int oc_fraunhofer_propagate(oc_context* ctx, const oc_band_geom* geom,
+                            const double* in, double z,
+                            const oc_fraunhofer_config* cfg, double* out,
+                            oc_fraunhofer_report* report);
+int oc_fraunhofer_propagate_dev_c128(oc_context* ctx, const oc_band_geom* geom,
+                                     const double* in, double z,
+                                     const oc_fraunhofer_config* cfg,
+                                     double* out,
+                                     oc_fraunhofer_report* report);
+int oc_fraunhofer_propagate_dev_c64(oc_context* ctx, const oc_band_geom* geom,
+                                    const float* in, double z,
+                                    const oc_fraunhofer_config* cfg,
+                                    float* out, oc_fraunhofer_report* report);
 
 /* ------------------------------------------------------------------------ */
 /* Rayleigh-Sommerfeld first kind (src/propagation/rayleigh_sommerfeld.rs).  */

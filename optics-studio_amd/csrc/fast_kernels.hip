@@ -207,7 +207,19 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
             for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], w[m]);
         } else {
             const int ek = EK >= 0 ? EK : P.eng.kind;
-            if (ek == ENG_MEASURE) {
+            if (ek == ENG_FRAUNHOFER) {
+                // forward only: every bin goes to its far-field sample
+                cpx<R>* out = static_cast<cpx<R>*>(P.out);
+                const int nxo = P.nx, nyo = P.ny;
+#pragma unroll
+                for (int m = 0; m < 16; ++m) {
+                    int ox_, oy_;
+                    const cpx<R> o = fraun_post<R>(P.eng, kx, t + m * T, nxo, nyo, v[m], &ox_, &oy_);
+                    if (!finite_c(o)) atomicOr(P.flag, 2);
+                    at32(out, (unsigned)oy_ * nxo + ox_) = o;
+                }
+                continue;
+            } else if (ek == ENG_MEASURE) {
                 const R* tb = static_cast<const R*>(P.ops.tamp_blk);
                 const unsigned k0 = ((unsigned)q * N + t) * B + bq;
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0;

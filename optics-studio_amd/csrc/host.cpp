@@ -304,6 +304,8 @@ struct Prob {
     oc_asm_report asm_rep{};
     oc_fresnel_report fr_rep{};
     oc_rs_report rs_rep{};
+    oc_fraunhofer_config fh_cfg{};
+    oc_fraunhofer_report fh_rep{};
     OpsDev ops{};
 };
 
@@ -350,6 +352,30 @@ int engine_tables(oc_context* ctx, const Prob& p, bool dbl, EngineDev* e) {
     const size_t nxp = p.nxp, nyp = p.nyp;
     std::memset(e, 0, sizeof *e);
     e->kind = p.engine;
+    if (p.engine == ENG_FRAUNHOFER) {  // fraunhofer.rs:360-428
+        const oc_fraunhofer_report& r = p.fh_rep;
+        e->lambda = p.g.lambda;
+        e->z = p.z;
+        e->sgn = p.z > 0.0 ? 1.0 : -1.0;
+        e->dx = p.g.dx;
+        e->dy = p.g.dy;
+        e->fcx = ((double)p.g.nx - 1.0) / 2.0;
+        e->fcy = ((double)p.g.ny - 1.0) / 2.0;
+        e->fs = r.source_x / (p.g.lambda * p.z);
+        e->gs = r.source_y / (p.g.lambda * p.z);
+        e->dxo = r.dx_out;
+        e->dyo = r.dy_out;
+        e->xs = r.source_x;
+        e->ys = r.source_y;
+        // e^{ikz} / (i lambda z), complex division written out
+        const double k = 2.0 * och::PI / p.g.lambda;
+        const double c = std::cos(k * p.z), s = std::sin(k * p.z);
+        const double dre = 0.0 * p.g.lambda * p.z, dim = 1.0 * p.g.lambda * p.z;
+        const double nsq = dre * dre + dim * dim;
+        e->quad_re = (c * dre + s * dim) / nsq;
+        e->quad_im = (s * dre - c * dim) / nsq;
+        return OC_OK;
+    }
     if (p.engine == ENG_RS) {
         e->lambda = p.g.lambda;
         e->z = p.z;
@@ -499,6 +525,22 @@ void prepare(Prob& p) {
         if (g.nx == 0 || g.ny == 0) { p.status = OC_EMPTY_GRID; return; }
         p.nxp = g.nx;
         p.nyp = g.ny;
+        p.launch = true;
+        return;
+    }
+    if (p.engine == ENG_FRAUNHOFER) {  // fraunhofer.rs:269-287, 351-358
+        if (!std::isfinite(p.z) || p.z == 0.0) { p.status = OC_INVALID_DISTANCE; return; }
+        if (!std::isfinite(g.lambda) || g.lambda <= 0.0) { p.status = OC_INVALID_WAVELENGTH; return; }
+        if (g.nx == 0 || g.ny == 0) { p.status = OC_EMPTY_GRID; return; }
+        p.fh_rep = och::fraunhofer_report(g, p.z, p.fh_cfg);
+        if (p.fh_cfg.strict && !p.fh_rep.regime_ok) {
+            p.check_only = true;  // the finiteness check keeps its precedence
+            p.pending = OC_REGIME_VIOLATION;
+            return;
+        }
+        p.nxp = g.nx;  // unpadded
+        p.nyp = g.ny;
+        p.ops.pre = PRE_FRAUN;
         p.launch = true;
         return;
     }
@@ -748,7 +790,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         d.scale = 1.0 / (double)(p.nxp * p.nyp);
         d.flag = dflags + cand[a];
         d.ops = p.ops;
-        if (p.engine == ENG_ASM || p.engine == ENG_FRESNEL || p.engine == ENG_RS) {
+        if (p.engine == ENG_ASM || p.engine == ENG_FRESNEL || p.engine == ENG_RS || p.engine == ENG_FRAUNHOFER) {
             st = engine_tables(ctx, p, dbl, &d.eng);
             if (st) return st;
         } else {
@@ -806,6 +848,9 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     b.pc[3].total = base_kgen;
     b.pc[2] = b.pc[0];
     if (b.sep) b.pc[0].total = 0;  // pass 1: columns, pass 2: rows
+    bool fraun = true;
+    for (const ProbDev& d : b.pd) fraun = fraun && d.eng.kind == ENG_FRAUNHOFER;
+    if (fraun) b.pc[2].total = 0;  // the column pass writes the far field
     const size_t bytes = b.pd.size() * sizeof(ProbDev);
     int st = batch_alloc(ctx, b, persistent, "probs" + tag, bytes, &b.dprobs);
     if (st) return st;
@@ -893,7 +938,10 @@ int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* df
         const Prob& p = probs[i];
         if (p.status != OC_OK || !p.launch) continue;
         const bool fast = !ctx->force_generic && p.nxp == p.nyp && ocl::fast_variants((int)p.nxp) > 0;
-        groups[fast ? std::make_pair(p.nxp, p.nyp) : std::make_pair((size_t)0, (size_t)0)].push_back(i);
+        // Fraunhofer problems (forward transforms only) never share a batch
+        // with the others: their batch has no inverse row pass
+        const size_t fh = p.engine == ENG_FRAUNHOFER ? ((size_t)1 << 40) : 0;
+        groups[fast ? std::make_pair(p.nxp, p.nyp + fh) : std::make_pair((size_t)0, fh)].push_back(i);
     }
     int gi = 0;
     for (auto& kv : groups) {
@@ -1009,6 +1057,7 @@ const char* oc_status_name(int s) {
         case OC_EMPTY_GRID: return "OC_EMPTY_GRID";
         case OC_NON_FINITE_SAMPLE: return "OC_NON_FINITE_SAMPLE";
         case OC_ALIASING_RISK: return "OC_ALIASING_RISK";
+        case OC_REGIME_VIOLATION: return "OC_REGIME_VIOLATION";
         case OC_INVALID_APERTURE: return "OC_INVALID_APERTURE";
         case OC_OUTSIDE_FIELD: return "OC_OUTSIDE_FIELD";
         case OC_INVALID_AXIS: return "OC_INVALID_AXIS";
@@ -1196,6 +1245,47 @@ int oc_fresnel_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const f
                                  const oc_fresnel_config* cfg, const oc_phase_mask* mask, float* out,
                                  oc_fresnel_report* rep) {
     return dev_fresnel(ctx, false, g, in, z, cfg, mask, out, rep);
+}
+
+int oc_fraunhofer_regime_report(const oc_band_geom* g, double z, const oc_fraunhofer_config* cfg,
+                                oc_fraunhofer_report* rep) {
+    if (!g || !cfg || !rep) return OC_ERR_INVALID_ARGUMENT;
+    *rep = och::fraunhofer_report(*g, z, *cfg);
+    return OC_OK;
+}
+
+int oc_fraunhofer_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                            const oc_fraunhofer_config* cfg, double* out, oc_fraunhofer_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    Prob done;
+    Prob p = make_prob(g, z, ENG_FRAUNHOFER, 0, cfg->strict, 1.0, 1.0, 1.0, nullptr, nullptr, nullptr);
+    p.fh_cfg = *cfg;
+    int st = run_host(ctx, p, in, out, nullptr, &done);
+    if (rep) *rep = done.fh_rep;
+    return st;
+}
+
+static int dev_fraunhofer(oc_context* ctx, bool dbl, const oc_band_geom* g, const void* in, double z,
+                          const oc_fraunhofer_config* cfg, void* out, oc_fraunhofer_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    std::vector<Prob> v{make_prob(g, z, ENG_FRAUNHOFER, 0, cfg->strict, 1.0, 1.0, 1.0, nullptr, in, out)};
+    v[0].fh_cfg = *cfg;
+    int st = run_batch(ctx, dbl, v);
+    if (rep) *rep = v[0].fh_rep;
+    return st;
+}
+
+int oc_fraunhofer_propagate_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                                     const oc_fraunhofer_config* cfg, double* out, oc_fraunhofer_report* rep) {
+    return dev_fraunhofer(ctx, true, g, in, z, cfg, out, rep);
+}
+int oc_fraunhofer_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
+                                    const oc_fraunhofer_config* cfg, float* out, oc_fraunhofer_report* rep) {
+    return dev_fraunhofer(ctx, false, g, in, z, cfg, out, rep);
 }
 
 int oc_rs_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z, const oc_rs_config* cfg,

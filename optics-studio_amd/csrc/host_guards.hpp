@@ -179,4 +179,37 @@ inline double wrap_angle(double psi) {  // common.rs:30-32 (rem_euclid into [0, 
     return r < 0.0 ? r + m : r;
 }
 
+// fraunhofer::regime_report (src/propagation/fraunhofer.rs:289-334): Fresnel
+// numbers from the declared aperture half-extent or, by default, the window
+// extent measured from the optical axis; the far-field pitch lambda|z|/(n d).
+inline oc_fraunhofer_report fraunhofer_report(const oc_band_geom& g, double z, const oc_fraunhofer_config& c) {
+    oc_fraunhofer_report r;
+    std::memset(&r, 0, sizeof r);
+    const double az = std::fabs(z);
+    const double xs = c.has_source_pos ? c.source_x : 0.0, ys = c.has_source_pos ? c.source_y : 0.0;
+    const double hx = ((double)g.nx - 1.0) / 2.0 * g.dx, hy = ((double)g.ny - 1.0) / 2.0 * g.dy;
+    const bool ap = c.has_aperture_extent && std::isfinite(c.aperture_extent) && c.aperture_extent > 0.0;
+    const double rx = ap ? c.aperture_extent : hx + std::fabs(xs);
+    const double ry = ap ? c.aperture_extent : hy + std::fabs(ys);
+    r.lambda = g.lambda;
+    r.z = z;
+    r.nx = g.nx;
+    r.ny = g.ny;
+    r.dx = g.dx;
+    r.dy = g.dy;
+    r.dx_out = g.lambda * az / ((double)g.nx * g.dx);
+    r.dy_out = g.lambda * az / ((double)g.ny * g.dy);
+    r.n_f_x = rx * rx / (g.lambda * az);
+    r.n_f_y = ry * ry / (g.lambda * az);
+    const double nf = std::fmax(r.n_f_x, r.n_f_y);
+    r.source_x = xs;
+    r.source_y = ys;
+    r.extent_used = std::fmax(rx, ry);
+    r.extent_is_aperture = ap ? 1 : 0;
+    r.quad_phase_error_rad = PI * nf;
+    r.max_n_f = c.max_n_f;
+    r.regime_ok = nf <= c.max_n_f ? 1 : 0;
+    return r;
+}
+
 }  // namespace och

@@ -265,6 +265,51 @@ __device__ __forceinline__ cpx<float> rs_sample_fast(const EngineDev& e, int ix,
 }
 
 /* ------------------------------------------------------------------ */
+/* Fraunhofer offset DFT (fraunhofer.rs:384-437).                       */
+/* ------------------------------------------------------------------ */
+// Pre-phase of input sample (ix, iy): e^{i 2 pi c_y iy/n_y} e^{i 2 pi g_s eta}
+// e^{i 2 pi (c_x ix/n_x + f_s xi)}.  complex128: the three factors and their
+// products as the reference forms them; complex64: one phase, summed in f64.
+template <typename R>
+__device__ __forceinline__ cpx<R> fraun_pre(const EngineDev& e, int ix, int iy, int nx, int ny) {
+    const double eta = ((double)iy - e.fcy) * e.dy;
+    const double xi = ((double)ix - e.fcx) * e.dx;
+    const double ty = 2.0 * kPi * e.fcy * (double)iy / (double)ny;
+    const double tr = 2.0 * kPi * e.gs * eta;
+    const double tx = 2.0 * kPi * (e.fcx * (double)ix / (double)nx + e.fs * xi);
+    if constexpr (sizeof(R) == 8) {
+        return cmul(cmul(unit_phase<double>(ty), unit_phase<double>(tr)), unit_phase<double>(tx));
+    } else {
+        return unit_phase<float>(ty + tr + tx);
+    }
+}
+
+// Far-field sample for raw bin (mx, my) with value b: its output position
+// (ix, iy) (identity for z > 0, mirrored for z < 0) and the value
+// e^{ikz}/(i lambda z) chirp(X, Y) post(mx, my) b dx dy.
+template <typename R>
+__device__ __forceinline__ cpx<R> fraun_post(const EngineDev& e, int mx, int my, int nx, int ny, cpx<R> b, int* ix,
+                                             int* iy) {
+    *ix = e.sgn > 0.0 ? mx : nx - 1 - mx;
+    *iy = e.sgn > 0.0 ? my : ny - 1 - my;
+    const double dxp = ((double)*ix - e.fcx) * e.dxo - e.xs;
+    const double dyp = ((double)*iy - e.fcy) * e.dyo - e.ys;
+    const double py = 2.0 * kPi * e.fcy * ((double)my - e.fcy) / (double)ny;
+    const double px = 2.0 * kPi * e.fcx * ((double)mx - e.fcx) / (double)nx;
+    const double pc = kPi * (dxp * dxp + dyp * dyp) / (e.lambda * e.z);
+    const double dxdy = e.dx * e.dy;
+    if constexpr (sizeof(R) == 8) {
+        const cpx<double> q = {e.quad_re, e.quad_im};
+        const cpx<double> post = cmul(unit_phase<double>(py), unit_phase<double>(px));
+        const cpx<double> v = cmul(cmul(cmul(q, unit_phase<double>(pc)), post), b);
+        return {v.re * dxdy, v.im * dxdy};
+    } else {
+        const cpx<float> q = {(float)(e.quad_re * dxdy), (float)(e.quad_im * dxdy)};
+        return cmul(cmul(q, unit_phase<float>(pc + py + px)), b);
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* Fused element-wise ops.                                              */
 /* ------------------------------------------------------------------ */
 __device__ __forceinline__ double rem_euclid(double x, double m) {
@@ -330,6 +375,8 @@ __device__ __forceinline__ cpx<R> load_pre(const ProbDev& P, int ix, int iy, boo
     bad |= !finite_c(v);
     if (pre == PRE_MASK) {
         v = apply_mask(P.mask, ix, iy, v);
+    } else if (pre == PRE_FRAUN) {
+        v = cmul(v, fraun_pre<R>(P.eng, ix, iy, P.nx, P.ny));
     } else if (pre == PRE_TMUL) {
         cpx<R> t;
         t = gp(static_cast<const cpx<R>*>(o.trans))[k];

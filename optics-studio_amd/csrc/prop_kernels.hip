@@ -104,6 +104,22 @@ __global__ void k_col(const ProbDev* __restrict__ probs, int nprob, int total) {
         const int t = u < ntr ? tid - u * T : py;
         run_fft<-1>(lds + uu * stride, lds + uu * stride + lds_stride(py), P.pcol, t, false);
         const int ek = P.eng.kind;
+        if (ek == ENG_FRAUNHOFER) {  // forward only: each bin to its far-field sample
+            __syncthreads();
+            cpx<R>* out = static_cast<cpx<R>*>(P.out);
+            bool bad = false;
+            for (int i = tid; i < B * py; i += bdim) {
+                const int b = i % B, my = i / B;
+                int ox_, oy_;
+                const cpx<R> o = fraun_post<R>(P.eng, q * B + b, my, P.nx, P.ny, lds[b * stride + lidx(my)], &ox_,
+                                               &oy_);
+                bad |= !finite_c(o);
+                out[(size_t)oy_ * P.nx + ox_] = o;
+            }
+            if (bad) atomicOr(P.flag, 2);
+            __syncthreads();
+            continue;
+        }
         if (ek == ENG_MEASURE) {  // |F|^2 statistics of the spectrum; no inverse, no store
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
             if (u < B) {

@@ -13,13 +13,15 @@ enum EngineKind : int {
     ENG_RS = 2,
     ENG_NONE = 3,        // plain 2-D FFT round trip (identity)
     ENG_GS_PROJECT = 4,  // Fourier-plane amplitude projection (FFT-form GS)
-    ENG_MEASURE = 5      // forward only: accumulate intensity statistics
+    ENG_MEASURE = 5,     // forward only: accumulate intensity statistics
+    ENG_FRAUNHOFER = 6   // forward only: far-field bin map and prefactor, written to the output
 };
 // Element-wise op fused into the first pass's loads.
 // PRE_TMUL multiplies by a precomputed transmission (a phase mask shared by
 // several problems of a launch, evaluated once: it does not depend on the
 // wavelength).
-enum PreOp : int { PRE_NONE = 0, PRE_MASK = 1, PRE_POLAR = 2, PRE_SEED_MSE = 3, PRE_TMUL = 4 };
+// PRE_FRAUN multiplies by the Fraunhofer offset-DFT pre-phase and axis ramp.
+enum PreOp : int { PRE_NONE = 0, PRE_MASK = 1, PRE_POLAR = 2, PRE_SEED_MSE = 3, PRE_TMUL = 4, PRE_FRAUN = 5 };
 // Element-wise op fused into the last pass's (cropped) stores.
 enum PostOp : int { POST_NONE = 0, POST_GS_TARGET = 1, POST_GS_PHASE = 2, POST_ADJ_GRAD = 3 };
 enum MaskMode : int { MASK_NONE = 0, MASK_SEPARABLE = 1, MASK_GENERAL = 2 };
@@ -67,6 +69,10 @@ struct EngineDev {
     const void* hy;
     // Rayleigh-Sommerfeld kernel sampling
     double lambda, z, k, sgn, dx, dy;
+    // Fraunhofer (fraunhofer.rs:376-437): origin indices c = (n - 1)/2, axis
+    // ramp frequencies, far-field pitch and axis, pi/(lambda z), and the
+    // prefactor e^{ikz}/(i lambda z)
+    double fcx, fcy, fs, gs, dxo, dyo, xs, ys, chirp, quad_re, quad_im;
 };
 
 // What a launch of the power-of-two kernels holds uniformly, selecting the

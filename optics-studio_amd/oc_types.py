@@ -18,6 +18,7 @@ STATUS = {
     4: "OC_EMPTY_GRID",
     5: "OC_NON_FINITE_SAMPLE",
     6: "OC_ALIASING_RISK",
+    7: "OC_REGIME_VIOLATION",
     20: "OC_INVALID_APERTURE",
     21: "OC_OUTSIDE_FIELD",
     22: "OC_INVALID_AXIS",
@@ -88,6 +89,32 @@ class FresnelReport(C.Structure):
                 ("max_phase_step_x", c_f64), ("max_phase_step_y", c_f64), ("nyquist_ok", c_i32),
                 ("kernel_nyquist_ok", c_i32), ("max_kernel_step_x", c_f64), ("max_kernel_step_y", c_f64),
                 ("max_phase_error_vs_asm", c_f64), ("paraxial_ok", c_i32), ("_pad0", c_i32)]
+
+
+class FraunhoferConfig(C.Structure):
+    """FraunhoferConfig (src/propagation/fraunhofer.rs:117-170); the Option
+    fields are a flag plus value."""
+    _fields_ = [("strict", c_i32), ("has_source_pos", c_i32), ("max_n_f", c_f64), ("source_x", c_f64),
+                ("source_y", c_f64), ("has_aperture_extent", c_i32), ("_pad0", c_i32), ("aperture_extent", c_f64)]
+
+    @classmethod
+    def default(cls, strict=True, max_n_f=1.0, source_pos=None, aperture_extent=None):
+        c = cls()
+        c.strict = 1 if strict else 0
+        c.max_n_f = max_n_f
+        if source_pos is not None:
+            c.has_source_pos, c.source_x, c.source_y = 1, source_pos[0], source_pos[1]
+        if aperture_extent is not None:
+            c.has_aperture_extent, c.aperture_extent = 1, aperture_extent
+        return c
+
+
+class FraunhoferReport(C.Structure):
+    """FraunhoferReport (src/propagation/fraunhofer.rs:178-222)."""
+    _fields_ = [("lambda_", c_f64), ("z", c_f64), ("nx", c_u64), ("ny", c_u64), ("dx", c_f64), ("dy", c_f64),
+                ("dx_out", c_f64), ("dy_out", c_f64), ("n_f_x", c_f64), ("n_f_y", c_f64), ("source_x", c_f64),
+                ("source_y", c_f64), ("extent_used", c_f64), ("extent_is_aperture", c_i32), ("regime_ok", c_i32),
+                ("quad_phase_error_rad", c_f64), ("max_n_f", c_f64)]
 
 
 class RsConfig(C.Structure):

@@ -56,6 +56,14 @@ EXPORTS = {
     "oc_asm_propagate_dev_c64": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), P(T.PhaseMask), vp,
                                   P(T.AsmReport)], C.c_int),
     "oc_fresnel_sampling_report": ([P(T.BandGeom), c_f64, P(T.FresnelConfig), P(T.FresnelReport)], C.c_int),
+    "oc_fraunhofer_regime_report": ([P(T.BandGeom), c_f64, P(T.FraunhoferConfig), P(T.FraunhoferReport)],
+                                    C.c_int),
+    "oc_fraunhofer_propagate": ([vp, P(T.BandGeom), vp, c_f64, P(T.FraunhoferConfig), vp,
+                                 P(T.FraunhoferReport)], C.c_int),
+    "oc_fraunhofer_propagate_dev_c128": ([vp, P(T.BandGeom), vp, c_f64, P(T.FraunhoferConfig), vp,
+                                          P(T.FraunhoferReport)], C.c_int),
+    "oc_fraunhofer_propagate_dev_c64": ([vp, P(T.BandGeom), vp, c_f64, P(T.FraunhoferConfig), vp,
+                                         P(T.FraunhoferReport)], C.c_int),
     "oc_fresnel_propagate": ([vp, P(T.BandGeom), vp, c_f64, P(T.FresnelConfig), vp, P(T.FresnelReport)],
                              C.c_int),
     "oc_fresnel_propagate_dev_c128": ([vp, P(T.BandGeom), vp, c_f64, P(T.FresnelConfig), P(T.PhaseMask), vp,
@@ -217,6 +225,25 @@ def fresnel_propagate(u, dx, dy, lam, z, cfg, ctx=None):
     return st, out, rep
 
 
+def fraunhofer_propagate(u, dx, dy, lam, z, cfg, ctx=None):
+    """fraunhofer::propagate, host f64; the far field's pitch is
+    (report.dx_out, report.dy_out)."""
+    ctx = ctx or context()
+    u = _c128(u)
+    out = np.empty_like(u)
+    rep = T.FraunhoferReport()
+    st = lib.oc_fraunhofer_propagate(ctx.h, C.byref(_geom(u, dx, dy, lam)), u.ctypes.data, z, C.byref(cfg),
+                                     out.ctypes.data, C.byref(rep))
+    return st, out, rep
+
+
+def fraunhofer_regime_report(shape, dx, dy, lam, z, cfg):
+    ny, nx = shape
+    rep = T.FraunhoferReport()
+    lib.oc_fraunhofer_regime_report(C.byref(T.geom(nx, ny, dx, dy, lam)), z, C.byref(cfg), C.byref(rep))
+    return rep
+
+
 def rs_propagate(u, dx, dy, lam, z, cfg, ctx=None):
     ctx = ctx or context()
     u = _c128(u)
@@ -328,6 +355,18 @@ def fresnel_propagate_dev(u_dev, dx, dy, lam, z, cfg, mask=None, out=None, ctx=N
     return st, out, rep
 
 
+def fraunhofer_propagate_dev(u_dev, dx, dy, lam, z, cfg, out=None, ctx=None):
+    torch = _torch()
+    ctx = ctx or context()
+    out = torch.empty_like(u_dev) if out is None else out
+    ny, nx = u_dev.shape
+    fn = (lib.oc_fraunhofer_propagate_dev_c64 if u_dev.dtype == torch.complex64
+          else lib.oc_fraunhofer_propagate_dev_c128)
+    rep = T.FraunhoferReport()
+    st = _dev_call(fn, u_dev, out, T.geom(nx, ny, dx, dy, lam), z, cfg, rep, False, ctx)
+    return st, out, rep
+
+
 def rs_propagate_dev(u_dev, dx, dy, lam, z, cfg, out=None, ctx=None):
     torch = _torch()
     ctx = ctx or context()
@@ -387,6 +426,14 @@ class DevEngine:
     def rs_propagate(self, u, dx, dy, lam, z, cfg):
         st, out, rep = rs_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
         return st, self._down(out), rep
+
+    def fraunhofer_propagate(self, u, dx, dy, lam, z, cfg):
+        st, out, rep = fraunhofer_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
+        return st, self._down(out), rep
+
+    @staticmethod
+    def fraunhofer_regime_report(shape, dx, dy, lam, z, cfg):
+        return fraunhofer_regime_report(shape, dx, dy, lam, z, cfg)
 
     class Mask:
         def __init__(self, values, dx, dy, orientation=0.0, aperture=(T.APERTURE_FIELD, 0.0, 0.0),

@@ -793,6 +793,99 @@ double or_fresnel_worst_axis_step(uint64_t n_pad, double d, double lambda, doubl
     return fresnel_worst_axis_step(n_pad, d, lambda, z);
 }
 
+// fraunhofer.rs:289-334
+int or_fraunhofer_regime_report(const oc_band_geom* g, double z, const oc_fraunhofer_config* cfg,
+                                oc_fraunhofer_report* rep) {
+    const size_t nx = g->nx, ny = g->ny;
+    const double dx = g->dx, dy = g->dy, lambda = g->lambda, az = std::fabs(z);
+    const double xs = cfg->has_source_pos ? cfg->source_x : 0.0;
+    const double ys = cfg->has_source_pos ? cfg->source_y : 0.0;
+    const double hx = ((double)nx - 1.0) / 2.0 * dx;
+    const double hy = ((double)ny - 1.0) / 2.0 * dy;
+    const bool ap = cfg->has_aperture_extent && std::isfinite(cfg->aperture_extent) && cfg->aperture_extent > 0.0;
+    const double rx = ap ? cfg->aperture_extent : hx + std::fabs(xs);
+    const double ry = ap ? cfg->aperture_extent : hy + std::fabs(ys);
+    std::memset(rep, 0, sizeof *rep);
+    rep->lambda = lambda;
+    rep->z = z;
+    rep->nx = nx;
+    rep->ny = ny;
+    rep->dx = dx;
+    rep->dy = dy;
+    rep->dx_out = lambda * az / ((double)nx * dx);
+    rep->dy_out = lambda * az / ((double)ny * dy);
+    rep->n_f_x = rx * rx / (lambda * az);
+    rep->n_f_y = ry * ry / (lambda * az);
+    const double n_f = fmaxr(rep->n_f_x, rep->n_f_y);
+    rep->source_x = xs;
+    rep->source_y = ys;
+    rep->extent_used = fmaxr(rx, ry);
+    rep->extent_is_aperture = ap ? 1 : 0;
+    rep->quad_phase_error_rad = PI * n_f;
+    rep->max_n_f = cfg->max_n_f;
+    rep->regime_ok = n_f <= cfg->max_n_f ? 1 : 0;
+    return OC_OK;
+}
+
+// fraunhofer.rs:351-443: offset DFT with the pre-phase (and the axis ramp),
+// unpadded fft2, then the bin map (mirrored for z < 0), post-phase, chirp
+// about the axis, e^{ikz}/(i lambda z) and dx dy.
+int or_fraunhofer_propagate(const oc_band_geom* g, const double* in_, double z, const oc_fraunhofer_config* cfg,
+                            double* out_, oc_fraunhofer_report* rep) {
+    const cx* in = reinterpret_cast<const cx*>(in_);
+    cx* out = reinterpret_cast<cx*>(out_);
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    if (!std::isfinite(z) || z == 0.0) return OC_INVALID_DISTANCE;
+    const double lambda = g->lambda;
+    if (!std::isfinite(lambda) || lambda <= 0.0) return OC_INVALID_WAVELENGTH;
+    const size_t nx = g->nx, ny = g->ny;
+    if (nx == 0 || ny == 0) return OC_EMPTY_GRID;
+    for (size_t i = 0; i < nx * ny; ++i)
+        if (!cfinite(in[i])) return OC_NON_FINITE_SAMPLE;
+    oc_fraunhofer_report r;
+    or_fraunhofer_regime_report(g, z, cfg, &r);
+    if (rep) *rep = r;
+    if (cfg->strict && !r.regime_ok) return OC_REGIME_VIOLATION;
+    const double dx = g->dx, dy = g->dy, dxo = r.dx_out, dyo = r.dy_out;
+    const double xs = r.source_x, ys = r.source_y;
+    const double k = 2.0 * PI / lambda;
+    const double f_s = xs / (lambda * z), g_s = ys / (lambda * z);
+    const double cxo = ((double)nx - 1.0) / 2.0, cyo = ((double)ny - 1.0) / 2.0;
+    std::vector<cx> buf(nx * ny);
+    for (size_t iy = 0; iy < ny; ++iy) {
+        const double eta = ((double)iy - cyo) * dy;
+        const cx pre_y = cexpi(2.0 * PI * cyo * (double)iy / (double)ny);
+        const cx ramp_y = cexpi(2.0 * PI * g_s * eta);
+        for (size_t ix = 0; ix < nx; ++ix) {
+            const double xi = ((double)ix - cxo) * dx;
+            const cx pre = cmul(cmul(pre_y, ramp_y), cexpi(2.0 * PI * (cxo * (double)ix / (double)nx + f_s * xi)));
+            buf[iy * nx + ix] = cmul(in[iy * nx + ix], pre);
+        }
+    }
+    fft2(buf.data(), nx, ny);
+    const double sgn = z > 0.0 ? 1.0 : -1.0;
+    // Complex::exp(i k z) / (Complex::I * lambda * z) with num-complex Div
+    const cx e = cexpi(k * z);
+    const cx den = {0.0 * lambda * z, 1.0 * lambda * z};
+    const double nsq = den.re * den.re + den.im * den.im;
+    const cx quad = {(e.re * den.re + e.im * den.im) / nsq, (e.im * den.re - e.re * den.im) / nsq};
+    for (size_t iy = 0; iy < ny; ++iy) {
+        const double dyp = ((double)iy - cyo) * dyo - ys;
+        const size_t my = (size_t)std::round(sgn * ((double)iy - cyo) + cyo);
+        const cx post_y = cexpi(2.0 * PI * cyo * ((double)my - cyo) / (double)ny);
+        for (size_t ix = 0; ix < nx; ++ix) {
+            const double dxp = ((double)ix - cxo) * dxo - xs;
+            const size_t mx = (size_t)std::round(sgn * ((double)ix - cxo) + cxo);
+            const cx post = cmul(post_y, cexpi(2.0 * PI * cxo * ((double)mx - cxo) / (double)nx));
+            const cx chirp = cexpi(PI * (dxp * dxp + dyp * dyp) / (lambda * z));
+            out[iy * nx + ix] = crmul(cmul(cmul(cmul(quad, chirp), post), buf[my * nx + mx]), dx * dy);
+        }
+    }
+    for (size_t i = 0; i < nx * ny; ++i)
+        if (!cfinite(out[i])) return OC_NON_FINITE_SAMPLE;
+    return OC_OK;
+}
+
 int or_rs_sampling_report(const oc_band_geom* g, double z, const oc_rs_config* cfg, oc_rs_report* rep) {
     *rep = rs_analyze(g, z, cfg->padding);
     return OC_OK;

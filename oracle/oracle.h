@@ -58,6 +58,10 @@ double or_fresnel_worst_axis_step(uint64_t n_pad, double d, double lambda,
 /* rayleigh_sommerfeld.rs */
 int or_rs_sampling_report(const oc_band_geom* g, double z,
                           const oc_rs_config* cfg, oc_rs_report* rep);
+int or_fraunhofer_regime_report(const oc_band_geom* g, double z, const oc_fraunhofer_config* cfg,
+                                oc_fraunhofer_report* rep);
+int or_fraunhofer_propagate(const oc_band_geom* g, const double* in, double z, const oc_fraunhofer_config* cfg,
+                            double* out, oc_fraunhofer_report* rep);
 int or_rs_propagate(const oc_band_geom* g, const double* in, double z,
                     const oc_rs_config* cfg, double* out, oc_rs_report* rep);
 void or_rs_kernel_value(double x, double y, double lambda, double z,

@@ -58,6 +58,9 @@ def _declare(L):
     L.or_fresnel_transfer_value.argtypes = [c_f64, c_f64, c_f64, c_f64, vp]
     L.or_fresnel_worst_axis_step.argtypes = [c_u64, c_f64, c_f64, c_f64]
     L.or_fresnel_worst_axis_step.restype = c_f64
+    FH = P(T.FraunhoferConfig)
+    L.or_fraunhofer_regime_report.argtypes = [G, c_f64, FH, P(T.FraunhoferReport)]
+    L.or_fraunhofer_propagate.argtypes = [G, vp, c_f64, FH, vp, P(T.FraunhoferReport)]
     L.or_rs_sampling_report.argtypes = [G, c_f64, R, P(T.RsReport)]
     L.or_rs_propagate.argtypes = [G, vp, c_f64, R, vp, P(T.RsReport)]
     L.or_rs_kernel_value.argtypes = [c_f64, c_f64, c_f64, c_f64, vp]
@@ -180,6 +183,24 @@ def rs_sampling_report(shape, dx, dy, lam, z, cfg):
     rep = T.RsReport()
     lib().or_rs_sampling_report(C.byref(T.geom(nx, ny, dx, dy, lam)), z, C.byref(cfg), C.byref(rep))
     return rep
+
+
+def fraunhofer_regime_report(shape, dx, dy, lam, z, cfg):
+    ny, nx = shape
+    rep = T.FraunhoferReport()
+    lib().or_fraunhofer_regime_report(C.byref(T.geom(nx, ny, dx, dy, lam)), z, C.byref(cfg), C.byref(rep))
+    return rep
+
+
+def fraunhofer_propagate(u, dx, dy, lam, z, cfg):
+    """Returns (status, far field, report); the far field's pitch is
+    (report.dx_out, report.dy_out)."""
+    u = _c128(u)
+    out = np.empty_like(u)
+    rep = T.FraunhoferReport()
+    st = lib().or_fraunhofer_propagate(C.byref(_geom(u, dx, dy, lam)), _ptr(u), z, C.byref(cfg), _ptr(out),
+                                       C.byref(rep))
+    return st, out, rep
 
 
 def rs_propagate(u, dx, dy, lam, z, cfg):

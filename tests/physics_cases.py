@@ -441,6 +441,158 @@ def mask_outside_field_rejected(eng, tol=1.0):
     assert st == T.CODE["OC_OUTSIDE_FIELD"]
 
 
+# --------------------------------------------------------------------------
+# Fraunhofer: src/propagation/fraunhofer.rs:465-727 (the reference's tests)
+# --------------------------------------------------------------------------
+
+def fh_power(u, dx, dy):
+    return float(np.sum(np.abs(u) ** 2) * dx * dy)
+
+
+def fraunhofer_delta_flat_amplitude_quadratic_phase(eng, tol=1.0):
+    n, d, z = 15, 2e-5, 4.0
+    u = np.zeros((n, n), np.complex128)
+    u[n // 2, n // 2] = 1.0
+    st, out, rep = eng.fraunhofer_propagate(u, d, d, LAMBDA, z, T.FraunhoferConfig.default())
+    ok(st)
+    k = 2 * math.pi / LAMBDA
+    amp = d * d / (LAMBDA * z)
+    dxo = LAMBDA * z / (n * d)
+    assert abs(rep.dx_out - dxo) < 1e-18 and abs(rep.dy_out - dxo) < 1e-18
+    for iy in range(n):
+        for ix in range(n):
+            s_ = out[iy, ix]
+            x = (ix - n / 2.0 + 0.5) * dxo
+            y = (iy - n / 2.0 + 0.5) * dxo
+            ph = k * z - math.pi / 2.0 + math.pi * (x * x + y * y) / (LAMBDA * z)
+            expected = complex(math.cos(ph), math.sin(ph)) * amp
+            # the reference's absolute bounds; reduced precision: relative to amp
+            assert abs(abs(s_) - amp) < (1e-12 if tol == 1.0 else tol * amp)
+            assert abs(s_ - expected) < (1e-9 if tol == 1.0 else tol * amp)
+
+
+def fraunhofer_uniform_concentrates_at_dc(eng, tol=1.0):
+    n, d, z, a = 15, 2e-5, 4.0, 0.75
+    u = np.full((n, n), a, np.complex128)
+    st, out, _ = eng.fraunhofer_propagate(u, d, d, LAMBDA, z, T.FraunhoferConfig.default())
+    ok(st)
+    expected = a * (n * d) ** 2 / (LAMBDA * z)
+    c = n // 2
+    bound = 1e-9 if tol == 1.0 else tol * expected
+    assert abs(abs(out[c, c]) - expected) < bound
+    mask = np.ones((n, n), bool)
+    mask[c, c] = False
+    assert np.max(np.abs(out[mask])) < bound
+
+
+def fraunhofer_power_equals_input_power(eng, tol=1.0):
+    n, d, z = 64, 1e-5, 2.0
+    u = gaussian(n, n, d, 5.0 * d)
+    rep = eng.fraunhofer_regime_report(u.shape, d, d, LAMBDA, z, T.FraunhoferConfig.default())
+    assert rep.regime_ok
+    st, out, rep = eng.fraunhofer_propagate(u, d, d, LAMBDA, z, T.FraunhoferConfig.default())
+    ok(st)
+    p_in, p_out = fh_power(u, d, d), fh_power(out, rep.dx_out, rep.dy_out)
+    assert abs((p_out - p_in) / p_in) < lim(1e-9, tol)
+
+
+def fraunhofer_regime_gate(eng, tol=1.0):
+    n, d = 64, 1e-5
+    u = gaussian(n, n, d, 5.0 * d)
+    st, _, rep = eng.fraunhofer_propagate(u, d, d, LAMBDA, 0.01, T.FraunhoferConfig.default())
+    assert st == T.CODE["OC_REGIME_VIOLATION"]
+    assert not rep.regime_ok and rep.n_f_x > 1.0
+    st, out, _ = eng.fraunhofer_propagate(u, d, d, LAMBDA, 0.01, T.FraunhoferConfig.default(strict=False))
+    ok(st)
+    assert out.shape == u.shape
+    tight = T.FraunhoferConfig.default(max_n_f=0.05)
+    assert not eng.fraunhofer_regime_report(u.shape, d, d, LAMBDA, 2.0, tight).regime_ok
+    assert eng.fraunhofer_regime_report(u.shape, d, d, LAMBDA, 5.0, tight).regime_ok
+
+
+def fraunhofer_negative_z_mirrors(eng, tol=1.0):
+    n, d, z = 64, 1e-5, 2.0
+    u = gaussian(n, n, d, 5.0 * d)
+    st, fwd, _ = eng.fraunhofer_propagate(u, d, d, LAMBDA, z, T.FraunhoferConfig.default())
+    ok(st)
+    st, back, _ = eng.fraunhofer_propagate(u, d, d, LAMBDA, -z, T.FraunhoferConfig.default())
+    ok(st)
+    scale = np.max(np.abs(fwd))
+    assert np.max(np.abs(np.abs(fwd) - np.abs(back[::-1, ::-1]))) < lim(1e-12, tol) * max(1.0, scale)
+
+
+def fraunhofer_invalid_inputs(eng, tol=1.0):
+    u = uniform(1.0, 8, 8)
+    cfg = T.FraunhoferConfig.default(strict=False)
+    for z in (0.0, float("nan")):
+        st, _, _ = eng.fraunhofer_propagate(u, 20e-6, 20e-6, LAMBDA, z, cfg)
+        assert st == T.CODE["OC_INVALID_DISTANCE"]
+    st, _, _ = eng.fraunhofer_propagate(u, 20e-6, 20e-6, -1.0, 1.0, cfg)
+    assert st == T.CODE["OC_INVALID_WAVELENGTH"]
+    bad = u.copy()
+    bad[2, 3] = complex(float("inf"), 0.0)
+    st, _, _ = eng.fraunhofer_propagate(bad, 20e-6, 20e-6, LAMBDA, 1.0, cfg)
+    assert st == T.CODE["OC_NON_FINITE_SAMPLE"]
+    # non-finite precedes the regime gate
+    st, _, _ = eng.fraunhofer_propagate(bad, 20e-6, 20e-6, LAMBDA, 1e-6, T.FraunhoferConfig.default())
+    assert st == T.CODE["OC_NON_FINITE_SAMPLE"]
+
+
+def fraunhofer_source_pos_none_equals_origin(eng, tol=1.0):
+    u = gaussian(64, 64, 1e-5, 5e-5)
+    st, a, _ = eng.fraunhofer_propagate(u, 1e-5, 1e-5, LAMBDA, 2.0, T.FraunhoferConfig.default())
+    ok(st)
+    st, b, _ = eng.fraunhofer_propagate(u, 1e-5, 1e-5, LAMBDA, 2.0,
+                                        T.FraunhoferConfig.default(source_pos=(0.0, 0.0)))
+    ok(st)
+    assert np.array_equal(a, b)
+
+
+def fraunhofer_off_axis_conserves_power(eng, tol=1.0):
+    s_ = (1e-4, -5e-5)
+    z = 2.0
+    cfg = T.FraunhoferConfig.default(source_pos=s_)
+    shifted = gaussian(64, 64, 1e-5, 5e-5, c=s_)
+    centered = gaussian(64, 64, 1e-5, 5e-5)
+    assert eng.fraunhofer_regime_report(shifted.shape, 1e-5, 1e-5, LAMBDA, z, cfg).regime_ok
+    st, out_s, rep = eng.fraunhofer_propagate(shifted, 1e-5, 1e-5, LAMBDA, z, cfg)
+    ok(st)
+    p_s = fh_power(shifted, 1e-5, 1e-5)
+    assert abs((fh_power(out_s, rep.dx_out, rep.dy_out) - p_s) / p_s) < lim(1e-9, tol)
+    st, out_c, rep_c = eng.fraunhofer_propagate(centered, 1e-5, 1e-5, LAMBDA, z, T.FraunhoferConfig.default())
+    ok(st)
+    pc_ = fh_power(out_c, rep_c.dx_out, rep_c.dy_out)
+    assert abs((fh_power(out_s, rep.dx_out, rep.dy_out) - pc_) / pc_) < lim(1e-9, tol)
+
+
+def fraunhofer_aperture_extent(eng, tol=1.0):
+    u = gaussian(64, 64, 1e-5, 2.0e-5)
+    z = 0.05
+    win = eng.fraunhofer_regime_report(u.shape, 1e-5, 1e-5, LAMBDA, z, T.FraunhoferConfig.default())
+    assert not win.regime_ok and not win.extent_is_aperture
+    st, _, _ = eng.fraunhofer_propagate(u, 1e-5, 1e-5, LAMBDA, z, T.FraunhoferConfig.default())
+    assert st == T.CODE["OC_REGIME_VIOLATION"]
+    cfg = T.FraunhoferConfig.default(aperture_extent=1e-4)
+    rep = eng.fraunhofer_regime_report(u.shape, 1e-5, 1e-5, LAMBDA, z, cfg)
+    assert rep.regime_ok and rep.extent_is_aperture
+    assert abs(rep.extent_used - 1e-4) < 1e-15 and abs(rep.n_f_x - 0.4) < 1e-9
+    st, out, _ = eng.fraunhofer_propagate(u, 1e-5, 1e-5, LAMBDA, z, cfg)
+    ok(st)
+    st, _, _ = eng.fraunhofer_propagate(u, 1e-5, 1e-5, LAMBDA, z, T.FraunhoferConfig.default(aperture_extent=0.0))
+    assert st == T.CODE["OC_REGIME_VIOLATION"]
+
+
+def fraunhofer_extent_from_axis(eng, tol=1.0):
+    u = gaussian(64, 64, 1e-5, 5e-5)
+    cfg = T.FraunhoferConfig.default(max_n_f=0.15)
+    assert eng.fraunhofer_regime_report(u.shape, 1e-5, 1e-5, LAMBDA, 2.0, cfg).regime_ok
+    off = T.FraunhoferConfig.default(max_n_f=0.15, source_pos=(2e-4, 0.0))
+    rep = eng.fraunhofer_regime_report(u.shape, 1e-5, 1e-5, LAMBDA, 2.0, off)
+    assert not rep.regime_ok and (rep.source_x, rep.source_y) == (2e-4, 0.0)
+    st, _, _ = eng.fraunhofer_propagate(u, 1e-5, 1e-5, LAMBDA, 2.0, off)
+    assert st == T.CODE["OC_REGIME_VIOLATION"]
+
+
 ASM_CASES = [asm_z_zero_identity, asm_plane_wave_picks_up_kz, asm_back_propagation_inverts, asm_invalid_inputs,
              asm_strict_flags_undersampled, asm_medium_plane_wave_nkz, asm_medium_round_trip,
              asm_strict_mitigates, asm_energy_conservation, asm_plane_wave_invariance,
@@ -450,5 +602,9 @@ FRESNEL_CASES = [fresnel_z_zero_identity, fresnel_plane_wave_bin_phase, fresnel_
                  fresnel_strict_flags_and_mitigates, fresnel_invalid_inputs, fresnel_gaussian_vs_asm_paraxial]
 RS_CASES = [rs_delta_is_sampled_kernel, rs_invalid_inputs, rs_strict_guard, rs_matches_direct_quadrature,
             rs_agrees_with_asm_smooth_band]
+FRAUNHOFER_CASES = [fraunhofer_delta_flat_amplitude_quadratic_phase, fraunhofer_uniform_concentrates_at_dc,
+                    fraunhofer_power_equals_input_power, fraunhofer_regime_gate, fraunhofer_negative_z_mirrors,
+                    fraunhofer_invalid_inputs, fraunhofer_source_pos_none_equals_origin,
+                    fraunhofer_off_axis_conserves_power, fraunhofer_aperture_extent, fraunhofer_extent_from_axis]
 MASK_CASES = [mask_identity_is_exact, mask_constant_pi_stop_and_span, mask_rotation_conjugates_sample,
               mask_transmission_probe, mask_outside_field_rejected]

@@ -218,3 +218,24 @@ def test_workload2_one_band_c64(G, orc):
     ef = rel_l2(f.cpu().numpy().astype(np.complex128), rf)
     say("  asm", ea, "fresnel", ef)
     assert ea <= C64_TOL and ef <= C64_TOL
+
+
+FH_CASES = [((64, 64), 2.0, None), ((256, 256), -3.0, None), ((1024, 1024), 5.0, (2e-5, -1e-5)),
+            ((37, 101), 2.5, None), ((200, 200), -2.0, (1e-5, 1e-5)), ((96, 256), 4.0, None)]
+
+
+@pytest.mark.parametrize("shape,z,src", FH_CASES, ids=[f"{s[0]}x{s[1]}-z{z}" for s, z, _ in FH_CASES])
+def test_fraunhofer_parity(G, c64, c128, orc, shape, z, src):
+    u = rand_field(shape, 31 + shape[0] + shape[1])
+    d, lam = 4e-6, 633e-9
+    cfg = T.FraunhoferConfig.default(strict=False, source_pos=src)
+    say("fraunhofer", shape, z)
+    st, ref, rref = orc.fraunhofer_propagate(u, d, d, lam, z, cfg)
+    assert st == 0
+    for name, eng, tol in (("f64", G, F64_TOL), ("c128", c128, F64_TOL), ("c64", c64, C64_TOL)):
+        st, out, rep = eng.fraunhofer_propagate(u, d, d, lam, z, cfg)
+        assert st == 0, (name, T.STATUS[st])
+        assert T.report_dict(rep) == T.report_dict(rref)
+        e = rel_l2(out, ref)
+        say("  ", name, e)
+        assert e <= tol, (name, e)

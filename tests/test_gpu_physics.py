@@ -7,7 +7,7 @@ import physics_cases as pc
 
 pytestmark = pytest.mark.gpu
 
-ALL = pc.ASM_CASES + pc.FRESNEL_CASES + pc.RS_CASES + pc.MASK_CASES
+ALL = pc.ASM_CASES + pc.FRESNEL_CASES + pc.RS_CASES + pc.FRAUNHOFER_CASES + pc.MASK_CASES
 C64_FLOOR = 2e-6  # about 16 FP32 ulps on unit-scale quantities
 
 

@@ -8,7 +8,7 @@ import pytest
 import oc_types as T
 import physics_cases as pc
 
-ALL = pc.ASM_CASES + pc.FRESNEL_CASES + pc.RS_CASES + pc.MASK_CASES
+ALL = pc.ASM_CASES + pc.FRESNEL_CASES + pc.RS_CASES + pc.FRAUNHOFER_CASES + pc.MASK_CASES
 
 
 @pytest.mark.parametrize("case", ALL, ids=[c.__name__ for c in ALL])
@@ -179,3 +179,37 @@ def test_workload_guards(orc):
         assert r.nyquist_ok and r.nx_pad == 4096
     r = orc.asm_sampling_report((1024, 1024), 3e-6, 3e-6, 1.55e-6, 10e-3, T.AsmConfig.default())
     assert r.nyquist_ok and r.nx_pad == 2048
+
+
+@pytest.mark.parametrize("shape,z,src", [((7, 6), 2.0, None), ((8, 8), -1.5, None), ((9, 5), 3.0, (3e-5, -2e-5))])
+def test_fraunhofer_oracle_equals_direct_sum(orc, shape, z, src):
+    """The far field as the textbook sum (fraunhofer.rs:1-17):
+    U(X, Y) = e^{ikz}/(i lam z) e^{i pi ((X-xs)^2 + (Y-ys)^2)/(lam z)}
+              dx dy sum u(xi, eta) e^{-i 2 pi (fx xi + fy eta)},
+    f = (X - xs)/(lam z), X = (i - c) dX, dX = lam |z| / (n dx); brute force
+    over every input and output sample."""
+    import math
+    ny, nx = shape
+    d, lam = 1e-5, 500e-9
+    rng = np.random.default_rng(nx * 10 + ny)
+    u = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    cfg = T.FraunhoferConfig.default(strict=False, source_pos=src)
+    st, out, rep = orc.fraunhofer_propagate(u, d, d, lam, z, cfg)
+    assert st == 0
+    xs, ys = src if src else (0.0, 0.0)
+    dxo, dyo = lam * abs(z) / (nx * d), lam * abs(z) / (ny * d)
+    xi = (np.arange(nx) - (nx - 1) / 2) * d
+    eta = (np.arange(ny) - (ny - 1) / 2) * d
+    X = (np.arange(nx) - (nx - 1) / 2) * dxo
+    Y = (np.arange(ny) - (ny - 1) / 2) * dyo
+    k = 2 * math.pi / lam
+    pref = np.exp(1j * k * z) / (1j * lam * z)
+    want = np.empty(shape, complex)
+    for iy in range(ny):
+        for ix in range(nx):
+            fx, fy = (X[ix] - xs) / (lam * z), (Y[iy] - ys) / (lam * z)
+            ft = d * d * np.sum(u * np.exp(-2j * math.pi * (fx * xi[None, :] + fy * eta[:, None])))
+            chirp = np.exp(1j * math.pi * ((X[ix] - xs) ** 2 + (Y[iy] - ys) ** 2) / (lam * z))
+            want[iy, ix] = pref * chirp * ft
+    e = np.linalg.norm(out - want) / np.linalg.norm(want)
+    assert e < 1e-11, e
