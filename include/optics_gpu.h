@@ -312,6 +312,26 @@ int oc_fresnel_propagate_dev_c64(oc_context* ctx, const oc_band_geom* geom,
                                  const oc_phase_mask* mask, float* out,
                                  oc_fresnel_report* report);
 
+/* Replaces fresnel::propagate_convolution (src/propagation/fresnel.rs:531-590):
+   the sampled paraxial impulse response's spectrum multiplies the field's;
+   strict mode checks the kernel sampling without densifying. */
+int oc_fresnel_propagate_convolution(oc_context* ctx, const oc_band_geom* geom,
+                                     const double* in, double z,
+                                     const oc_fresnel_config* cfg, double* out,
+                                     oc_fresnel_report* report);
+int oc_fresnel_propagate_convolution_dev_c128(oc_context* ctx,
+                                              const oc_band_geom* geom,
+                                              const double* in, double z,
+                                              const oc_fresnel_config* cfg,
+                                              double* out,
+                                              oc_fresnel_report* report);
+int oc_fresnel_propagate_convolution_dev_c64(oc_context* ctx,
+                                             const oc_band_geom* geom,
+                                             const float* in, double z,
+                                             const oc_fresnel_config* cfg,
+                                             float* out,
+                                             oc_fresnel_report* report);
+
 /* ------------------------------------------------------------------------ */
 /* Fraunhofer far field (src/propagation/fraunhofer.rs).                     */
 /* ------------------------------------------------------------------------ */
@@ -364,7 +384,8 @@ int oc_rs_propagate_dev_c64(oc_context* ctx, const oc_band_geom* geom,
 typedef enum oc_engine {
     OC_ENGINE_ASM = 0,
     OC_ENGINE_FRESNEL = 1,
-    OC_ENGINE_RS = 2
+    OC_ENGINE_RS = 2,
+    OC_ENGINE_FRESNEL_CONV = 7  /* fresnel::propagate_convolution */
 } oc_engine;
 
 /* One problem of a batch.  `in`/`out` are device pointers (complex64 for the

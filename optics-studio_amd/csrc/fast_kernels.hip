@@ -357,10 +357,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
         cpx<R> v[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m)
-            if constexpr (sizeof(R) == 4)
-                v[m] = rowok ? rs_sample_fast(P.eng, t + m * T, y, N, N) : cpx<R>{R(0), R(0)};
-            else
-                v[m] = rowok ? rs_sample<R>(P.eng, t + m * T, y, N, N) : cpx<R>{R(0), R(0)};
+            v[m] = rowok ? conv_sample<R>(P.eng, t + m * T, y, N, N) : cpx<R>{R(0), R(0)};
         fft_regs<R, N, G, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         if (rowok) {
             cpx<R>* kb = static_cast<cpx<R>*>(P.kbuf);

@@ -376,6 +376,20 @@ int engine_tables(oc_context* ctx, const Prob& p, bool dbl, EngineDev* e) {
         e->quad_im = (s * dre - c * dim) / nsq;
         return OC_OK;
     }
+    if (p.engine == ENG_FRESNEL_CONV) {  // fresnel.rs:319-321
+        e->lambda = p.g.lambda;
+        e->z = p.z;
+        e->dx = p.g.dx;
+        e->dy = p.g.dy;
+        const double k = 2.0 * och::PI / p.g.lambda;
+        const double c = std::cos(k * p.z), s = std::sin(k * p.z);
+        const double dre = 0.0 * p.g.lambda * p.z, dim = 1.0 * p.g.lambda * p.z;
+        const double nsq = dre * dre + dim * dim;
+        const double qre = (c * dre + s * dim) / nsq, qim = (s * dre - c * dim) / nsq;
+        e->pref_re = qre * (p.g.dx * p.g.dy);
+        e->pref_im = qim * (p.g.dx * p.g.dy);
+        return OC_OK;
+    }
     if (p.engine == ENG_RS) {
         e->lambda = p.g.lambda;
         e->z = p.z;
@@ -569,6 +583,13 @@ void prepare(Prob& p) {
         if (c.strict && !r.nyquist_ok) { p.check_only = true; p.pending = OC_ALIASING_RISK; return; }
         p.nxp = r.nx_pad;
         p.nyp = r.ny_pad;
+    } else if (p.engine == ENG_FRESNEL_CONV) {  // fresnel.rs:531-553: no densification
+        oc_fresnel_config c{p.padding, p.strict, 0, p.max_padding};
+        oc_fresnel_report r = och::fresnel_analyze(g, p.z, c, c.padding);
+        p.fr_rep = r;
+        if (c.strict && !r.kernel_nyquist_ok) { p.check_only = true; p.pending = OC_ALIASING_RISK; return; }
+        p.nxp = r.nx_pad;
+        p.nyp = r.ny_pad;
     } else if (p.engine == ENG_FRESNEL) {
         oc_fresnel_config c{p.padding, p.strict, 0, p.max_padding};
         oc_fresnel_report r = och::fresnel_analyze(g, p.z, c, c.padding);
@@ -686,7 +707,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         woff.push_back(work_elems);
         work_elems += p.g.ny * p.nxp;
         koff.push_back(kbuf_elems);
-        if (p.engine == ENG_RS) kbuf_elems += p.nyp * p.nxp;
+        if (conv_engine(p.engine)) kbuf_elems += p.nyp * p.nxp;
         cand.push_back(i);
     }
     // power-of-two fast path: one shared square padded size, uniform RS-ness
@@ -694,9 +715,9 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     if (!cand.empty() && !ctx->force_generic) {
         const size_t n0 = probs[cand[0]].nxp;
         bool ok = n0 >= 32 && n0 <= 8192 && (n0 & (n0 - 1)) == 0;
-        const bool rs0 = probs[cand[0]].engine == ENG_RS;
+        const bool rs0 = conv_engine(probs[cand[0]].engine);
         for (int i : cand)
-            ok = ok && probs[i].nxp == n0 && probs[i].nyp == n0 && (probs[i].engine == ENG_RS) == rs0;
+            ok = ok && probs[i].nxp == n0 && probs[i].nyp == n0 && conv_engine(probs[i].engine) == rs0;
         const int v = ctx->variant_for((int)n0, dbl);
         if (ok && ocl::fast_supported(dbl, (int)n0, v, &fG, &fB, &fC)) {
             b.fast = true;
@@ -754,7 +775,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         if (st) return st;
         st = make_plan(ctx, d.py, dbl, &d.pcol);
         if (st) return st;
-        const bool rs = p.engine == ENG_RS;
+        const bool rs = conv_engine(p.engine);
         Layout L;
         if (b.fast) {
             const size_t slots = (size_t)b.fast_n + b.fast_n / 16;
@@ -790,7 +811,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         d.scale = 1.0 / (double)(p.nxp * p.nyp);
         d.flag = dflags + cand[a];
         d.ops = p.ops;
-        if (p.engine == ENG_ASM || p.engine == ENG_FRESNEL || p.engine == ENG_RS || p.engine == ENG_FRAUNHOFER) {
+        if (p.engine == ENG_ASM || p.engine == ENG_FRESNEL || conv_engine(p.engine) || p.engine == ENG_FRAUNHOFER) {
             st = engine_tables(ctx, p, dbl, &d.eng);
             if (st) return st;
         } else {
@@ -1247,6 +1268,40 @@ int oc_fresnel_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const f
     return dev_fresnel(ctx, false, g, in, z, cfg, mask, out, rep);
 }
 
+int oc_fresnel_propagate_convolution(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                                     const oc_fresnel_config* cfg, double* out, oc_fresnel_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    Prob done;
+    Prob p = make_prob(g, z, ENG_FRESNEL_CONV, 0, cfg->strict, cfg->padding, cfg->max_padding, 1.0, nullptr, nullptr,
+                       nullptr);
+    int st = run_host(ctx, p, in, out, nullptr, &done);
+    if (rep) *rep = done.fr_rep;
+    return st;
+}
+
+static int dev_fresnel_conv(oc_context* ctx, bool dbl, const oc_band_geom* g, const void* in, double z,
+                            const oc_fresnel_config* cfg, void* out, oc_fresnel_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    std::vector<Prob> v{
+        make_prob(g, z, ENG_FRESNEL_CONV, 0, cfg->strict, cfg->padding, cfg->max_padding, 1.0, nullptr, in, out)};
+    int st = run_batch(ctx, dbl, v);
+    if (rep) *rep = v[0].fr_rep;
+    return st;
+}
+
+int oc_fresnel_propagate_convolution_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                                              const oc_fresnel_config* cfg, double* out, oc_fresnel_report* rep) {
+    return dev_fresnel_conv(ctx, true, g, in, z, cfg, out, rep);
+}
+int oc_fresnel_propagate_convolution_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
+                                             const oc_fresnel_config* cfg, float* out, oc_fresnel_report* rep) {
+    return dev_fresnel_conv(ctx, false, g, in, z, cfg, out, rep);
+}
+
 int oc_fraunhofer_regime_report(const oc_band_geom* g, double z, const oc_fraunhofer_config* cfg,
                                 oc_fraunhofer_report* rep) {
     if (!g || !cfg || !rep) return OC_ERR_INVALID_ARGUMENT;
@@ -1327,8 +1382,9 @@ static int batch(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
     v.reserve(count);
     for (size_t i = 0; i < count; ++i) {
         const oc_prop_desc& x = d[i];
-        if (x.engine < OC_ENGINE_ASM || x.engine > OC_ENGINE_RS) return OC_ERR_INVALID_ARGUMENT;
-        if (x.engine == OC_ENGINE_RS && x.mask) return OC_ERR_INVALID_ARGUMENT;
+        if ((x.engine < OC_ENGINE_ASM || x.engine > OC_ENGINE_RS) && x.engine != OC_ENGINE_FRESNEL_CONV)
+            return OC_ERR_INVALID_ARGUMENT;
+        if ((x.engine == OC_ENGINE_RS || x.engine == OC_ENGINE_FRESNEL_CONV) && x.mask) return OC_ERR_INVALID_ARGUMENT;
         v.push_back(make_prob(&x.geom, x.z, x.engine, x.band_limited, x.strict, x.padding, x.max_padding,
                               x.engine == OC_ENGINE_ASM ? x.medium : 1.0, x.mask, x.in, x.out));
     }

@@ -264,6 +264,28 @@ __device__ __forceinline__ cpx<float> rs_sample_fast(const EngineDev& e, int ix,
             (float)((are * (double)u.im + aim * (double)u.re) * dxdy)};
 }
 
+// Paraxial impulse response of the Fresnel convolution form in DFT order
+// (fresnel.rs:312-333): pref exp(i pi (x^2 + y^2)/(lambda z)).
+template <typename R>
+__device__ __forceinline__ cpx<R> fconv_sample(const EngineDev& e, int ix, int iy, int px, int py) {
+    const double x = (ix <= px / 2) ? (double)ix * e.dx : ((double)ix - (double)px) * e.dx;
+    const double y = (iy <= py / 2) ? (double)iy * e.dy : ((double)iy - (double)py) * e.dy;
+    const double th = kPi * (x * x + y * y) / (e.lambda * e.z);
+    if constexpr (sizeof(R) == 8) {
+        return cmul(cpx<double>{e.pref_re, e.pref_im}, unit_phase<double>(th));
+    } else {
+        return cmul(cpx<float>{(float)e.pref_re, (float)e.pref_im}, unit_phase<float>(th));
+    }
+}
+
+// Kernel sample of a convolution engine.
+template <typename R>
+__device__ __forceinline__ cpx<R> conv_sample(const EngineDev& e, int ix, int iy, int px, int py) {
+    if (e.kind == ENG_FRESNEL_CONV) return fconv_sample<R>(e, ix, iy, px, py);
+    if constexpr (sizeof(R) == 4) return rs_sample_fast(e, ix, iy, px, py);
+    else return rs_sample<R>(e, ix, iy, px, py);
+}
+
 /* ------------------------------------------------------------------ */
 /* Fraunhofer offset DFT (fraunhofer.rs:384-437).                       */
 /* ------------------------------------------------------------------ */

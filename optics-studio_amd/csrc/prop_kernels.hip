@@ -81,7 +81,7 @@ __global__ void k_col(const ProbDev* __restrict__ probs, int nprob, int total) {
         const int q = tile - P.base_col;
         const int B = P.B, py = P.py, ny = P.ny, oy = P.oy;
         const int stride = fft_lds_stride(P.pcol);
-        const bool rs = P.eng.kind == ENG_RS;
+        const bool rs = conv_engine(P.eng.kind);
         const int ntr = rs ? 2 * B : B;  // RS: B field columns + B kernel columns
         cpx<R>* __restrict__ work = static_cast<cpx<R>*>(P.work);
         for (int i = tid; i < B * py; i += bdim) {
@@ -231,10 +231,7 @@ __global__ void k_rs_kgen(const ProbDev* __restrict__ probs, int nprob, int tota
         for (int i = tid; i < cnt * px; i += bdim) {
             const int u = i / px, x = i - u * px, y = y0 + u;
             cpx<R> v = {R(0), R(0)};
-            if (y < py) {
-                if constexpr (sizeof(R) == 4) v = rs_sample_fast(P.eng, x, y, px, py);
-                else v = rs_sample<R>(P.eng, x, y, px, py);
-            }
+            if (y < py) v = conv_sample<R>(P.eng, x, y, px, py);
             lds[u * stride + lidx(x)] = v;
         }
         __syncthreads();

@@ -14,8 +14,12 @@ enum EngineKind : int {
     ENG_NONE = 3,        // plain 2-D FFT round trip (identity)
     ENG_GS_PROJECT = 4,  // Fourier-plane amplitude projection (FFT-form GS)
     ENG_MEASURE = 5,     // forward only: accumulate intensity statistics
-    ENG_FRAUNHOFER = 6   // forward only: far-field bin map and prefactor, written to the output
+    ENG_FRAUNHOFER = 6,  // forward only: far-field bin map and prefactor, written to the output
+    ENG_FRESNEL_CONV = 7 // Fresnel convolution form: the RS pipeline with the paraxial kernel
 };
+// Engines that multiply by the spectrum of a sampled kernel (kernel
+// generation pass + product in the column pass).
+__host__ __device__ inline bool conv_engine(int kind) { return kind == ENG_RS || kind == ENG_FRESNEL_CONV; }
 // Element-wise op fused into the first pass's loads.
 // PRE_TMUL multiplies by a precomputed transmission (a phase mask shared by
 // several problems of a launch, evaluated once: it does not depend on the
@@ -73,6 +77,8 @@ struct EngineDev {
     // ramp frequencies, far-field pitch and axis, pi/(lambda z), and the
     // prefactor e^{ikz}/(i lambda z)
     double fcx, fcy, fs, gs, dxo, dyo, xs, ys, chirp, quad_re, quad_im;
+    // Fresnel convolution kernel (fresnel.rs:312-333): pref = e^{ikz}/(i lambda z) dx dy
+    double pref_re, pref_im;
 };
 
 // What a launch of the power-of-two kernels holds uniformly, selecting the

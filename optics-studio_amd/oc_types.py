@@ -40,7 +40,7 @@ STATUS = {
 CODE = {v: k for k, v in STATUS.items()}
 
 APERTURE_FIELD, APERTURE_CIRCLE, APERTURE_ELLIPSE, APERTURE_RECTANGLE = 0, 1, 2, 3
-ENGINE_ASM, ENGINE_FRESNEL, ENGINE_RS = 0, 1, 2
+ENGINE_ASM, ENGINE_FRESNEL, ENGINE_RS, ENGINE_FRESNEL_CONV = 0, 1, 2, 7
 
 
 class BandGeom(C.Structure):

@@ -56,6 +56,12 @@ EXPORTS = {
     "oc_asm_propagate_dev_c64": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), P(T.PhaseMask), vp,
                                   P(T.AsmReport)], C.c_int),
     "oc_fresnel_sampling_report": ([P(T.BandGeom), c_f64, P(T.FresnelConfig), P(T.FresnelReport)], C.c_int),
+    "oc_fresnel_propagate_convolution": ([vp, P(T.BandGeom), vp, c_f64, P(T.FresnelConfig), vp,
+                                          P(T.FresnelReport)], C.c_int),
+    "oc_fresnel_propagate_convolution_dev_c128": ([vp, P(T.BandGeom), vp, c_f64, P(T.FresnelConfig), vp,
+                                                   P(T.FresnelReport)], C.c_int),
+    "oc_fresnel_propagate_convolution_dev_c64": ([vp, P(T.BandGeom), vp, c_f64, P(T.FresnelConfig), vp,
+                                                  P(T.FresnelReport)], C.c_int),
     "oc_fraunhofer_regime_report": ([P(T.BandGeom), c_f64, P(T.FraunhoferConfig), P(T.FraunhoferReport)],
                                     C.c_int),
     "oc_fraunhofer_propagate": ([vp, P(T.BandGeom), vp, c_f64, P(T.FraunhoferConfig), vp,
@@ -225,6 +231,17 @@ def fresnel_propagate(u, dx, dy, lam, z, cfg, ctx=None):
     return st, out, rep
 
 
+def fresnel_propagate_convolution(u, dx, dy, lam, z, cfg, ctx=None):
+    """fresnel::propagate_convolution, host f64."""
+    ctx = ctx or context()
+    u = _c128(u)
+    out = np.empty_like(u)
+    rep = T.FresnelReport()
+    st = lib.oc_fresnel_propagate_convolution(ctx.h, C.byref(_geom(u, dx, dy, lam)), u.ctypes.data, z,
+                                              C.byref(cfg), out.ctypes.data, C.byref(rep))
+    return st, out, rep
+
+
 def fraunhofer_propagate(u, dx, dy, lam, z, cfg, ctx=None):
     """fraunhofer::propagate, host f64; the far field's pitch is
     (report.dx_out, report.dy_out)."""
@@ -355,6 +372,18 @@ def fresnel_propagate_dev(u_dev, dx, dy, lam, z, cfg, mask=None, out=None, ctx=N
     return st, out, rep
 
 
+def fresnel_propagate_convolution_dev(u_dev, dx, dy, lam, z, cfg, out=None, ctx=None):
+    torch = _torch()
+    ctx = ctx or context()
+    out = torch.empty_like(u_dev) if out is None else out
+    ny, nx = u_dev.shape
+    fn = (lib.oc_fresnel_propagate_convolution_dev_c64 if u_dev.dtype == torch.complex64
+          else lib.oc_fresnel_propagate_convolution_dev_c128)
+    rep = T.FresnelReport()
+    st = _dev_call(fn, u_dev, out, T.geom(nx, ny, dx, dy, lam), z, cfg, rep, False, ctx)
+    return st, out, rep
+
+
 def fraunhofer_propagate_dev(u_dev, dx, dy, lam, z, cfg, out=None, ctx=None):
     torch = _torch()
     ctx = ctx or context()
@@ -425,6 +454,10 @@ class DevEngine:
 
     def rs_propagate(self, u, dx, dy, lam, z, cfg):
         st, out, rep = rs_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
+        return st, self._down(out), rep
+
+    def fresnel_propagate_convolution(self, u, dx, dy, lam, z, cfg):
+        st, out, rep = fresnel_propagate_convolution_dev(self._up(u), dx, dy, lam, z, cfg)
         return st, self._down(out), rep
 
     def fraunhofer_propagate(self, u, dx, dy, lam, z, cfg):

@@ -442,6 +442,64 @@ def mask_outside_field_rejected(eng, tol=1.0):
 
 
 # --------------------------------------------------------------------------
+# Fresnel convolution form: src/propagation/fresnel.rs:531-590, 800-913
+# --------------------------------------------------------------------------
+
+def fresnel_conv_delta_is_kernel(eng, tol=1.0):
+    """fresnel.rs:800-845: delta -> d^2/(i lam z) e^{ikz} chirp, |h| flat."""
+    n, d, z, lam = 32, 1e-5, 1e-3, 500e-9
+    u = np.zeros((n, n), np.complex128)
+    u[0, 0] = 1
+    st, out, _ = eng.fresnel_propagate_convolution(u, d, d, lam, z, T.FresnelConfig(1.0, 0, 0, 1.0))
+    ok(st)
+    amp = d * d / (lam * z)
+    k = 2 * math.pi / lam
+    pref = complex(math.cos(k * z), math.sin(k * z)) / (1j * lam * z) * d * d
+    b = 1e-12 if tol == 1.0 else tol * amp
+    assert abs(out[0, 0] - pref) < b
+    for iy in [0, 11, 16]:
+        for ix in [1, 5, 16, 31]:
+            s_ = out[iy, ix]
+            assert abs(abs(s_) - amp) < b
+            x = (ix if ix <= n // 2 else ix - n) * d
+            y = (iy if iy <= n // 2 else iy - n) * d
+            ch = math.pi * (x * x + y * y) / (lam * z)
+            assert abs(s_ / out[0, 0] - complex(math.cos(ch), math.sin(ch))) < lim(1e-9, tol)
+
+
+def fresnel_conv_equals_tf_at_critical_sampling(eng, tol=1.0):
+    """fresnel.rs:878-913."""
+    n, z, lam = 256, 2e-3, 500e-9
+    d = math.sqrt(lam * z / n)
+    u = gaussian(n, n, d, 50 * d)
+    cfg = T.FresnelConfig(1.0, 1, 0, 1.0)
+    st, tf, _ = eng.fresnel_propagate(u, d, d, lam, z, cfg)
+    ok(st)
+    st, cv, _ = eng.fresnel_propagate_convolution(u, d, d, lam, z, cfg)
+    ok(st)
+    assert rel_frob(cv, tf) < lim(1e-6, tol)
+
+
+def fresnel_conv_guard_and_identity(eng, tol=1.0):
+    """fresnel.rs:531-553: z = 0 returns the input; strict checks the kernel
+    sampling and never densifies."""
+    u = gaussian(32, 32, 1e-5, 6e-5) * np.exp(0.3j)
+    st, out, _ = eng.fresnel_propagate_convolution(u, 1e-5, 1e-5, LAMBDA, 0.0, T.FresnelConfig(2.0, 1, 0, 8.0))
+    ok(st)
+    assert np.array_equal(out, in_engine_type(u, tol))
+    # fine pitch, short z: the chirp kernel is undersampled
+    st, _, rep = eng.fresnel_propagate_convolution(u, 1e-6, 1e-6, LAMBDA, 1e-4, T.FresnelConfig(2.0, 1, 0, 8.0))
+    assert st == T.CODE["OC_ALIASING_RISK"]
+    assert not rep.kernel_nyquist_ok and rep.used_padding == 2.0
+    st, _, _ = eng.fresnel_propagate_convolution(u, 1e-6, 1e-6, LAMBDA, 1e-4, T.FresnelConfig(2.0, 0, 0, 8.0))
+    ok(st)
+    bad = u.copy()
+    bad[3, 3] = complex(float("nan"), 0.0)
+    st, _, _ = eng.fresnel_propagate_convolution(bad, 1e-6, 1e-6, LAMBDA, 1e-4, T.FresnelConfig(2.0, 1, 0, 8.0))
+    assert st == T.CODE["OC_NON_FINITE_SAMPLE"]
+
+
+# --------------------------------------------------------------------------
 # Fraunhofer: src/propagation/fraunhofer.rs:465-727 (the reference's tests)
 # --------------------------------------------------------------------------
 
@@ -599,7 +657,9 @@ ASM_CASES = [asm_z_zero_identity, asm_plane_wave_picks_up_kz, asm_back_propagati
              asm_tilted_plane_wave_ramp, asm_round_trip_padded, asm_guard_and_mitigation,
              asm_adjoint_inner_product]
 FRESNEL_CASES = [fresnel_z_zero_identity, fresnel_plane_wave_bin_phase, fresnel_round_trip,
-                 fresnel_strict_flags_and_mitigates, fresnel_invalid_inputs, fresnel_gaussian_vs_asm_paraxial]
+                 fresnel_strict_flags_and_mitigates, fresnel_invalid_inputs, fresnel_gaussian_vs_asm_paraxial,
+                 fresnel_conv_delta_is_kernel, fresnel_conv_equals_tf_at_critical_sampling,
+                 fresnel_conv_guard_and_identity]
 RS_CASES = [rs_delta_is_sampled_kernel, rs_invalid_inputs, rs_strict_guard, rs_matches_direct_quadrature,
             rs_agrees_with_asm_smooth_band]
 FRAUNHOFER_CASES = [fraunhofer_delta_flat_amplitude_quadratic_phase, fraunhofer_uniform_concentrates_at_dc,

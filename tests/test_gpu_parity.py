@@ -239,3 +239,23 @@ def test_fraunhofer_parity(G, c64, c128, orc, shape, z, src):
         e = rel_l2(out, ref)
         say("  ", name, e)
         assert e <= tol, (name, e)
+
+
+FC_SHAPES = [(64, 64), (37, 101), (200, 200), (512, 512)]
+
+
+@pytest.mark.parametrize("shape", FC_SHAPES, ids=[f"{s[0]}x{s[1]}" for s in FC_SHAPES])
+def test_fresnel_convolution_parity(G, c64, c128, orc, shape):
+    u = rand_field(shape, 41 + shape[0] * 3 + shape[1])
+    d, lam, z = 3e-6, 600e-9, 8e-3
+    cfg = T.FresnelConfig(2.0, 0, 0, 2.0)
+    say("fresnel conv", shape)
+    st, ref, rref = orc.fresnel_propagate_convolution(u, d, d, lam, z, cfg)
+    assert st == 0
+    for name, eng, tol in (("f64", G, F64_TOL), ("c128", c128, F64_TOL), ("c64", c64, C64_TOL)):
+        st, out, rep = eng.fresnel_propagate_convolution(u, d, d, lam, z, cfg)
+        assert st == 0, (name, T.STATUS[st])
+        assert T.report_dict(rep) == T.report_dict(rref)
+        e = rel_l2(out, ref)
+        say("  ", name, e)
+        assert e <= tol, (name, e)
