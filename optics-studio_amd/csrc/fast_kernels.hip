@@ -328,6 +328,16 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
                             if (P.ops.dphi_out) at32(P.ops.dphi_out, k) = dphi;
                         }
                         if (P.out) at32(out, k) = o;
+                    } else if constexpr (QK == POST_ADJ_GRAD) {
+                        // adjoint.rs:222-236 with t and V from the element's mask
+                        const unsigned k = o0 + x;
+                        const cpx<R> tt = apply_mask(P.mask, x, iy, cpx<R>{R(1), R(0)});
+                        cpx<R> ck;
+                        ck = at32(static_cast<const cpx<R>*>(P.ops.ckpt), k);
+                        const cpx<R> vv = cmul(ck, tt);
+                        const double gim = (double)o.re * (double)vv.im + (-(double)o.im) * (double)vv.re;
+                        at32(P.ops.grad, k) += -2.0 * gim;
+                        at32(out, k) = cmul(o, cpx<R>{tt.re, -tt.im});
                     } else {
                         store_post<R>(P, x, iy, o, acc0, acc1, acc2);
                     }
@@ -572,6 +582,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                 hipLaunchKernelGGL((f_row_inv<R, N, V, POST_NONE>), g, brow, row_lds, st, d, np, total);
             else if (s.post_kind == POST_GS_PHASE)
                 hipLaunchKernelGGL((f_row_inv<R, N, V, POST_GS_PHASE>), g, brow, row_lds, st, d, np, total);
+            else if (s.post_kind == POST_ADJ_GRAD)
+                hipLaunchKernelGGL((f_row_inv<R, N, V, POST_ADJ_GRAD>), g, brow, row_lds, st, d, np, total);
             else
                 hipLaunchKernelGGL((f_row_inv<R, N, V, -1>), g, brow, row_lds, st, d, np, total);
             break;
@@ -599,6 +611,7 @@ static hipError_t attr_nv() {
                        (const void*)f_col<R, N, V, false, -1>,
                        (const void*)f_row_inv<R, N, V, POST_NONE>,
                        (const void*)f_row_inv<R, N, V, POST_GS_PHASE>,
+                       (const void*)f_row_inv<R, N, V, POST_ADJ_GRAD>,
                        (const void*)f_row_inv<R, N, V, -1>,
                        (const void*)f_rs_kgen<R, N, V>};
     for (const void* f : k) {

@@ -851,7 +851,8 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             b.pc[3].block = std::max(b.pc[3].block, brow);
             b.pc[3].lds = std::max(b.pc[3].lds, L.row_bytes * L.cnt);
         }
-        const int qk = (d.ops.post == POST_NONE || d.ops.post == POST_GS_PHASE) ? d.ops.post : -1;
+        const int qk =
+            (d.ops.post == POST_NONE || d.ops.post == POST_GS_PHASE || d.ops.post == POST_ADJ_GRAD) ? d.ops.post : -1;
         b.post_kind = b.post_kind == -2 ? qk : (b.post_kind == qk ? qk : -1);
         const int ek = (p.engine == ENG_ASM || p.engine == ENG_FRESNEL) ? p.engine : -1;
         b.col_kind = b.col_kind == -2 ? ek : (b.col_kind == ek ? ek : -1);

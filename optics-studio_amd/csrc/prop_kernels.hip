@@ -499,7 +499,9 @@ hipError_t launch_block_reorder(bool dbl, const void* src, void* dst, int nx, in
 
 hipError_t launch_seed_mse(bool dbl, const void* u, const double* target, void* out, size_t n, double* sums,
                            hipStream_t st) {
-    int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
+    // few enough workgroups that the per-wave atomics into one accumulator
+    // do not serialise (2048 waves)
+    int grid = (int)std::min<size_t>((n + 255) / 256, 512);
     if (grid < 1) grid = 1;
     if (dbl)
         hipLaunchKernelGGL(k_seed_mse<double>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<double>*>(u),
