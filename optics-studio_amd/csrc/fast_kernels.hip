@@ -168,7 +168,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     }
 }
 
-// EK: engine (ENG_ASM, ENG_FRESNEL, or -1 for the descriptor's); RS: the
+// EK: engine (ENG_ASM, ENG_FRESNEL, ENG_GS_PROJECT, or -1 for the descriptor's); RS: the
 // Rayleigh-Sommerfeld product with the kernel spectrum instead.
 template <typename R, int N, int V, bool RS, int EK>
 __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(const ProbDev* __restrict__ probs,
@@ -315,7 +315,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
                         const unsigned k = o0 + x;
                         if constexpr (sizeof(R) == 4) {
                             const float p = at32(ph, k);
-                            const float dphi = wrap_pi_f(atan2f(o.im, o.re) - p);
+                            const float dphi = wrap_pi_f(atan2_f(o.im, o.re) - p);
                             at32(ph, k) =
                                 encode_bounds_f(p + (float)P.ops.beta_p * dphi, (float)P.ops.lo, (float)P.ops.hi);
                             acc2 += (double)dphi * dphi;
@@ -347,6 +347,107 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
         if (bad) atomicOr(P.flag, 2);
         if (QK == POST_GS_PHASE || (QK < 0 && (P.ops.post == POST_GS_TARGET || P.ops.post == POST_GS_PHASE)))
             accumulate4(acc0, acc1, acc2, 0.0, P.ops.sums);
+    }
+}
+
+// One GS iteration's inverse row pass fused with the next iteration's
+// forward row pass (gs.rs:484-492, 557-567): row IFFT, scale, then
+//  FULL:  phase update phi <- encode(phi + beta wrap(arg V - phi)) and the
+//         new source field A e^{i phi} (masked by the stop); the phase
+//         written is the one read back, as the separate passes would;
+//  !FULL: beta = 1 with bounds spanning 2 pi, where the updated phase is
+//         arg V mod 2 pi and A e^{i phi} = A V / |V| (V = 0: arg is 0 or pi
+//         by the sign bit of Re V): the source field without the phase,
+//         which the last iteration's inverse pass writes;
+// then the row FFT, stored back in place.
+template <typename R, int N, int V, bool FULL>
+__global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(const ProbDev* __restrict__ probs,
+                                                                        int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, G = LY::G, Q = LY::B;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
+    const int t = tb + Q * tq;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
+        const int iy = (tile - P.base_row) * G + g;
+        const bool rowok = iy < P.ny;
+        const int ny = P.ny, nx = P.nx, ox = P.ox;
+        cpx<R>* work = static_cast<cpx<R>*>(P.work);
+        const unsigned b0 = ((unsigned)tq * ny + iy) * Q + tb;
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            v[m] = {R(0), R(0)};
+            if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
+        }
+        fft_regs<R, N, G, +1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        const R s = (R)P.scale;
+        bool bad = false;
+        double acc2 = 0.0;
+        R* ph = static_cast<R*>(P.ops.phase_io);
+        const R* amp = static_cast<const R*>(P.ops.amp);
+        const unsigned char* open = P.ops.open;
+        cpx<R>* out = static_cast<cpx<R>*>(P.out);
+        const unsigned o0 = (unsigned)iy * nx;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int x = t + m * T - ox;
+            if (rowok && x >= 0 && x < nx && !FULL) {
+                const cpx<R> o = {v[m].re * s, v[m].im * s};
+                bad |= !finite_c(o);
+                const unsigned k = o0 + x;
+                R a = at32(amp, k);
+                if (open && !at32(open, k)) a = R(0);
+                if constexpr (sizeof(R) == 4) {
+                    const float r2 = fmadd(o.re, o.re, o.im * o.im);
+                    const float sc = a * __builtin_amdgcn_rsqf(r2);
+                    v[m] = r2 > 0.0f ? cpx<R>{o.re * sc, o.im * sc}
+                                     : cpx<R>{__builtin_signbit(o.re) ? -a : a, R(0)};
+                } else {
+                    const double mg = hypot(o.re, o.im);
+                    v[m] = mg > 0.0 ? cpx<R>{o.re * (a / mg), o.im * (a / mg)}
+                                    : cpx<R>{__builtin_signbit(o.re) ? -a : a, R(0)};
+                }
+            } else if (rowok && x >= 0 && x < nx) {
+                const cpx<R> o = {v[m].re * s, v[m].im * s};
+                bad |= !finite_c(o);
+                const unsigned k = o0 + x;
+                R pn;
+                if constexpr (sizeof(R) == 4) {
+                    const float p = at32(ph, k);
+                    const float dphi = wrap_pi_f(atan2_f(o.im, o.re) - p);
+                    pn = encode_bounds_f(p + (float)P.ops.beta_p * dphi, (float)P.ops.lo, (float)P.ops.hi);
+                    acc2 += (double)dphi * dphi;
+                    if (P.ops.dphi_out) at32(P.ops.dphi_out, k) = dphi;
+                } else {
+                    const double p = at32(ph, k);
+                    const double dphi = wrap_pi(atan2((double)o.im, (double)o.re) - p);
+                    pn = (R)encode_bounds(p + P.ops.beta_p * dphi, P.ops.lo, P.ops.hi);
+                    acc2 += dphi * dphi;
+                    if (P.ops.dphi_out) at32(P.ops.dphi_out, k) = dphi;
+                }
+                at32(ph, k) = pn;
+                if (out) at32(out, k) = o;
+                // the next iteration's source sample (the phase just written
+                // is the one its forward pass reads)
+                R a = at32(amp, k);
+                if (open && !at32(open, k)) a = R(0);
+                const cpx<R> e = unit_phase<R>((double)pn);
+                v[m] = {a * e.re, a * e.im};
+            } else {
+                v[m] = {R(0), R(0)};
+            }
+        }
+        if (bad) atomicOr(P.flag, 2);
+        if (FULL) accumulate4(0.0, 0.0, acc2, 0.0, P.ops.sums);
+        fft_regs<R, N, G, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        if (rowok) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q) = v[m];
+        }
     }
 }
 
@@ -574,6 +675,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                 hipLaunchKernelGGL((f_col<R, N, V, false, ENG_ASM>), g, bcol, col_lds, st, d, np, total);
             else if (s.col_kind == ENG_FRESNEL)
                 hipLaunchKernelGGL((f_col<R, N, V, false, ENG_FRESNEL>), g, bcol, col_lds, st, d, np, total);
+            else if (s.col_kind == ENG_GS_PROJECT)
+                hipLaunchKernelGGL((f_col<R, N, V, false, ENG_GS_PROJECT>), g, bcol, col_lds, st, d, np, total);
             else
                 hipLaunchKernelGGL((f_col<R, N, V, false, -1>), g, bcol, col_lds, st, d, np, total);
             break;
@@ -588,6 +691,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                 hipLaunchKernelGGL((f_row_inv<R, N, V, -1>), g, brow, row_lds, st, d, np, total);
             break;
         case 3: hipLaunchKernelGGL((f_rs_kgen<R, N, V>), g, brow, row_lds, st, d, np, total); break;
+        case 4: hipLaunchKernelGGL((f_row_gs<R, N, V, true>), g, brow, row_lds, st, d, np, total); break;
+        case 5: hipLaunchKernelGGL((f_row_gs<R, N, V, false>), g, brow, row_lds, st, d, np, total); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -608,12 +713,15 @@ static hipError_t attr_nv() {
                        (const void*)f_col<R, N, V, true, -1>,
                        (const void*)f_col<R, N, V, false, ENG_ASM>,
                        (const void*)f_col<R, N, V, false, ENG_FRESNEL>,
+                       (const void*)f_col<R, N, V, false, ENG_GS_PROJECT>,
                        (const void*)f_col<R, N, V, false, -1>,
                        (const void*)f_row_inv<R, N, V, POST_NONE>,
                        (const void*)f_row_inv<R, N, V, POST_GS_PHASE>,
                        (const void*)f_row_inv<R, N, V, POST_ADJ_GRAD>,
                        (const void*)f_row_inv<R, N, V, -1>,
-                       (const void*)f_rs_kgen<R, N, V>};
+                       (const void*)f_rs_kgen<R, N, V>,
+                       (const void*)f_row_gs<R, N, V, true>,
+                       (const void*)f_row_gs<R, N, V, false>};
     for (const void* f : k) {
         const hipError_t e = allow_lds(f);
         if (e != hipSuccess) return e;

@@ -854,7 +854,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         const int qk =
             (d.ops.post == POST_NONE || d.ops.post == POST_GS_PHASE || d.ops.post == POST_ADJ_GRAD) ? d.ops.post : -1;
         b.post_kind = b.post_kind == -2 ? qk : (b.post_kind == qk ? qk : -1);
-        const int ek = (p.engine == ENG_ASM || p.engine == ENG_FRESNEL) ? p.engine : -1;
+        const int ek = (p.engine == ENG_ASM || p.engine == ENG_FRESNEL || p.engine == ENG_GS_PROJECT) ? p.engine : -1;
         b.col_kind = b.col_kind == -2 ? ek : (b.col_kind == ek ? ek : -1);
         const int pk =
             (d.ops.pre == PRE_NONE || d.ops.pre == PRE_MASK || d.ops.pre == PRE_TMUL || d.ops.pre == PRE_POLAR)
@@ -893,6 +893,12 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
 }
 
 constexpr unsigned kAllPasses = 0xF;
+// pass 4 (power-of-two batches with a GS phase post-op): one iteration's
+// inverse row pass fused with the next one's forward row pass
+constexpr int PASS_GS_ROW = 4;
+// pass 5: the same with beta = 1 and 2 pi wide bounds (amplitude replacement;
+// the phases are left to the last iteration's inverse pass)
+constexpr int PASS_GS_AMP = 5;
 // timing slots of the separable Fresnel passes (see OC_NUM_PASS_KINDS)
 constexpr int PASS_T_SEP_COL = 6, PASS_T_SEP_ROW = 7;
 
@@ -902,13 +908,16 @@ int launch_batch(oc_context* ctx, Batch& b, unsigned mask = kAllPasses) {
     if (b.pd.empty()) return OC_OK;
     const int nprob = (int)b.pd.size();
     const ProbDev* dprobs = static_cast<const ProbDev*>(b.dprobs);
-    const int order[4] = {3, 0, 1, 2};
-    for (int k = 0; k < 4; ++k) {
+    const int order[6] = {3, 0, 1, 2, PASS_GS_ROW, PASS_GS_AMP};
+    for (int k = 0; k < 6; ++k) {
         const int pass = order[k];
-        if (!(mask & (1u << pass)) || b.pc[pass].total <= 0) continue;
-        const int grid = std::min(b.pc[pass].total, ctx->num_cu * 8);
-        const PassCfg& c = b.pc[pass];
-        const int tpass = b.sep ? (pass == 1 ? PASS_T_SEP_COL : PASS_T_SEP_ROW) : pass;
+        if (!(mask & (1u << pass))) continue;
+        const bool fused = pass == PASS_GS_ROW || pass == PASS_GS_AMP;
+        if (fused && !b.fast) return fail(ctx, OC_ERR_UNSUPPORTED, "fused GS row pass needs N = 2^k");
+        const PassCfg& c = b.pc[fused ? 0 : pass];
+        if (c.total <= 0) continue;
+        const int grid = std::min(c.total, ctx->num_cu * 8);
+        const int tpass = b.sep ? (pass == 1 ? PASS_T_SEP_COL : PASS_T_SEP_ROW) : (fused ? 4 : pass);
         hipError_t e = timed(ctx, tpass, [&] {
             if (b.fast) {
                 const ocd::FastSel sel{pass,       b.fast_n,   b.fast_v,     b.rs ? 1 : 0, b.post_kind,
@@ -1597,7 +1606,50 @@ int gs_fft_impl(oc_context* ctx, bool dbl, uint64_t n, uint64_t R, const void* A
     }
     const unsigned passes3 = (1u << 0) | (1u << 1) | (1u << 2);
     uint64_t done = 0;
-    if (chunk > 0 && iterations >= chunk) {
+    if (bi.fast && iterations > 0) {
+        // power-of-two grids: row_fwd once, then per iteration the column
+        // pass and the fused row pass (inverse rows + phase update + next
+        // forward rows); the last iteration ends with the plain inverse
+        // rows.  Graph-captured in chunks of [col, fused row].
+        // beta = 1 over [0, 2 pi) with no per-iteration outputs: amplitude
+        // replacement in the fused pass, phases from the last inverse pass
+        const bool amp_only = it[0].ops.beta_p == 1.0 && it[0].ops.hi - it[0].ops.lo >= 2.0 * och::PI &&
+                              !it[0].ops.sums && !it[0].ops.dphi_out && !it[0].out;
+        const unsigned cf = (1u << 1) | (1u << (amp_only ? PASS_GS_AMP : PASS_GS_ROW));
+        st = launch_batch(ctx, bi, 1u << 0);
+        if (st) { cleanup(); return st; }
+        const uint64_t body = iterations - 1;  // iterations with a fused row pass
+        if (chunk > 0 && body >= chunk) {
+            const bool was = ctx->timing;
+            ctx->timing = false;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+            if (e != hipSuccess) { ctx->timing = was; cleanup(); return fail(ctx, OC_ERR_DEVICE, "begin capture"); }
+            for (uint64_t k = 0; k < chunk && st == OC_OK; ++k) st = launch_batch(ctx, bi, cf);
+            e = hipStreamEndCapture(ctx->stream, &graph);
+            ctx->timing = was;
+            if (st || e != hipSuccess) { cleanup(); return st ? st : fail(ctx, OC_ERR_DEVICE, "end capture"); }
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) { (void)hipGraphDestroy(graph); cleanup(); return fail(ctx, OC_ERR_DEVICE, "instantiate"); }
+            while (body - done >= chunk) {
+                e = timed(ctx, 4, [&] { return hipGraphLaunch(exec, ctx->stream); });
+                if (e != hipSuccess) break;
+                done += chunk;
+            }
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) { cleanup(); return fail(ctx, OC_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e)); }
+        }
+        for (; done < body; ++done) {
+            st = launch_batch(ctx, bi, cf);
+            if (st) { cleanup(); return st; }
+        }
+        st = launch_batch(ctx, bi, (1u << 1) | (1u << 2));
+        if (st) { cleanup(); return st; }
+        done = iterations;
+    }
+    if (chunk > 0 && iterations - done >= chunk) {
         // capture `chunk` iterations once, replay
         const bool was = ctx->timing;
         ctx->timing = false;

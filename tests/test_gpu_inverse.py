@@ -136,6 +136,40 @@ def test_gs_fft_graph_equals_eager(G):
     assert np.allclose(e1, e2, rtol=1e-12, atol=0.0)
 
 
+@pytest.mark.parametrize("dbl", [False, True])
+def test_gs_fft_fused_rows_equal_three_pass(G, dbl):
+    """Power-of-two grids run each iteration as the column pass plus one row
+    pass that fuses the inverse rows, the phase update and the next forward
+    rows; the generic kernels run the three passes apart.  Same iteration,
+    different FFT kernels: the phases agree to rounding."""
+    import torch
+    n, R, its = 256, 2, 7
+    amp, tgt, ph = W.workload3(n=n, restarts=R, spots=8)
+    tgt = np.fft.ifftshift(tgt)
+    dt = np.float64 if dbl else np.float32
+    a = torch.as_tensor(amp.astype(dt)).cuda()
+    t = torch.as_tensor(tgt.astype(dt)).cuda()
+    fused = torch.as_tensor(ph.astype(dt)).cuda()
+    split = fused.clone()
+    say("gs_fft fused vs three-pass, double", dbl)
+    st, e1 = G.gs_fft_run_dev(a, t, fused, its, 3)
+    assert st == 0
+    gen = G.Context(0)
+    gen.force_generic(True)
+    st, e2 = G.gs_fft_run_dev(a, t, split, its, 3, ctx=gen)
+    assert st == 0
+    p1 = fused.cpu().numpy().astype(np.float64)
+    p2 = split.cpu().numpy().astype(np.float64)
+    tol = 1e-9 if dbl else 1e-4
+    for r in range(R):
+        f1 = amp * np.exp(1j * p1[r])
+        f2 = amp * np.exp(1j * p2[r])
+        e = np.linalg.norm(f1 - f2) / np.linalg.norm(f2)
+        say("  restart", r, e, e1[r], e2[r])
+        assert e < tol
+        assert abs(e1[r] - e2[r]) < tol
+
+
 def design(G, orc, n, d, z, seeds=(11, 12)):
     p1 = 2 * math.pi * W.splitmix_uniform(seeds[0], n * n).reshape(n, n)
     p2 = 2 * math.pi * W.splitmix_uniform(seeds[1], n * n).reshape(n, n)
