@@ -195,14 +195,16 @@ def bench_rs(G, W, T, torch, np):
     return r
 
 
-def read_traffic():
-    """HBM bytes per launch of the column pass from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
+def read_traffic(variant):
+    """HBM bytes per launch of the column pass for the layout variant in use,
+    from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE,
+    copy-kernel calibrated), if any."""
     p = os.path.join(REPO, "profiles", "traffic_col_c64_workload2.txt")
     if not os.path.exists(p):
         return None
+    key = f"traffic_bytes_per_launch_v{variant}="
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch="):
+        if line.startswith(key):
             return float(line.split("=", 1)[1])
     return None
 
@@ -297,7 +299,7 @@ def main():
     dom = max(alg, key=lambda k: timing[k][0])
     avg_ms = timing[dom][0] / max(timing[dom][1], 1)
     achieved = alg[dom] / (avg_ms * 1e-3) / 1e9
-    traffic = read_traffic() if dom == "col" else None
+    traffic = read_traffic(variant) if dom == "col" else None
     passes = {k: {"avg_ms": timing[k][0] / max(timing[k][1], 1), "launches": timing[k][1],
                   "alg_GBps": alg[k] / (timing[k][0] / max(timing[k][1], 1) * 1e-3) / 1e9}
               for k in alg if timing[k][1]}

@@ -66,8 +66,16 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
-        const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
-        const int iy = (tile - P.base_row) * G + g;
+        int pi, lt;
+        if (PK == PRE_TMUL && probs[0].ilv) {
+            pi = tile % nprob;
+            lt = tile / nprob;
+        } else {
+            pi = find_problem<PASS_ROW>(probs, nprob, tile);
+            lt = tile - probs[pi].base_row;
+        }
+        const ProbDev& P = probs[pi];
+        const int iy = lt * G + g;
         const bool rowok = iy < P.ny;
         const int ox = P.ox, nx = P.nx;
         bool bad = false;

@@ -870,6 +870,11 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     b.pc[3].total = base_kgen;
     b.pc[2] = b.pc[0];
     if (b.sep) b.pc[0].total = 0;  // pass 1: columns, pass 2: rows
+    if (b.fast && !b.sep && b.pre_kind == PRE_TMUL) {
+        bool even = true;
+        for (const ProbDev& d : b.pd) even = even && d.tiles_row == b.pd[0].tiles_row;
+        for (ProbDev& d : b.pd) d.ilv = even ? 1 : 0;
+    }
     bool fraun = true;
     for (const ProbDev& d : b.pd) fraun = fraun && d.eng.kind == ENG_FRAUNHOFER;
     if (fraun) b.pc[2].total = 0;  // the column pass writes the far field

@@ -132,6 +132,11 @@ struct ProbDev {
     int B;         // column block width of the intermediate layout
     int base_row, base_col, base_kgen;  // tile prefix offsets per pass
     int tiles_row, tiles_col, tiles_kgen;
+    // set on every problem of a batch whose row pass reads one shared
+    // transmission and whose problems have equal row tile counts: row tile
+    // k of problem p is launch tile k*nprob + p, so that the problems' same
+    // rows run together and share the transmission rows in L2
+    int ilv;
     FftPlanDev prow, pcol;
     const void* in;
     void* out;
