@@ -557,6 +557,51 @@ int oc_adjoint_grid_gradient_dev_c128(oc_context* ctx,
                                       const double* target, double* dphi_out,
                                       double* merit_out);
 
+/* Renderer::render (crates/optics-inverse/src/observe.rs:213-314) for
+   designs whose elements are PhaseMasks.  The caller keeps the two closures
+   of the Rust renderer: it builds the design for every wavelength the
+   render needs (Carrier::build) and evaluates the source for every render
+   (the Source closure), then this call runs every frame and FD render as
+   one batch per beamline step and writes the Observation data.
+
+   Renders per (band, scene), in this order: base; (x0 + h, x0 - h) if
+   deriv_x; (y0 + h, y0 - h) if deriv_y; (lambda + dl, lambda - dl) if
+   deriv_lambda (h = scene_step, dl = lambda_step).
+   inputs: [band][scene][render] fields of geom's grid (complex, row-major).
+   designs: [band][variant][nsteps] with variant 0 = the band's wavelength
+   and, when deriv_lambda, 1 = lambda + dl, 2 = lambda - dl.
+   obs: [channel][band][scene][ny][nx] f64, channel 0 = |U|^2, then the
+   enabled derivatives in x, y, lambda order, each (I+ - I-) / (2 step).
+   geom->lambda is not used: every render's wavelength comes from bands.
+   Errors: OC_INV_RENDER for an empty band or scene list, a non-positive or
+   non-finite step of an enabled derivative, a failed propagation or mask,
+   or a non-finite base intensity (observe.rs:214-231, 296-303). */
+typedef struct oc_render_config {
+    int32_t deriv_x, deriv_y, deriv_lambda, _pad0;
+    double scene_step;  /* m */
+    double lambda_step; /* m */
+} oc_render_config;
+
+/* Host form: inputs (complex128) and obs in host memory. */
+int oc_render(oc_context* ctx, const oc_band_geom* geom, const double* bands,
+              size_t n_bands, size_t n_scenes, const oc_render_config* rc,
+              const oc_design_step* designs, size_t nsteps,
+              const oc_asm_config* cfg, const double* inputs, double* obs);
+/* Device forms: inputs (complex128 / complex64), mask values and obs on
+   the device. */
+int oc_render_dev_c128(oc_context* ctx, const oc_band_geom* geom,
+                       const double* bands, size_t n_bands, size_t n_scenes,
+                       const oc_render_config* rc,
+                       const oc_design_step* designs, size_t nsteps,
+                       const oc_asm_config* cfg, const void* inputs,
+                       double* obs);
+int oc_render_dev_c64(oc_context* ctx, const oc_band_geom* geom,
+                      const double* bands, size_t n_bands, size_t n_scenes,
+                      const oc_render_config* rc,
+                      const oc_design_step* designs, size_t nsteps,
+                      const oc_asm_config* cfg, const void* inputs,
+                      double* obs);
+
 #ifdef __cplusplus
 }
 #endif

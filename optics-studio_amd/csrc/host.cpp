@@ -64,6 +64,8 @@ hipError_t launch_adj_elem(bool dbl, void* co, const void* ckpt, double* grad, c
 hipError_t launch_block_reorder(bool dbl, const void* src, void* dst, int nx, int ny, int B, hipStream_t st);
 hipError_t launch_seed_mse(bool dbl, const void* u, const double* target, void* out, size_t n, double* sums,
                            hipStream_t st);
+hipError_t launch_render_obs(bool dbl, const void* u, double* obs, size_t n, int frames, int nr, int cx, int cy,
+                             int cl, double hs, double hl, int* flag, hipStream_t st);
 }  // namespace ocl
 
 using namespace ocd;
@@ -2016,9 +2018,217 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
     return finish();
 }
 
+// observe.rs:214-231, before any buffer is read.
+int render_check(oc_context* ctx, const oc_band_geom& g, size_t nb, size_t ns, const oc_render_config& rc) {
+    if (nb == 0 || ns == 0) return fail(ctx, OC_INV_RENDER, "renderer needs at least one band and one scene");
+    if ((rc.deriv_x || rc.deriv_y) && (!std::isfinite(rc.scene_step) || rc.scene_step <= 0.0))
+        return fail(ctx, OC_INV_RENDER, "scene_step must be positive and finite");
+    if (rc.deriv_lambda && (!std::isfinite(rc.lambda_step) || rc.lambda_step <= 0.0))
+        return fail(ctx, OC_INV_RENDER, "lambda_step must be positive and finite");
+    if (g.nx * g.ny == 0) return fail(ctx, OC_INV_RENDER, "empty grid");
+    return OC_OK;
+}
+
+// Renderer::render (observe.rs:213-314) over PhaseMask designs.  Every
+// render of every (band, scene) goes through the beamline together: one
+// batch per propagation step (masks fused as the batch's pre-op, shared per
+// band), then one kernel writes the intensity and FD channels.
+int render_impl(oc_context* ctx, bool dbl, const oc_band_geom& g, const double* bands, size_t nb, size_t ns,
+                const oc_render_config& rc, const oc_design_step* designs, size_t nsteps, const oc_asm_config& cfg,
+                const void* inputs, double* obs) {
+    int st = render_check(ctx, g, nb, ns, rc);
+    if (st) return st;
+    const size_t n = g.nx * g.ny;
+    const int nd = (rc.deriv_x ? 1 : 0) + (rc.deriv_y ? 1 : 0) + (rc.deriv_lambda ? 1 : 0);
+    const int nr = 1 + 2 * nd;             // renders per frame
+    const int nv = rc.deriv_lambda ? 3 : 1;  // designs per band
+    const size_t frames = nb * ns, nrend = frames * nr;
+    const size_t esz = dbl ? 16 : 8;
+    // per render: band, design variant, wavelength
+    std::vector<int> rb(nrend), rv(nrend);
+    std::vector<double> rl(nrend);
+    for (size_t f = 0; f < frames; ++f) {
+        const size_t b = f / ns;
+        for (int r = 0; r < nr; ++r) {
+            const size_t i = f * nr + r;
+            rb[i] = (int)b;
+            int v = 0;
+            if (rc.deriv_lambda && r >= nr - 2) v = r == nr - 2 ? 1 : 2;
+            rv[i] = v;
+            rl[i] = v == 0 ? bands[b] : (v == 1 ? bands[b] + rc.lambda_step : bands[b] - rc.lambda_step);
+        }
+    }
+    auto step_of = [&](size_t i, size_t s) -> const oc_design_step& {
+        return designs[((size_t)rb[i] * nv + rv[i]) * nsteps + s];
+    };
+    for (size_t b = 0; b < nb; ++b)
+        for (int v = 0; v < nv; ++v)
+            for (size_t s = 0; s < nsteps; ++s) {
+                const oc_design_step& d = designs[(b * nv + v) * nsteps + s];
+                if (d.has_element && mask_footprint(g, d.mask) != OC_OK)
+                    return fail(ctx, OC_INV_RENDER, "phase mask does not fit the grid");
+            }
+    // one batch per step needs the same beamline structure at every
+    // wavelength (distances and media may differ)
+    for (size_t s = 0; s < nsteps; ++s)
+        for (size_t dv = 1; dv < nb * nv; ++dv) {
+            const oc_design_step& d = designs[dv * nsteps + s];
+            if (d.has_element != designs[s].has_element || (d.z_after == 0.0) != (designs[s].z_after == 0.0))
+                return fail(ctx, OC_ERR_UNSUPPORTED, "designs differ in structure across wavelengths");
+        }
+    void *bufa, *bufb, *fl;
+    if ((st = ensure_buf(ctx, "render_a", nrend * n * esz, &bufa)) ||
+        (st = ensure_buf(ctx, "render_b", nrend * n * esz, &bufb)) ||
+        (st = ensure_buf(ctx, "render_flags", sizeof(int) * (nrend * (nsteps + 1) + 1), &fl)))
+        return st;
+    int* dflags = static_cast<int*>(fl);
+    HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, (int)(nrend * (nsteps + 1) + 1), ctx->stream));
+    const unsigned char* cur = static_cast<const unsigned char*>(inputs);
+    unsigned char* nxt = static_cast<unsigned char*>(bufa);
+    auto advance = [&]() {
+        cur = nxt;
+        nxt = nxt == static_cast<unsigned char*>(bufa) ? static_cast<unsigned char*>(bufb)
+                                                       : static_cast<unsigned char*>(bufa);
+    };
+    // a mask with z_after = 0 is carried into the next propagation's
+    // pre-op (apply, then propagate: the same as apply; identity; apply
+    // nothing; propagate), or applied on its own where none follows
+    std::vector<std::vector<Prob>> queued;
+    queued.reserve(nsteps);
+    int nflag = 0;
+    long pend = -1;  // step whose mask is pending
+    auto flush = [&]() -> int {
+        if (pend < 0) return OC_OK;
+        for (size_t i = 0; i < nrend; ++i) {
+            const oc_design_step& d = step_of(i, (size_t)pend);
+            MaskDev md;
+            int s2 = mask_tables(ctx, g, d.mask, d.mask.values, &md);
+            if (s2) return s2;
+            const hipError_t e = dbl ? ocl::launch_mask_apply_c128(cur + i * n * esz, nxt + i * n * esz, md, (int)g.nx,
+                                                                   (int)g.ny, nullptr, ctx->stream)
+                                     : ocl::launch_mask_apply_c64(cur + i * n * esz, nxt + i * n * esz, md, (int)g.nx,
+                                                                  (int)g.ny, nullptr, ctx->stream);
+            if (e != hipSuccess) return fail(ctx, OC_ERR_DEVICE, std::string("mask apply: ") + hipGetErrorString(e));
+        }
+        advance();
+        pend = -1;
+        return OC_OK;
+    };
+    for (size_t s = 0; s < nsteps; ++s) {
+        const oc_design_step& d0 = designs[s];  // same structure for every design (checked above)
+        if (d0.has_element) {
+            if ((st = flush())) return st;
+            pend = (long)s;
+        }
+        if (d0.z_after == 0.0) continue;
+        std::vector<Prob> probs;
+        probs.reserve(nrend);
+        for (size_t i = 0; i < nrend; ++i) {
+            const oc_design_step& d = step_of(i, s);
+            const oc_design_step* dm = pend >= 0 ? &step_of(i, (size_t)pend) : nullptr;
+            oc_band_geom gi = g;
+            gi.lambda = rl[i];
+            probs.push_back(make_prob(&gi, d.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
+                                      cfg.max_padding, d.medium, dm ? &dm->mask : nullptr, cur + i * n * esz,
+                                      nxt + i * n * esz));
+        }
+        queued.push_back(std::move(probs));
+        st = launch_problems(ctx, dbl, queued.back(), dflags + nflag);
+        if (st) return st;
+        for (const Prob& p : queued.back())
+            if (p.status != OC_OK) return fail(ctx, OC_INV_RENDER, std::string("propagation: ") + oc_status_name(p.status));
+        nflag += (int)nrend;
+        pend = -1;
+        advance();
+    }
+    if ((st = flush())) return st;
+    int* oflag = dflags + nrend * (nsteps + 1);
+    HIP_TRY(ctx, ocl::launch_render_obs(dbl, cur, obs, n, (int)frames, nr, rc.deriv_x, rc.deriv_y, rc.deriv_lambda,
+                                        rc.scene_step, rc.lambda_step, oflag, ctx->stream));
+    int of = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&of, oflag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    int off = 0;
+    for (auto& q : queued) {
+        st = finish_problems(ctx, q, dflags + off);
+        off += (int)q.size();
+        if (st) return fail(ctx, OC_INV_RENDER, std::string("propagation: ") + oc_status_name(st));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (of) return fail(ctx, OC_INV_RENDER, "non-finite intensity");
+    return OC_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+static size_t render_counts(const oc_render_config* rc, size_t nb, size_t ns, size_t* nv, size_t* nch) {
+    const size_t nd = (rc->deriv_x ? 1 : 0) + (rc->deriv_y ? 1 : 0) + (rc->deriv_lambda ? 1 : 0);
+    *nv = rc->deriv_lambda ? 3 : 1;
+    *nch = 1 + nd;
+    return nb * ns * (1 + 2 * nd);
+}
+
+int oc_render_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* bands, size_t nb, size_t ns,
+                       const oc_render_config* rc, const oc_design_step* designs, size_t nsteps,
+                       const oc_asm_config* cfg, const void* inputs, double* obs) {
+    if (!ctx || !g || !rc || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    int st = render_check(ctx, *g, nb, ns, *rc);
+    if (st) return st;
+    if (!bands || (!designs && nsteps) || !inputs || !obs) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    evict_tables(ctx);
+    return render_impl(ctx, true, *g, bands, nb, ns, *rc, designs, nsteps, *cfg, inputs, obs);
+}
+
+int oc_render_dev_c64(oc_context* ctx, const oc_band_geom* g, const double* bands, size_t nb, size_t ns,
+                      const oc_render_config* rc, const oc_design_step* designs, size_t nsteps,
+                      const oc_asm_config* cfg, const void* inputs, double* obs) {
+    if (!ctx || !g || !rc || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    int st = render_check(ctx, *g, nb, ns, *rc);
+    if (st) return st;
+    if (!bands || (!designs && nsteps) || !inputs || !obs) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    evict_tables(ctx);
+    return render_impl(ctx, false, *g, bands, nb, ns, *rc, designs, nsteps, *cfg, inputs, obs);
+}
+
+int oc_render(oc_context* ctx, const oc_band_geom* g, const double* bands, size_t nb, size_t ns,
+              const oc_render_config* rc, const oc_design_step* designs, size_t nsteps, const oc_asm_config* cfg,
+              const double* inputs, double* obs) {
+    if (!ctx || !g || !rc || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    int st = render_check(ctx, *g, nb, ns, *rc);
+    if (st) return st;
+    if (!bands || (!designs && nsteps) || !inputs || !obs) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    evict_tables(ctx);
+    size_t nv, nch;
+    const size_t nrend = render_counts(rc, nb, ns, &nv, &nch);
+    const size_t n = g->nx * g->ny;
+    Scoped mem(ctx);
+    void *din, *dobs;
+    if ((st = mem.alloc(nrend * n * 16, &din)) || (st = mem.alloc(nch * nb * ns * n * 8, &dobs))) return st;
+    HIP_TRY(ctx, hipMemcpy(din, inputs, nrend * n * 16, hipMemcpyHostToDevice));
+    std::vector<oc_design_step> ds(designs, designs + nb * nv * nsteps);
+    std::map<const double*, const double*> up;
+    for (auto& d : ds) {
+        if (!d.has_element) continue;
+        if (oc_phasemask_validate(&d.mask)) return fail(ctx, OC_INV_RENDER, "invalid phase mask");
+        auto it = up.find(d.mask.values);
+        if (it == up.end()) {
+            void* mv;
+            const size_t mn = d.mask.nx * d.mask.ny;
+            if ((st = mem.alloc(mn * 8, &mv))) return st;
+            HIP_TRY(ctx, hipMemcpy(mv, d.mask.values, mn * 8, hipMemcpyHostToDevice));
+            it = up.emplace(d.mask.values, static_cast<const double*>(mv)).first;
+        }
+        d.mask.values = it->second;
+    }
+    st = render_impl(ctx, true, *g, bands, nb, ns, *rc, ds.data(), nsteps, *cfg, din, static_cast<double*>(dobs));
+    if (st) return st;
+    HIP_TRY(ctx, hipMemcpy(obs, dobs, nch * nb * ns * n * 8, hipMemcpyDeviceToHost));
+    return OC_OK;
+}
 
 int oc_adjoint_push_back(oc_context* ctx, const oc_design_step* steps, size_t nsteps, const oc_band_geom* g,
                          const double* input, const oc_asm_config* cfg, const double* dmdi, double* dphi_out) {

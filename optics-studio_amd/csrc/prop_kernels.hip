@@ -330,6 +330,42 @@ __global__ void k_block_reorder(const T* __restrict__ src, T* __restrict__ dst, 
 
 // out = (2 (|u|^2 - target)/n) u and sums[0] += (|u|^2 - target)^2
 // (merit.rs:123-165 with adjoint.rs:200-201)
+// Observation frames of Renderer::render (observe.rs:258-310): per frame
+// f = band * n_scenes + scene and pixel k, channel 0 = |U_base|^2 and each
+// enabled derivative (I+ - I-) / (2 step) from the frame's renders
+// [base, x+, x-, y+, y-, l+, l-] (enabled ones, in that order).  Flags a
+// non-finite base intensity (observe.rs:296-303).
+template <typename R>
+__global__ void k_render_obs(const cpx<R>* __restrict__ u, double* __restrict__ obs, size_t n, int frames, int nr,
+                             int cx, int cy, int cl, double hs, double hl, int* flag) {
+    const size_t total = n * (size_t)frames;
+    bool bad = false;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t f = i / n, k = i - f * n;
+        const cpx<R>* uf = u + f * (size_t)nr * n + k;
+        auto inten = [&](int r) {
+            const cpx<R> v = uf[(size_t)r * n];
+            return (double)v.re * (double)v.re + (double)v.im * (double)v.im;
+        };
+        const double i0 = inten(0);
+        bad |= !isfinite(i0);
+        obs[i] = i0;
+        int c = 1, r = 1;
+        if (cx) {
+            obs[((size_t)c * frames + f) * n + k] = (inten(r) - inten(r + 1)) / (2.0 * hs);
+            ++c;
+            r += 2;
+        }
+        if (cy) {
+            obs[((size_t)c * frames + f) * n + k] = (inten(r) - inten(r + 1)) / (2.0 * hs);
+            ++c;
+            r += 2;
+        }
+        if (cl) obs[((size_t)c * frames + f) * n + k] = (inten(r) - inten(r + 1)) / (2.0 * hl);
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
 template <typename R>
 __global__ void k_seed_mse(const cpx<R>* __restrict__ u, const double* __restrict__ target,
                            cpx<R>* __restrict__ out, size_t n, double nf, double* sums) {
@@ -509,6 +545,19 @@ hipError_t launch_seed_mse(bool dbl, const void* u, const double* target, void* 
     else
         hipLaunchKernelGGL(k_seed_mse<float>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<float>*>(u),
                            target, static_cast<cpx<float>*>(out), n, (double)n, sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_render_obs(bool dbl, const void* u, double* obs, size_t n, int frames, int nr, int cx, int cy,
+                             int cl, double hs, double hl, int* flag, hipStream_t st) {
+    int grid = (int)std::min<size_t>((n * (size_t)frames + 255) / 256, 16384);
+    if (grid < 1) grid = 1;
+    if (dbl)
+        hipLaunchKernelGGL(k_render_obs<double>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<double>*>(u), obs,
+                           n, frames, nr, cx, cy, cl, hs, hl, flag);
+    else
+        hipLaunchKernelGGL(k_render_obs<float>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<float>*>(u), obs,
+                           n, frames, nr, cx, cy, cl, hs, hl, flag);
     return hipGetLastError();
 }
 

@@ -185,6 +185,26 @@ class DesignStep(C.Structure):
                 ("medium", c_f64)]
 
 
+class RenderConfig(C.Structure):
+    """Renderer derivative channels and FD steps (observe.rs:50-58, 152-170)."""
+    _fields_ = [("deriv_x", c_i32), ("deriv_y", c_i32), ("deriv_lambda", c_i32), ("_pad0", c_i32),
+                ("scene_step", c_f64), ("lambda_step", c_f64)]
+
+    @staticmethod
+    def make(x=False, y=False, lam=False, scene_step=1e-6, lambda_step=1e-9):
+        """Renderer::new defaults: no derivatives, 1 um scene step, 1 nm band step."""
+        return RenderConfig(int(x), int(y), int(lam), 0, float(scene_step), float(lambda_step))
+
+    def renders(self):
+        return 1 + 2 * (self.deriv_x + self.deriv_y + self.deriv_lambda)
+
+    def variants(self):
+        return 3 if self.deriv_lambda else 1
+
+    def channels(self):
+        return 1 + self.deriv_x + self.deriv_y + self.deriv_lambda
+
+
 def geom(nx, ny, dx, dy, lam):
     return BandGeom(int(nx), int(ny), float(dx), float(dy), float(lam))
 

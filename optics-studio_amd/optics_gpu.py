@@ -94,6 +94,12 @@ EXPORTS = {
     "oc_adjoint_push_back": ([vp, P(T.DesignStep), c_sz, P(T.BandGeom), vp, P(T.AsmConfig), vp, vp], C.c_int),
     "oc_adjoint_grid_gradient_dev_c128": ([vp, P(T.DesignStep), c_sz, P(T.BandGeom), vp, P(T.AsmConfig), vp, vp,
                                            P(c_f64)], C.c_int),
+    "oc_render": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz, P(T.AsmConfig), vp,
+                   vp], C.c_int),
+    "oc_render_dev_c128": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz,
+                            P(T.AsmConfig), vp, vp], C.c_int),
+    "oc_render_dev_c64": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz,
+                           P(T.AsmConfig), vp, vp], C.c_int),
 }
 
 
@@ -585,3 +591,57 @@ def adjoint_grid_gradient_dev(dev_steps, u_dev, dx, dy, lam, asm_cfg, target_dev
                                                grads_dev.data_ptr() if grads_dev is not None else None,
                                                C.byref(merit))
     return st, merit.value
+
+
+def _design_table(designs):
+    """designs[band][variant] = [(mask struct or PhaseMask or None, z_after, medium), ...]
+    -> flat DesignStep array [band][variant][nsteps]."""
+    flat = [st for per_band in designs for steps in per_band for st in steps]
+    arr = (T.DesignStep * max(len(flat), 1))()
+    for i, (m, z, med) in enumerate(flat):
+        arr[i].has_element = 1 if m is not None else 0
+        if m is not None:
+            arr[i].mask = m.struct if hasattr(m, "struct") else m
+        arr[i].z_after = z
+        arr[i].medium = med
+    return arr
+
+
+def render(designs, inputs, dx, dy, bands, asm_cfg, rc, ctx=None):
+    """Renderer::render (observe.rs:213-314), host buffers.  designs[band][variant]:
+    step lists (variant 0 = the band's wavelength; with the lambda channel
+    1 = lambda + dl, 2 = lambda - dl); inputs: complex (bands, scenes, renders, ny, nx)
+    from the source closure, renders ordered base, x+, x-, y+, y-, l+, l- (enabled ones).
+    Returns (status, obs (channels, bands, scenes, ny, nx) float64)."""
+    ctx = ctx or context()
+    u = _c128(inputs)
+    nb, ns, nr, ny, nx = u.shape
+    if nr != rc.renders():
+        raise ValueError(f"inputs hold {nr} renders per frame, the config needs {rc.renders()}")
+    nsteps = len(designs[0][0]) if nb else 0
+    obs = np.zeros((rc.channels(), nb, ns, ny, nx))
+    b = np.ascontiguousarray(bands, np.float64)
+    st = lib.oc_render(ctx.h, C.byref(T.geom(nx, ny, dx, dy, 0.0)), b.ctypes.data, nb, ns, C.byref(rc),
+                       _design_table(designs), nsteps, C.byref(asm_cfg), u.ctypes.data, obs.ctypes.data)
+    return st, obs
+
+
+def render_dev(designs, inputs_dev, dx, dy, bands, asm_cfg, rc, obs_dev, ctx=None):
+    """Device form of render: inputs_dev complex128 or complex64 (bands, scenes,
+    renders, ny, nx), mask values on the device (dev_mask), obs_dev float64
+    (channels, bands, scenes, ny, nx).  Returns the status."""
+    torch = _torch()
+    ctx = ctx or context()
+    nb, ns, nr, ny, nx = inputs_dev.shape
+    if nr != rc.renders():
+        raise ValueError(f"inputs hold {nr} renders per frame, the config needs {rc.renders()}")
+    if tuple(obs_dev.shape) != (rc.channels(), nb, ns, ny, nx) or obs_dev.dtype != torch.float64:
+        raise ValueError("obs_dev must be float64 (channels, bands, scenes, ny, nx)")
+    if not (inputs_dev.is_contiguous() and obs_dev.is_contiguous()):
+        raise ValueError("inputs_dev and obs_dev must be contiguous")
+    fn = {torch.complex128: lib.oc_render_dev_c128, torch.complex64: lib.oc_render_dev_c64}[inputs_dev.dtype]
+    nsteps = len(designs[0][0]) if nb else 0
+    b = np.ascontiguousarray(bands, np.float64)
+    torch.cuda.synchronize()
+    return fn(ctx.h, C.byref(T.geom(nx, ny, dx, dy, 0.0)), b.ctypes.data, nb, ns, C.byref(rc),
+              _design_table(designs), nsteps, C.byref(asm_cfg), inputs_dev.data_ptr(), obs_dev.data_ptr())

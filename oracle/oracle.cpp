@@ -1267,6 +1267,56 @@ int or_trace(const or_design_step* steps, uint64_t nsteps, const oc_band_geom* g
     return OC_OK;
 }
 
+// observe.rs:213-314
+int or_render(const oc_band_geom* g, const double* bands, uint64_t nb, uint64_t ns, const oc_render_config* rc,
+              const or_design_step* designs, uint64_t nsteps, const oc_asm_config* asm_cfg, const double* inputs,
+              double* obs) {
+    if (nb == 0 || ns == 0) return OC_INV_RENDER;
+    if ((rc->deriv_x || rc->deriv_y) && (!std::isfinite(rc->scene_step) || rc->scene_step <= 0.0))
+        return OC_INV_RENDER;
+    if (rc->deriv_lambda && (!std::isfinite(rc->lambda_step) || rc->lambda_step <= 0.0)) return OC_INV_RENDER;
+    const size_t n = g->nx * g->ny;
+    const int nd = (rc->deriv_x ? 1 : 0) + (rc->deriv_y ? 1 : 0) + (rc->deriv_lambda ? 1 : 0);
+    const int nr = 1 + 2 * nd, nv = rc->deriv_lambda ? 3 : 1;
+    const size_t frames = nb * ns;
+    // one render_frame: the design at this wavelength traced from the source
+    std::vector<double> fld(2 * n);
+    auto frame = [&](size_t b, size_t s, int r, int v, double lambda, std::vector<double>& inten) -> int {
+        oc_band_geom gi = *g;
+        gi.lambda = lambda;
+        const double* in = inputs + ((b * ns + s) * nr + r) * 2 * n;
+        if (or_trace(designs + (b * nv + v) * nsteps, nsteps, &gi, in, asm_cfg, fld.data())) return OC_INV_RENDER;
+        inten.resize(n);
+        for (size_t k = 0; k < n; ++k) inten[k] = fld[2 * k] * fld[2 * k] + fld[2 * k + 1] * fld[2 * k + 1];
+        return OC_OK;
+    };
+    auto slot = [&](int c, size_t f) { return obs + ((size_t)c * frames + f) * n; };
+    std::vector<double> base, hi, lo;
+    for (size_t b = 0; b < nb; ++b) {
+        for (size_t s = 0; s < ns; ++s) {
+            const size_t f = b * ns + s;
+            if (frame(b, s, 0, 0, bands[b], base)) return OC_INV_RENDER;
+            for (double v : base)
+                if (!std::isfinite(v)) return OC_INV_RENDER;  // check_grid, observe.rs:296-303
+            std::copy(base.begin(), base.end(), slot(0, f));
+            int c = 1, r = 1;
+            auto fd = [&](int v_hi, int v_lo, double l_hi, double l_lo, double step) -> int {
+                if (frame(b, s, r, v_hi, l_hi, hi) || frame(b, s, r + 1, v_lo, l_lo, lo)) return OC_INV_RENDER;
+                double* d = slot(c, f);
+                for (size_t k = 0; k < n; ++k) d[k] = (hi[k] - lo[k]) / (2.0 * step);
+                ++c;
+                r += 2;
+                return OC_OK;
+            };
+            if (rc->deriv_x && fd(0, 0, bands[b], bands[b], rc->scene_step)) return OC_INV_RENDER;
+            if (rc->deriv_y && fd(0, 0, bands[b], bands[b], rc->scene_step)) return OC_INV_RENDER;
+            if (rc->deriv_lambda && fd(1, 2, bands[b] + rc->lambda_step, bands[b] - rc->lambda_step, rc->lambda_step))
+                return OC_INV_RENDER;
+        }
+    }
+    return OC_OK;
+}
+
 // adjoint.rs:140-246
 int or_adjoint_push_back(const or_design_step* steps, uint64_t nsteps, const oc_band_geom* g, const double* input,
                          const oc_asm_config* asm_cfg, const double* dmdi, double* dphi_out) {

@@ -159,6 +159,16 @@ int or_adjoint_push_back(const or_design_step* steps, uint64_t nsteps,
 int or_trace(const or_design_step* steps, uint64_t nsteps,
              const oc_band_geom* g, const double* input,
              const oc_asm_config* asm_cfg, double* out);
+/* Renderer::render (observe.rs:213-314): renders in the reference's order
+   (per band, per scene: base, x+/x-, y+/y-, lambda+/lambda-), each a trace
+   of the design built for its wavelength; inputs [band][scene][render],
+   designs [band][variant][nsteps], obs [channel][band][scene][ny][nx]
+   (the layout of oc_render).  Returns OC_INV_RENDER on any failure. */
+int or_render(const oc_band_geom* g, const double* bands, uint64_t nb,
+              uint64_t ns, const oc_render_config* rc,
+              const or_design_step* designs, uint64_t nsteps,
+              const oc_asm_config* asm_cfg, const double* inputs,
+              double* obs);
 
 #ifdef __cplusplus
 }

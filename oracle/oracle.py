@@ -81,6 +81,7 @@ def _declare(L):
     L.or_gs_fft.argtypes = [c_u64, vp, vp, vp, c_u64, c_u64, vp]
     L.or_adjoint_push_back.argtypes = [P(DesignStep), c_u64, G, vp, A, vp, vp]
     L.or_trace.argtypes = [P(DesignStep), c_u64, G, vp, A, vp]
+    L.or_render.argtypes = [G, vp, c_u64, c_u64, P(T.RenderConfig), P(DesignStep), c_u64, A, vp, vp]
 
 
 def _c128(a):
@@ -352,6 +353,28 @@ def trace(steps, u, dx, dy, lam, asm_cfg):
     st = lib().or_trace(_steps(steps), len(steps), C.byref(_geom(u, dx, dy, lam)), _ptr(u), C.byref(asm_cfg),
                         _ptr(out))
     return st, out
+
+
+def _design_table(designs, nsteps):
+    """designs[band][variant] = step list -> flat [band][variant][nsteps]."""
+    flat = [st for per_band in designs for steps in per_band for st in steps]
+    assert len(flat) == sum(len(v) for b in designs for v in b)
+    return _steps(flat) if flat else (DesignStep * 1)()
+
+
+def render(designs, inputs, dx, dy, bands, asm_cfg, rc):
+    """Renderer::render (observe.rs:213-314).  designs[band][variant] are
+    step lists (variant 0 = band wavelength; with the lambda channel 1 =
+    lambda + dl, 2 = lambda - dl); inputs: complex (bands, scenes, renders,
+    ny, nx).  Returns (status, obs (channels, bands, scenes, ny, nx))."""
+    u = _c128(inputs)
+    nb, ns, nr, ny, nx = u.shape
+    nsteps = len(designs[0][0]) if nb else 0
+    obs = np.zeros((rc.channels(), nb, ns, ny, nx))
+    b = np.ascontiguousarray(bands, np.float64)
+    st = lib().or_render(C.byref(T.geom(nx, ny, dx, dy, 0.0)), _ptr(b), nb, ns, C.byref(rc),
+                         _design_table(designs, nsteps), nsteps, C.byref(asm_cfg), _ptr(u), _ptr(obs))
+    return st, obs
 
 
 def adjoint_push_back(steps, u, dx, dy, lam, asm_cfg, dmdi):
