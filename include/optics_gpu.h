@@ -602,6 +602,54 @@ int oc_render_dev_c64(oc_context* ctx, const oc_band_geom* geom,
                       const oc_asm_config* cfg, const void* inputs,
                       double* obs);
 
+/* fd::gradient (crates/optics-inverse/src/fd.rs:139-184) for the objective
+   "trace one band and scene through a PhaseMask design with ASM and score
+   the intensity with MseToTarget" (observe.rs:195-210, merit.rs:124-137).
+   The caller keeps Params: it validates x, clamps the probe points into
+   [lo, hi] and builds their designs (Carrier::build).
+   designs: [dim][2][nsteps], slot i's up probe then its down probe.
+   up, dn, lo, hi: host arrays of dim probe values and bounds; a slot with
+   lo == hi is skipped (gradient 0, designs unread, no evaluation).
+   All probes run as one batch per beamline step from the one input field
+   (device); each probe's merit is reduced on the device in a fixed order.
+   g_i = (M(up) - M(dn)) / (up_i - dn_i).  grad_out[dim] (host);
+   merits_out[2*dim] (host, may be NULL; 0 for skipped slots);
+   evaluations_out (host, may be NULL) = 2 x live slots.
+   Errors: OC_INV_RENDER for a failed render, a non-finite merit or
+   gradient, or a live slot with up_i - dn_i <= 0. */
+int oc_fd_gradient_dev_c128(oc_context* ctx, const oc_band_geom* geom,
+                            const oc_design_step* designs, size_t nsteps,
+                            size_t dim, const double* up, const double* dn,
+                            const double* lo, const double* hi,
+                            const oc_asm_config* cfg, const void* input,
+                            const double* target, double* grad_out,
+                            double* merits_out, size_t* evaluations_out);
+int oc_fd_gradient_dev_c64(oc_context* ctx, const oc_band_geom* geom,
+                           const oc_design_step* designs, size_t nsteps,
+                           size_t dim, const double* up, const double* dn,
+                           const double* lo, const double* hi,
+                           const oc_asm_config* cfg, const void* input,
+                           const double* target, double* grad_out,
+                           double* merits_out, size_t* evaluations_out);
+
+/* AdamConfig (fd.rs:62-118); has_clip = 0 is clip_norm = None. */
+typedef struct oc_adam_config {
+    double lr_frac, beta1, beta2, eps;
+    int32_t has_clip, _pad0;
+    double clip_norm;
+} oc_adam_config;
+
+/* One Adam epoch of fd::adam (fd.rs:223-244) on the device: gradient-norm
+   clip, moment updates with bias corrections 1 - beta^t, the step
+   lr_frac (hi - lo) m_hat / (sqrt(v_hat) + eps), then the clamp into
+   [lo, hi] (solve.rs:74-78).  x, m, v updated in place; g, lo, hi read;
+   all device f64 arrays of dim.  t >= 1 is the epoch.  The norm is
+   reduced in a fixed order.  Errors: OC_INV_RENDER for an invalid config
+   (fd.rs:95-118), OC_ERR_INVALID_ARGUMENT for t = 0. */
+int oc_adam_step_dev(oc_context* ctx, size_t dim, double* x, const double* g,
+                     double* m, double* v, const double* lo, const double* hi,
+                     uint64_t t, const oc_adam_config* cfg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -366,6 +366,88 @@ __global__ void k_render_obs(const cpx<R>* __restrict__ u, double* __restrict__ 
     if (bad) atomicOr(flag, 1);
 }
 
+// Per-render MseToTarget sums (merit.rs:124-137) in a fixed order:
+// workgroup (chunk c, render r) reduces its strided share of render r's
+// samples through LDS into part[r][c]; k_mse_final adds the chunks in order
+// and divides by n.  The result does not depend on the launch.
+constexpr int kRedChunks = 64;
+template <typename R>
+__global__ __launch_bounds__(256) void k_mse_part(const cpx<R>* __restrict__ u, const double* __restrict__ t,
+                                                  size_t n, int r0, double* __restrict__ part) {
+    const int r = r0 + (int)blockIdx.y, c = (int)blockIdx.x;
+    const cpx<R>* ur = u + (size_t)r * n;
+    double acc = 0.0;
+    for (size_t k = (size_t)c * 256 + threadIdx.x; k < n; k += (size_t)kRedChunks * 256) {
+        const double re = ur[k].re, im = ur[k].im;
+        const double d = (re * re + im * im) - t[k];
+        acc += d * d;
+    }
+    __shared__ double sm[256];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)r * kRedChunks + c] = sm[0];
+}
+
+__global__ void k_mse_final(const double* __restrict__ part, int nr, double nf, double* __restrict__ merit) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < nr; r += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int c = 0; c < kRedChunks; ++c) s += part[(size_t)r * kRedChunks + c];
+        merit[r] = s / nf;
+    }
+}
+
+// Adam (fd.rs:223-244).  k_sq_part: fixed-order partial sums of g^2;
+// k_clip_scale: norm = sqrt(sum), scale = cn / norm when clipping applies
+// (else 1), written to scale[0]; k_adam: the per-coordinate update and the
+// clamp into [lo, hi] (solve.rs:74-78).
+__global__ __launch_bounds__(256) void k_sq_part(const double* __restrict__ g, size_t n, double* __restrict__ part) {
+    const int c = (int)blockIdx.x;
+    double acc = 0.0;
+    for (size_t k = (size_t)c * 256 + threadIdx.x; k < n; k += (size_t)kRedChunks * 256) acc += g[k] * g[k];
+    __shared__ double sm[256];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[c] = sm[0];
+}
+
+__global__ void k_clip_scale(const double* __restrict__ part, int has_clip, double cn, double* __restrict__ scale) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int c = 0; c < kRedChunks; ++c) s += part[c];
+    const double norm = sqrt(s);
+    scale[0] = (has_clip && norm > cn && norm > 0.0) ? cn / norm : 1.0;
+}
+
+__global__ void k_adam(double* __restrict__ x, const double* __restrict__ g, double* __restrict__ m,
+                       double* __restrict__ v, const double* __restrict__ lo, const double* __restrict__ hi, size_t n,
+                       const double* __restrict__ scale, double lr, double b1, double b2, double eps, double bc1,
+                       double bc2) {
+    const double sc = scale[0];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double gi = g[i] * sc;
+        const double mi = b1 * m[i] + (1.0 - b1) * gi;
+        const double vi = b2 * v[i] + (1.0 - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const double mh = mi / bc1;
+        const double vh = vi / bc2;
+        const double denom = sqrt(vh) + eps;
+        const double l = lo[i], h = hi[i];
+        double xi = x[i];
+        if (denom > 0.0) xi -= lr * (h - l) * mh / denom;
+        xi = xi < l ? l : (xi > h ? h : xi);  // f64::clamp: NaN stays NaN
+        x[i] = xi;
+    }
+}
+
 template <typename R>
 __global__ void k_seed_mse(const cpx<R>* __restrict__ u, const double* __restrict__ target,
                            cpx<R>* __restrict__ out, size_t n, double nf, double* sums) {
@@ -558,6 +640,33 @@ hipError_t launch_render_obs(bool dbl, const void* u, double* obs, size_t n, int
     else
         hipLaunchKernelGGL(k_render_obs<float>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<float>*>(u), obs,
                            n, frames, nr, cx, cy, cl, hs, hl, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_mse(bool dbl, const void* u, const double* target, size_t n, int nr, double* part, double* merit,
+                      hipStream_t st) {
+    for (int r0 = 0; r0 < nr; r0 += 65535) {
+        const dim3 grid(kRedChunks, (unsigned)std::min(65535, nr - r0));
+        if (dbl)
+            hipLaunchKernelGGL(k_mse_part<double>, grid, dim3(256), 0, st, static_cast<const cpx<double>*>(u), target,
+                               n, r0, part);
+        else
+            hipLaunchKernelGGL(k_mse_part<float>, grid, dim3(256), 0, st, static_cast<const cpx<float>*>(u), target,
+                               n, r0, part);
+    }
+    hipLaunchKernelGGL(k_mse_final, dim3((nr + 255) / 256), dim3(256), 0, st, part, nr, (double)n, merit);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(double* x, const double* g, double* m, double* v, const double* lo, const double* hi, size_t n,
+                       double* part, double* scale, int has_clip, double cn, double lr, double b1, double b2,
+                       double eps, double bc1, double bc2, hipStream_t st) {
+    hipLaunchKernelGGL(k_sq_part, dim3(kRedChunks), dim3(256), 0, st, g, n, part);
+    hipLaunchKernelGGL(k_clip_scale, dim3(1), dim3(64), 0, st, part, has_clip, cn, scale);
+    int grid = (int)std::min<size_t>((n + 255) / 256, 16384);
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_adam, dim3(grid), dim3(256), 0, st, x, g, m, v, lo, hi, n, scale, lr, b1, b2, eps, bc1,
+                       bc2);
     return hipGetLastError();
 }
 

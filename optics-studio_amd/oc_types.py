@@ -205,6 +205,19 @@ class RenderConfig(C.Structure):
         return 1 + self.deriv_x + self.deriv_y + self.deriv_lambda
 
 
+class AdamConfig(C.Structure):
+    """AdamConfig (crates/optics-inverse/src/fd.rs:62-93); has_clip = 0 is clip_norm = None."""
+    _fields_ = [("lr_frac", c_f64), ("beta1", c_f64), ("beta2", c_f64), ("eps", c_f64), ("has_clip", c_i32),
+                ("_pad0", c_i32), ("clip_norm", c_f64)]
+
+    @staticmethod
+    def default(**kw):
+        c = AdamConfig(0.02, 0.9, 0.999, 1.0e-8, 1, 0, 1.0)
+        for k, v in kw.items():
+            setattr(c, k, v)
+        return c
+
+
 def geom(nx, ny, dx, dy, lam):
     return BandGeom(int(nx), int(ny), float(dx), float(dy), float(lam))
 

@@ -94,6 +94,11 @@ EXPORTS = {
     "oc_adjoint_push_back": ([vp, P(T.DesignStep), c_sz, P(T.BandGeom), vp, P(T.AsmConfig), vp, vp], C.c_int),
     "oc_adjoint_grid_gradient_dev_c128": ([vp, P(T.DesignStep), c_sz, P(T.BandGeom), vp, P(T.AsmConfig), vp, vp,
                                            P(c_f64)], C.c_int),
+    "oc_fd_gradient_dev_c128": ([vp, P(T.BandGeom), P(T.DesignStep), c_sz, c_sz, vp, vp, vp, vp, P(T.AsmConfig), vp,
+                                 vp, vp, vp, P(c_sz)], C.c_int),
+    "oc_fd_gradient_dev_c64": ([vp, P(T.BandGeom), P(T.DesignStep), c_sz, c_sz, vp, vp, vp, vp, P(T.AsmConfig), vp,
+                                vp, vp, vp, P(c_sz)], C.c_int),
+    "oc_adam_step_dev": ([vp, c_sz, vp, vp, vp, vp, vp, vp, C.c_uint64, P(T.AdamConfig)], C.c_int),
     "oc_render": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz, P(T.AsmConfig), vp,
                    vp], C.c_int),
     "oc_render_dev_c128": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz,
@@ -645,3 +650,42 @@ def render_dev(designs, inputs_dev, dx, dy, bands, asm_cfg, rc, obs_dev, ctx=Non
     torch.cuda.synchronize()
     return fn(ctx.h, C.byref(T.geom(nx, ny, dx, dy, 0.0)), b.ctypes.data, nb, ns, C.byref(rc),
               _design_table(designs), nsteps, C.byref(asm_cfg), inputs_dev.data_ptr(), obs_dev.data_ptr())
+
+
+def fd_gradient_dev(designs, u_dev, dx, dy, lam, asm_cfg, target_dev, up, dn, lo, hi, ctx=None):
+    """fd::gradient (fd.rs:139-184) for the objective trace + MseToTarget on the
+    device.  designs[i] = (up design, down design) step lists of slot i (mask
+    structs with device lattices); u_dev complex128/complex64 (ny, nx);
+    target_dev float64 (ny, nx); up, dn, lo, hi: per-slot probe values and
+    bounds.  Returns (status, grad, merits (dim, 2), evaluations)."""
+    torch = _torch()
+    ctx = ctx or context()
+    dim = len(up)
+    ny, nx = u_dev.shape
+    if tuple(target_dev.shape) != (ny, nx) or target_dev.dtype != torch.float64 or not target_dev.is_contiguous():
+        raise ValueError("target_dev must be contiguous float64 of the field's shape")
+    if not u_dev.is_contiguous():
+        raise ValueError("u_dev must be contiguous")
+    fn = {torch.complex128: lib.oc_fd_gradient_dev_c128, torch.complex64: lib.oc_fd_gradient_dev_c64}[u_dev.dtype]
+    nsteps = len(designs[0][0]) if dim else 0
+    table = _design_table([list(pair) for pair in designs])
+    a = [np.ascontiguousarray(v, np.float64) for v in (up, dn, lo, hi)]
+    g = np.zeros(dim)
+    mer = np.zeros((dim, 2))
+    ev = c_sz(0)
+    torch.cuda.synchronize()
+    st = fn(ctx.h, C.byref(T.geom(nx, ny, dx, dy, lam)), table, nsteps, dim, *[v.ctypes.data for v in a],
+            C.byref(asm_cfg), u_dev.data_ptr(), target_dev.data_ptr(), g.ctypes.data, mer.ctypes.data, C.byref(ev))
+    return st, g, mer, ev.value
+
+
+def adam_step_dev(x_dev, g_dev, m_dev, v_dev, lo_dev, hi_dev, t, cfg, ctx=None):
+    """One Adam epoch (fd.rs:210-244) on contiguous float64 device vectors."""
+    torch = _torch()
+    ctx = ctx or context()
+    ts = (x_dev, g_dev, m_dev, v_dev, lo_dev, hi_dev)
+    n = x_dev.numel()
+    if any(t_.dtype != torch.float64 or not t_.is_contiguous() or t_.numel() != n for t_ in ts):
+        raise ValueError("x, g, m, v, lo, hi must be contiguous float64 of one length")
+    torch.cuda.synchronize()
+    return lib.oc_adam_step_dev(ctx.h, n, *[t_.data_ptr() for t_ in ts], int(t), C.byref(cfg))

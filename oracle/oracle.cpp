@@ -1267,6 +1267,75 @@ int or_trace(const or_design_step* steps, uint64_t nsteps, const oc_band_geom* g
     return OC_OK;
 }
 
+// fd.rs:139-184; the objective: observe.rs:195-210 then merit.rs:124-137
+int or_fd_gradient(const oc_band_geom* g, const or_design_step* designs, uint64_t nsteps, uint64_t dim,
+                   const double* up, const double* dn, const double* lo, const double* hi,
+                   const oc_asm_config* asm_cfg, const double* input, const double* target, double* grad,
+                   double* merits, uint64_t* evaluations) {
+    const size_t n = g->nx * g->ny;
+    std::vector<double> fld(2 * n);
+    auto objective = [&](const or_design_step* steps, double* out) -> int {
+        if (or_trace(steps, nsteps, g, input, asm_cfg, fld.data())) return OC_INV_RENDER;
+        double s = 0.0;
+        for (size_t k = 0; k < n; ++k) {
+            const double a = fld[2 * k] * fld[2 * k] + fld[2 * k + 1] * fld[2 * k + 1];
+            s += (a - target[k]) * (a - target[k]);
+        }
+        *out = s / (double)n;
+        return OC_OK;
+    };
+    uint64_t ev = 0;
+    for (uint64_t i = 0; i < dim; ++i) {
+        grad[i] = 0.0;
+        if (merits) merits[2 * i] = merits[2 * i + 1] = 0.0;
+    }
+    for (uint64_t i = 0; i < dim; ++i) {
+        if (lo[i] == hi[i]) continue;
+        double fu, fd;
+        if (objective(designs + (2 * i) * nsteps, &fu) || objective(designs + (2 * i + 1) * nsteps, &fd))
+            return OC_INV_RENDER;
+        ev += 2;
+        if (merits) {
+            merits[2 * i] = fu;
+            merits[2 * i + 1] = fd;
+        }
+        const double d = up[i] - dn[i];
+        if (d <= 0.0 || std::isnan(d)) return OC_INV_RENDER;
+        const double gi = (fu - fd) / d;
+        if (!std::isfinite(gi)) return OC_INV_RENDER;
+        grad[i] = gi;
+    }
+    if (evaluations) *evaluations = ev;
+    return OC_OK;
+}
+
+// fd.rs:210-244 (one epoch, gradient given), solve.rs:74-78
+int or_adam_step(uint64_t dim, double* x, const double* g, double* m, double* v, const double* lo, const double* hi,
+                 uint64_t t, const oc_adam_config* cfg) {
+    double norm = 0.0;
+    for (uint64_t i = 0; i < dim; ++i) norm += g[i] * g[i];
+    norm = std::sqrt(norm);
+    const double scale = (cfg->has_clip && norm > cfg->clip_norm && norm > 0.0) ? cfg->clip_norm / norm : 1.0;
+    const double bc1 = 1.0 - std::pow(cfg->beta1, (double)t);
+    const double bc2 = 1.0 - std::pow(cfg->beta2, (double)t);
+    for (uint64_t i = 0; i < dim; ++i) {
+        const double gi = g[i] * scale;
+        m[i] = cfg->beta1 * m[i] + (1.0 - cfg->beta1) * gi;
+        v[i] = cfg->beta2 * v[i] + (1.0 - cfg->beta2) * gi * gi;
+        const double mh = m[i] / bc1;
+        const double vh = v[i] / bc2;
+        const double denom = std::sqrt(vh) + cfg->eps;
+        if (denom > 0.0) x[i] -= cfg->lr_frac * (hi[i] - lo[i]) * mh / denom;
+    }
+    for (uint64_t i = 0; i < dim; ++i) {
+        double xi = x[i];
+        if (xi < lo[i]) xi = lo[i];
+        if (xi > hi[i]) xi = hi[i];
+        x[i] = xi;
+    }
+    return OC_OK;
+}
+
 // observe.rs:213-314
 int or_render(const oc_band_geom* g, const double* bands, uint64_t nb, uint64_t ns, const oc_render_config* rc,
               const or_design_step* designs, uint64_t nsteps, const oc_asm_config* asm_cfg, const double* inputs,

@@ -164,6 +164,20 @@ int or_trace(const or_design_step* steps, uint64_t nsteps,
    of the design built for its wavelength; inputs [band][scene][render],
    designs [band][variant][nsteps], obs [channel][band][scene][ny][nx]
    (the layout of oc_render).  Returns OC_INV_RENDER on any failure. */
+/* fd::gradient (fd.rs:139-184) with the objective "trace the design,
+   MseToTarget on the intensity" (merit.rs:124-137); designs [dim][2][nsteps],
+   the oc_fd_gradient_dev layout.  merits (2*dim) may be NULL. */
+int or_fd_gradient(const oc_band_geom* g, const or_design_step* designs,
+                   uint64_t nsteps, uint64_t dim, const double* up,
+                   const double* dn, const double* lo, const double* hi,
+                   const oc_asm_config* asm_cfg, const double* input,
+                   const double* target, double* grad, double* merits,
+                   uint64_t* evaluations);
+/* One epoch of the Adam update (fd.rs:210-244) given the gradient g, then
+   the clamp (solve.rs:74-78); x, m, v updated in place. */
+int or_adam_step(uint64_t dim, double* x, const double* g, double* m,
+                 double* v, const double* lo, const double* hi, uint64_t t,
+                 const oc_adam_config* cfg);
 int or_render(const oc_band_geom* g, const double* bands, uint64_t nb,
               uint64_t ns, const oc_render_config* rc,
               const or_design_step* designs, uint64_t nsteps,

@@ -81,6 +81,8 @@ def _declare(L):
     L.or_gs_fft.argtypes = [c_u64, vp, vp, vp, c_u64, c_u64, vp]
     L.or_adjoint_push_back.argtypes = [P(DesignStep), c_u64, G, vp, A, vp, vp]
     L.or_trace.argtypes = [P(DesignStep), c_u64, G, vp, A, vp]
+    L.or_fd_gradient.argtypes = [G, P(DesignStep), c_u64, c_u64, vp, vp, vp, vp, A, vp, vp, vp, vp, vp]
+    L.or_adam_step.argtypes = [c_u64, vp, vp, vp, vp, vp, vp, c_u64, P(T.AdamConfig)]
     L.or_render.argtypes = [G, vp, c_u64, c_u64, P(T.RenderConfig), P(DesignStep), c_u64, A, vp, vp]
 
 
@@ -360,6 +362,34 @@ def _design_table(designs, nsteps):
     flat = [st for per_band in designs for steps in per_band for st in steps]
     assert len(flat) == sum(len(v) for b in designs for v in b)
     return _steps(flat) if flat else (DesignStep * 1)()
+
+
+def fd_gradient(designs, u, dx, dy, lam, asm_cfg, target, up, dn, lo, hi):
+    """fd::gradient (fd.rs:139-184) with the trace + MseToTarget objective.
+    designs[i] = (up design, down design) step lists per slot.
+    Returns (status, grad, merits (dim, 2), evaluations)."""
+    u = _c128(u)
+    dim = len(up)
+    nsteps = len(designs[0][0]) if dim else 0
+    flat = [st for pair in designs for steps in pair for st in steps]
+    arr = _steps(flat) if flat else (DesignStep * 1)()
+    g = np.zeros(dim)
+    mer = np.zeros((dim, 2))
+    ev = C.c_uint64(0)
+    t = np.ascontiguousarray(target, np.float64)
+    a = [np.ascontiguousarray(v, np.float64) for v in (up, dn, lo, hi)]
+    st = lib().or_fd_gradient(C.byref(_geom(u, dx, dy, lam)), arr, nsteps, dim, *[_ptr(v) for v in a],
+                              C.byref(asm_cfg), _ptr(u), _ptr(t), _ptr(g), _ptr(mer), C.byref(ev))
+    return st, g, mer, ev.value
+
+
+def adam_step(x, g, m, v, lo, hi, t, cfg):
+    """One Adam epoch (fd.rs:210-244) in place on float64 arrays."""
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (g, lo, hi)]
+    for a in (x, m, v):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    return lib().or_adam_step(len(x), _ptr(x), _ptr(arrs[0]), _ptr(m), _ptr(v), _ptr(arrs[1]), _ptr(arrs[2]), t,
+                              C.byref(cfg))
 
 
 def render(designs, inputs, dx, dy, bands, asm_cfg, rc):
