@@ -195,6 +195,118 @@ def bench_rs(G, W, T, torch, np):
     return r
 
 
+def render_design(G, W, n, d, bands):
+    """Two DOEs (height profiles: phase scales as 550 nm / lambda) and two
+    ASM gaps, built for each band: designs[band][0] = step list with
+    device-resident lattices (the keep list holds them)."""
+    import numpy as np
+    h1 = 2 * np.pi * W.splitmix_uniform(101, n * n).reshape(n, n)
+    h2 = 2 * np.pi * W.splitmix_uniform(102, n * n).reshape(n, n)
+    keep, designs = [], []
+    for lam in bands:
+        m1, k1 = G.dev_mask(h1 * 550e-9 / lam, d, d)
+        m2, k2 = G.dev_mask(h2 * 550e-9 / lam, d, d)
+        keep += [k1, k2]
+        designs.append([[(m1, 4e-3, 1.0), (m2, 6e-3, 1.0)]])
+    return designs, keep, (h1, h2)
+
+
+def bench_render(G, W, T, torch, np, cpu=True):
+    """Renderer::render: 8 bands (450-650 nm) x 4 scenes with the x and y FD
+    channels (5 renders per frame, 160 renders), 512^2 at 4 um, two DOEs
+    and two ASM gaps (P = 1024), complex64, one call."""
+    n, d = 512, 4e-6
+    bands = list(np.linspace(450e-9, 650e-9, 8))
+    scenes = [(0.0, 0.0), (40e-6, 0.0), (0.0, -40e-6), (-60e-6, 60e-6)]
+    rc = T.RenderConfig.make(x=True, y=True, scene_step=1e-6)
+    designs, keep, hs = render_design(G, W, n, d, bands)
+    src = lambda x0, y0, lam: W.gaussian_beam(n, d, lam, 400e-6, 1.0, (x0, y0))
+    ins = []
+    for lam in bands:
+        for (x0, y0) in scenes:
+            h = rc.scene_step
+            ins.append(np.stack([src(x0, y0, lam), src(x0 + h, y0, lam), src(x0 - h, y0, lam),
+                                 src(x0, y0 + h, lam), src(x0, y0 - h, lam)]))
+    u = torch.as_tensor(np.stack(ins).reshape(len(bands), len(scenes), 5, n, n).astype(np.complex64)).cuda()
+    obs = torch.zeros((3, len(bands), len(scenes), n, n), dtype=torch.float64, device="cuda")
+    cfg = T.AsmConfig.default(strict=0)
+    st = G.render_dev(designs, u, d, d, bands, cfg, rc, obs)
+    if st:
+        raise RuntimeError(f"render failed: {st}")
+    k = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        G.render_dev(designs, u, d, d, bands, cfg, rc, obs)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    nr = len(bands) * len(scenes) * 5
+    r = {"workload": "Renderer::render 8 bands x 4 scenes, x and y FD channels (160 renders), 512^2 at 4 um, "
+                     "2 DOEs + 2 ASM gaps (P = 1024), complex64, one call",
+         "renders_per_s": nr * k / (t1 - t0), "frames_per_s": len(bands) * len(scenes) * k / (t1 - t0),
+         "ms_per_call": 1e3 * (t1 - t0) / k}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as orc
+        m1 = orc.Mask(hs[0] * 550e-9 / bands[0], d, d)
+        m2 = orc.Mask(hs[1] * 550e-9 / bands[0], d, d)
+        steps = [(m1, 4e-3, 1.0), (m2, 6e-3, 1.0)]
+        one = np.ascontiguousarray(u[0:1, 0:1].cpu().numpy().astype(np.complex128))
+        c0 = time.perf_counter()
+        st, _ = orc.render([[steps]], one, d, d, bands[:1], cfg, rc)
+        c1 = time.perf_counter()
+        r["cpu_baseline"] = {"renders_per_s": 5 / (c1 - c0), "cores": 1, "kind": "port",
+                             "sample": f"one frame (5 renders) with the f64 oracle, single thread, {c1 - c0:.1f} s"}
+    log(f"render: {r['renders_per_s']:.0f} renders/s ({r['ms_per_call']:.2f} ms per 160-render call)")
+    return r
+
+
+def bench_fd(G, W, T, torch, np):
+    """fd::gradient with the render + MseToTarget objective: 64 pixel-phase
+    slots of a two-DOE design (128 probe renders), 256^2 at 4 um, P = 512,
+    complex128, one call."""
+    n, d, lam = 256, 4e-6, 633e-9
+    p1 = 2 * np.pi * W.splitmix_uniform(111, n * n).reshape(n, n)
+    p2 = 2 * np.pi * W.splitmix_uniform(112, n * n).reshape(n, n)
+    u = torch.as_tensor(W.gaussian_beam(n, d, lam, 300e-6)).cuda()
+    tgt = torch.zeros((n, n), dtype=torch.float64, device="cuda")
+    tgt[n // 2, n // 2] = 1.0
+    rng = np.random.default_rng(7)
+    pix = rng.choice(n * n, 64, replace=False)
+    h = 2 * np.pi * 1e-4
+    o1, k1 = G.dev_mask(p1, d, d)
+    o2, k2 = G.dev_mask(p2, d, d)
+    keep, designs, up, dn = [k1, k2], [], [], []
+    for k_, q in enumerate(pix):
+        e = k_ % 2  # probes alternate between the two DOEs
+        pair = []
+        for sgn in (1.0, -1.0):
+            lat = [p1, p2][e].copy().reshape(-1)
+            lat[q] = min(max(lat[q] + sgn * h, 0.0), 2 * np.pi)
+            (up if sgn > 0 else dn).append(lat[q])
+            m, t_ = G.dev_mask(lat.reshape(n, n), d, d)
+            keep.append(t_)
+            pair.append([(m, 2e-3, 1.0), (o2, 2e-3, 1.0)] if e == 0 else [(o1, 2e-3, 1.0), (m, 2e-3, 1.0)])
+        designs.append(tuple(pair))
+    lo, hi = [0.0] * 64, [2 * np.pi] * 64
+    cfg = T.AsmConfig.default(strict=0)
+    st, g, _, ev = G.fd_gradient_dev(designs, u, d, d, lam, cfg, tgt, up, dn, lo, hi)
+    if st:
+        raise RuntimeError(f"fd failed: {st}")
+    k = 5
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(k):
+        G.fd_gradient_dev(designs, u, d, d, lam, cfg, tgt, up, dn, lo, hi)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    r = {"workload": "fd::gradient, render + MseToTarget objective, 64 pixel slots (128 probe renders) of a "
+                     "2-DOE + 2-gap design, 256^2 at 4 um (P = 512), complex128, one call",
+         "probe_renders_per_s": ev * k / (t1 - t0), "ms_per_gradient": 1e3 * (t1 - t0) / k}
+    log(f"fd: {r['probe_renders_per_s']:.0f} probe renders/s ({r['ms_per_gradient']:.2f} ms per gradient)")
+    return r
+
+
 def read_traffic(variant):
     """HBM bytes per launch of the column pass for the layout variant in use,
     from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE,
@@ -364,6 +476,8 @@ def main():
         extra["gs"] = bench_gs(G, W, torch, np)
         extra["adjoint"] = bench_adjoint(G, W, T, torch, np, cpu=not args.no_cpu)
         extra["rs_batch"] = bench_rs(G, W, T, torch, np)
+        extra["render"] = bench_render(G, W, T, torch, np, cpu=not args.no_cpu)
+        extra["fd"] = bench_fd(G, W, T, torch, np)
 
     out = {
         "metric": METRIC,
