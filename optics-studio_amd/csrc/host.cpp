@@ -317,13 +317,13 @@ struct Prob {
     OpsDev ops{};
 };
 
+// Cache key: the tag and the exact bit patterns of the values (no text
+// formatting: these keys are built for every problem of every call).
 std::string fmtkey(const char* tag, std::initializer_list<double> vals) {
     std::string s(tag);
-    char b[40];
-    for (double v : vals) {
-        std::snprintf(b, sizeof b, "|%a", v);
-        s += b;
-    }
+    s.reserve(s.size() + 1 + 8 * vals.size());
+    s += '|';
+    for (double v : vals) s.append(reinterpret_cast<const char*>(&v), sizeof v);
     return s;
 }
 
