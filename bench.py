@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--bands", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the f64 and host drop-in legs")
+    ap.add_argument("--only", choices=["gs", "adjoint", "rs", "render", "fd"],
+                    help="run only that secondary workload and print its record (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB (8 launches) for PMC counter calibration")
     return ap.parse_args()
@@ -327,8 +329,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by band/problem, "
-              "see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the workloads split into "
+              "independent bands/problems: one process per GPU would run this on its share).", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch
@@ -336,6 +338,15 @@ def main():
     import oc_types as T
     import optics_gpu as G
     import workloads as W
+
+    if args.only:
+        runners = {"gs": lambda: bench_gs(G, W, torch, np),
+                   "adjoint": lambda: bench_adjoint(G, W, T, torch, np, cpu=not args.no_cpu),
+                   "rs": lambda: bench_rs(G, W, T, torch, np),
+                   "render": lambda: bench_render(G, W, T, torch, np, cpu=not args.no_cpu),
+                   "fd": lambda: bench_fd(G, W, T, torch, np)}
+        print(json.dumps({"only": args.only, "record": runners[args.only]()}))
+        return
 
     c = W.CFG2
     n, d = c["n"], c["d"]

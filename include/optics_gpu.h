@@ -246,7 +246,9 @@ int oc_context_pass_timing(oc_context* ctx, double* ms, uint64_t* launches,
    of every variant at length n on a synthetic batch and keeps the fastest for
    this context; ms_out (optional, OC_FFT_VARIANTS entries) receives the
    per-variant times (negative for variants that do not exist).  oc_context_set_fft_variant forces a choice;
-   oc_context_fft_variant reports the current one (-1: no specialised kernel).
+   oc_context_fft_variant reports the current one (-1: no specialised kernel);
+   untuned lengths use the layout measured fastest on MI355X for that length
+   and precision.
    oc_context_force_generic(ctx, 1) routes every batch through the
    runtime-plan kernels instead. */
 #define OC_FFT_VARIANTS 4

@@ -54,7 +54,8 @@ struct FastLayout {
     static_assert(G >= 1 && B >= 1 && CB >= 1 && (T % B) == 0 && (B % CB) == 0, "layout");
 };
 
-// PK: pre-op (PRE_NONE, PRE_MASK, PRE_TMUL, PRE_POLAR, or -1 for the descriptor's).
+// PK: pre-op (PRE_NONE, PRE_MASK, PRE_TMUL, PRE_POLAR, PRE_MASKS for a mix of the
+// first three, or -1 for the descriptor's).
 template <typename R, int N, int V, int PK>
 __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd(const ProbDev* __restrict__ probs,
                                                                          int nprob, int total) {
@@ -81,7 +82,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
         bool bad = false;
         double acc = 0.0;
         cpx<R> v[16];
-        if constexpr (PK == PRE_NONE || PK == PRE_MASK || PK == PRE_TMUL) {
+        if constexpr (PK == PRE_NONE || PK == PRE_MASK || PK == PRE_TMUL || PK == PRE_MASKS) {
             // the row's 16 samples as one batch of loads, then the checks,
             // then the mask with its mode decided once for the row
             const cpx<R>* in = static_cast<const cpx<R>*>(P.in);
@@ -94,7 +95,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
             }
 #pragma unroll
             for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
-            if constexpr (PK == PRE_TMUL) {
+            if (PK == PRE_TMUL || (PK == PRE_MASKS && P.ops.pre == PRE_TMUL)) {
                 const cpx<R>* tr = static_cast<const cpx<R>*>(P.ops.trans);
 #pragma unroll
                 for (int m = 0; m < 16; ++m) {
@@ -106,7 +107,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
                     }
                 }
             }
-            if constexpr (PK == PRE_MASK) {
+            if (PK == PRE_MASK || (PK == PRE_MASKS && P.ops.pre == PRE_MASK)) {
                 const MaskDev& mk = P.mask;
                 if (mk.mode == MASK_SEPARABLE) {
                     MaskAxis Y;
@@ -500,7 +501,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
 // cropped N_y x N_x field, blocked [ceil(N_x/B)][N_y][B].
 
 // Pass A: CB input columns per tile, read from the natural-layout input
-// (pre-op PK: none or phase mask), written to the blocked intermediate.
+// (pre-op PK: none, phase mask, shared transmission or PRE_MASKS for a mix),
+// written to the blocked intermediate.
 template <typename R, int N, int V, int PK>
 __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_col(const ProbDev* __restrict__ probs,
                                                                           int nprob, int total) {
@@ -527,14 +529,14 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
 #pragma unroll
         for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
         if (bad) atomicOr(P.flag, 1);
-        if constexpr (PK == PRE_MASK) {
+        if (PK == PRE_MASK || (PK == PRE_MASKS && P.ops.pre == PRE_MASK)) {
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
                 const int y = t + m * T;
                 if (colok && y >= oy && y < oy + ny) v[m] = apply_mask(P.mask, kx, y - oy, v[m]);
             }
         }
-        if constexpr (PK == PRE_TMUL) {
+        if (PK == PRE_TMUL || (PK == PRE_MASKS && P.ops.pre == PRE_TMUL)) {
             const cpx<R>* tr = static_cast<const cpx<R>*>(P.ops.trans);
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
@@ -652,8 +654,12 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                     hipLaunchKernelGGL((f_sep_col<R, N, V, PRE_MASK>), g, bcol, col_lds, st, d, np, total);
                 else if (s.pre_kind == PRE_TMUL)
                     hipLaunchKernelGGL((f_sep_col<R, N, V, PRE_TMUL>), g, bcol, col_lds, st, d, np, total);
-                else
+                else if (s.pre_kind == PRE_MASKS)
+                    hipLaunchKernelGGL((f_sep_col<R, N, V, PRE_MASKS>), g, bcol, col_lds, st, d, np, total);
+                else if (s.pre_kind == PRE_NONE)
                     hipLaunchKernelGGL((f_sep_col<R, N, V, PRE_NONE>), g, bcol, col_lds, st, d, np, total);
+                else
+                    return hipErrorInvalidValue;  // the two-pass form has no other pre-op
                 return hipGetLastError();
             }
             if (s.pass == 2) {
@@ -673,6 +679,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_TMUL>), g, brow, row_lds, st, d, np, total);
             else if (s.pre_kind == PRE_POLAR)
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_POLAR>), g, brow, row_lds, st, d, np, total);
+            else if (s.pre_kind == PRE_MASKS)
+                hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_MASKS>), g, brow, row_lds, st, d, np, total);
             else
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, -1>), g, brow, row_lds, st, d, np, total);
             break;
@@ -717,6 +725,7 @@ static hipError_t attr_nv() {
                        (const void*)f_row_fwd<R, N, V, PRE_MASK>,
                        (const void*)f_row_fwd<R, N, V, PRE_TMUL>,
                        (const void*)f_row_fwd<R, N, V, PRE_POLAR>,
+                       (const void*)f_row_fwd<R, N, V, PRE_MASKS>,
                        (const void*)f_row_fwd<R, N, V, -1>,
                        (const void*)f_col<R, N, V, true, -1>,
                        (const void*)f_col<R, N, V, false, ENG_ASM>,
@@ -736,7 +745,8 @@ static hipError_t attr_nv() {
     }
     if constexpr (sizeof(R) == 4) {
         const void* ks[] = {(const void*)f_sep_col<R, N, V, PRE_NONE>, (const void*)f_sep_col<R, N, V, PRE_MASK>,
-                            (const void*)f_sep_col<R, N, V, PRE_TMUL>, (const void*)f_sep_row<R, N, V>};
+                            (const void*)f_sep_col<R, N, V, PRE_TMUL>, (const void*)f_sep_col<R, N, V, PRE_MASKS>,
+                            (const void*)f_sep_row<R, N, V>};
         for (const void* f : ks) {
             const hipError_t e = allow_lds(f);
             if (e != hipSuccess) return e;

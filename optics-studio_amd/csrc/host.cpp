@@ -117,10 +117,18 @@ struct oc_context {
     uint64_t pass_n[OC_NUM_PASS_KINDS] = {0};
     bool force_generic = false;  // route every batch through the runtime-plan kernels
     std::map<std::pair<int, int>, int> fft_variant;  // (n, double?) -> fast-kernel layout variant
+    // The autotuned layout if there is one, else the layout that timed
+    // fastest for a masked-ASM + Fresnel round on MI355X (oc_autotune_fft
+    // over 16 problems, both precisions; 1024/2048 complex128 are within 1%
+    // across layouts).
     int variant_for(int n, bool dbl) const {
         auto it = fft_variant.find(std::make_pair(n, dbl ? 1 : 0));
-        if (it == fft_variant.end()) return 0;
-        return it->second;
+        if (it != fft_variant.end()) return it->second;
+        switch (n) {
+            case 1024: case 2048: return 1;
+            case 4096: case 8192: return dbl ? 3 : 2;
+            default: return 0;
+        }
     }
 };
 
@@ -868,7 +876,12 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             (d.ops.pre == PRE_NONE || d.ops.pre == PRE_MASK || d.ops.pre == PRE_TMUL || d.ops.pre == PRE_POLAR)
                 ? d.ops.pre
                 : -1;
-        b.pre_kind = b.pre_kind == -2 ? pk : (b.pre_kind == pk ? pk : -1);
+        // mixes of none / mask / shared transmission keep a specialised
+        // kernel that branches per problem
+        auto masklike = [](int k) { return k == PRE_NONE || k == PRE_MASK || k == PRE_TMUL || k == PRE_MASKS; };
+        if (b.pre_kind == -2 || b.pre_kind == pk) b.pre_kind = pk;
+        else if (masklike(b.pre_kind) && masklike(pk)) b.pre_kind = PRE_MASKS;
+        else b.pre_kind = -1;
         b.pd.push_back(d);
         b.idx.push_back(cand[a]);
     }

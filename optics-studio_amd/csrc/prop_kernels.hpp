@@ -26,6 +26,9 @@ __host__ __device__ inline bool conv_engine(int kind) { return kind == ENG_RS ||
 // wavelength).
 // PRE_FRAUN multiplies by the Fraunhofer offset-DFT pre-phase and axis ramp.
 enum PreOp : int { PRE_NONE = 0, PRE_MASK = 1, PRE_POLAR = 2, PRE_SEED_MSE = 3, PRE_TMUL = 4, PRE_FRAUN = 5 };
+// Kernel selector (never a descriptor's op): a launch whose problems mix
+// PRE_NONE, PRE_MASK and PRE_TMUL; each tile takes its problem's branch.
+constexpr int PRE_MASKS = 6;
 // Element-wise op fused into the last pass's (cropped) stores.
 enum PostOp : int { POST_NONE = 0, POST_GS_TARGET = 1, POST_GS_PHASE = 2, POST_ADJ_GRAD = 3 };
 enum MaskMode : int { MASK_NONE = 0, MASK_SEPARABLE = 1, MASK_GENERAL = 2 };

@@ -259,3 +259,42 @@ def test_shared_mask_batch_equals_single(G, engine, dtype):
         say("   band", b, dtype, "max |diff|", float((outs[b] - single).abs().max()))
         assert torch.equal(outs[b], single)
     ctx.close()
+
+
+@pytest.mark.parametrize("engine", [T.ENGINE_ASM, T.ENGINE_FRESNEL])
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_mixed_mask_batch_equals_single(G, engine, dtype):
+    """One batch with a shared mask (two bands), a mask of its own and no
+    mask at all: the launch takes the mixed pre-op kernels (complex64 Fresnel:
+    the two-pass form) and every problem must equal its single-problem
+    result bit for bit."""
+    import torch
+    n, d = 512, 2e-6
+    rng = np.random.default_rng(8)
+    shared, k1 = G.dev_mask(rng.uniform(0, 2 * np.pi, (n, n)), d, d, aperture=(T.APERTURE_CIRCLE, 0.45 * n * d, 0.0))
+    own, k2 = G.dev_mask(rng.uniform(0, 2 * np.pi, (n, n)), d, d)
+    cdt = torch.complex64 if dtype == "c64" else torch.complex128
+    u = torch.as_tensor(field(n, 11)).to(cdt).cuda()
+    plan = [(450e-9, shared), (532e-9, shared), (600e-9, own), (633e-9, None)]
+    outs = [torch.empty_like(u) for _ in plan]
+    arr = (T.PropDesc * len(plan))()
+    for b, (lam, m) in enumerate(plan):
+        x = arr[b]
+        x.geom = T.geom(n, n, d, d, lam)
+        x.engine = engine
+        x.z, x.padding, x.band_limited, x.strict, x.max_padding, x.medium = 2e-3, 2.0, 1, 0, 2.0, 1.0
+        if m is not None:
+            x.mask = T.C.pointer(m)
+        x.in_ = u.data_ptr()
+        x.out = outs[b].data_ptr()
+    ctx = G.Context(0)
+    say("mixed-mask batch", engine, dtype)
+    assert G.propagate_batch_dev(arr, double=dtype == "c128", ctx=ctx) == 0
+    for b, (lam, m) in enumerate(plan):
+        cfg = T.AsmConfig(2.0, 1, 0, 2.0, 1.0) if engine == T.ENGINE_ASM else T.FresnelConfig(2.0, 0, 0, 2.0)
+        fn = G.asm_propagate_dev if engine == T.ENGINE_ASM else G.fresnel_propagate_dev
+        st, single, _ = fn(u, d, d, lam, 2e-3, cfg, mask=m, ctx=ctx)
+        assert st == 0
+        say("   problem", b, "max |diff|", float((outs[b] - single).abs().max()))
+        assert torch.equal(outs[b], single)
+    ctx.close()
