@@ -379,12 +379,15 @@ def main():
     d_fr = descs(T.ENGINE_FRESNEL, u_asm, u_fr)
 
     def step(da, df, dbl=False):
-        st = G.propagate_batch_dev(da, double=dbl, ctx=ctx)
+        # both batches queued, one wait: the Fresnel batch reads the ASM
+        # batch's outputs and runs after it on the context stream
+        for what, ds in (("ASM", da), ("Fresnel", df)):
+            st = G.propagate_batch_submit_dev(ds, double=dbl, ctx=ctx)
+            if st:
+                raise RuntimeError(f"{what} batch failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
+        st = G.wait_dev(ctx)
         if st:
-            raise RuntimeError(f"ASM batch failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
-        st = G.propagate_batch_dev(df, double=dbl, ctx=ctx)
-        if st:
-            raise RuntimeError(f"Fresnel batch failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
+            raise RuntimeError(f"step failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
 
     # pick the fastest fast-kernel layout for the padded length (untimed)
     variant, variant_ms = ctx.autotune(2 * n, double=False)

@@ -418,6 +418,20 @@ int oc_propagate_batch_dev_c64(oc_context* ctx, oc_prop_desc* descs,
 int oc_propagate_batch_dev_c128(oc_context* ctx, oc_prop_desc* descs,
                                 size_t count);
 
+/* The same without waiting (no counterpart in the reference: a pipelining
+   form for serving loops).  Host-side checks fill descs[i].status now and
+   the batch is queued on the context stream; oc_context_wait resolves the
+   device-side statuses of every queued batch, writes them into their descs
+   (which must stay valid until then) and returns the first non-OK status
+   in submit order.  Queued batches run in submit order, so a later batch
+   may read an earlier one's outputs.  The submit returns the first
+   host-detected status (its other problems still run) or OC_OK. */
+int oc_propagate_batch_submit_dev_c64(oc_context* ctx, oc_prop_desc* descs,
+                                      size_t count);
+int oc_propagate_batch_submit_dev_c128(oc_context* ctx, oc_prop_desc* descs,
+                                       size_t count);
+int oc_context_wait(oc_context* ctx);
+
 /* ------------------------------------------------------------------------ */
 /* Phase mask (src/element.rs).                                              */
 /* ------------------------------------------------------------------------ */

@@ -116,6 +116,7 @@ struct oc_context {
     double pass_ms[OC_NUM_PASS_KINDS] = {0};
     uint64_t pass_n[OC_NUM_PASS_KINDS] = {0};
     bool force_generic = false;  // route every batch through the runtime-plan kernels
+    void* queued = nullptr;      // batches submitted without waiting (QueuedSet, created on first use)
     std::map<std::pair<int, int>, int> fft_variant;  // (n, double?) -> fast-kernel layout variant
     // The autotuned layout if there is one, else the layout that timed
     // fastest for a masked-ASM + Fresnel round on MI355X (oc_autotune_fft
@@ -689,6 +690,24 @@ void free_batch(oc_context* ctx, Batch& b) {
     b.dprobs = nullptr;
 }
 
+// Batches of oc_propagate_batch_submit_dev_* not yet waited for.
+struct Queued {
+    std::vector<Prob> probs;
+    int* dflags = nullptr;
+    oc_prop_desc* descs = nullptr;
+};
+struct QueuedSet {
+    std::vector<Queued> batches;
+};
+QueuedSet& queued_set(oc_context* ctx) {
+    if (!ctx->queued) ctx->queued = new QueuedSet();
+    return *static_cast<QueuedSet*>(ctx->queued);
+}
+void release_queued(oc_context* ctx) {
+    delete static_cast<QueuedSet*>(ctx->queued);
+    ctx->queued = nullptr;
+}
+
 int batch_alloc(oc_context* ctx, Batch& b, bool persistent, const std::string& name, size_t bytes, void** out) {
     if (!persistent) return ensure_buf(ctx, name, bytes, out);
     HIP_TRY(ctx, hipMalloc(out, std::max<size_t>(bytes, 256)));
@@ -1169,6 +1188,7 @@ void oc_context_destroy(oc_context* ctx) {
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->pinned_done) (void)hipEventDestroy(ctx->pinned_done);
+    release_queued(ctx);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
 }
@@ -1411,10 +1431,7 @@ int oc_rs_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float*
     return dev_rs(ctx, false, g, in, z, cfg, out, rep);
 }
 
-static int batch(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
-    if (!ctx || (!d && count)) return OC_ERR_INVALID_ARGUMENT;
-    (void)hipSetDevice(ctx->device);
-    std::vector<Prob> v;
+static int batch_probs(const oc_prop_desc* d, size_t count, std::vector<Prob>& v) {
     v.reserve(count);
     for (size_t i = 0; i < count; ++i) {
         const oc_prop_desc& x = d[i];
@@ -1424,13 +1441,70 @@ static int batch(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
         v.push_back(make_prob(&x.geom, x.z, x.engine, x.band_limited, x.strict, x.padding, x.max_padding,
                               x.engine == OC_ENGINE_ASM ? x.medium : 1.0, x.mask, x.in, x.out));
     }
-    int st = run_batch(ctx, dbl, v);
+    return OC_OK;
+}
+
+static int batch(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
+    if (!ctx || (!d && count)) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    std::vector<Prob> v;
+    int st = batch_probs(d, count, v);
+    if (st) return st;
+    st = run_batch(ctx, dbl, v);
     for (size_t i = 0; i < count; ++i) d[i].status = v[i].status;
     return st;
 }
 
+// Submit / wait: batches queued on the context stream with their flags in
+// buffers of their own, resolved together by oc_context_wait.
+static int submit(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
+    if (!ctx || (!d && count)) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    QueuedSet& qs = queued_set(ctx);
+    Queued q;
+    int st = batch_probs(d, count, q.probs);
+    if (st) return st;
+    void* f;
+    st = ensure_buf(ctx, "queued_flags_" + std::to_string(qs.batches.size()), sizeof(int) * std::max<size_t>(count, 1),
+                    &f);
+    if (st) return st;
+    q.dflags = static_cast<int*>(f);
+    q.descs = d;
+    qs.batches.push_back(std::move(q));
+    Queued& b = qs.batches.back();
+    st = launch_problems(ctx, dbl, b.probs, b.dflags);
+    if (st) return st;  // a device error: the queued batches are resolved by the next wait
+    int first = OC_OK;
+    for (size_t i = 0; i < count; ++i) {
+        d[i].status = b.probs[i].status;  // host-detected statuses now, the rest at the wait
+        if (first == OC_OK && b.probs[i].status != OC_OK) first = b.probs[i].status;
+    }
+    return first;
+}
+
 int oc_propagate_batch_dev_c64(oc_context* ctx, oc_prop_desc* d, size_t count) { return batch(ctx, false, d, count); }
 int oc_propagate_batch_dev_c128(oc_context* ctx, oc_prop_desc* d, size_t count) { return batch(ctx, true, d, count); }
+int oc_propagate_batch_submit_dev_c64(oc_context* ctx, oc_prop_desc* d, size_t count) {
+    return submit(ctx, false, d, count);
+}
+int oc_propagate_batch_submit_dev_c128(oc_context* ctx, oc_prop_desc* d, size_t count) {
+    return submit(ctx, true, d, count);
+}
+
+int oc_context_wait(oc_context* ctx) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (!ctx->queued) return OC_OK;
+    QueuedSet& qs = queued_set(ctx);
+    int first = OC_OK;
+    for (Queued& q : qs.batches) {
+        const int st = finish_problems(ctx, q.probs, q.dflags);
+        for (size_t i = 0; i < q.probs.size(); ++i) q.descs[i].status = q.probs[i].status;
+        if (first == OC_OK && st != OC_OK) first = st;
+    }
+    qs.batches.clear();
+    return first;
+}
 
 int oc_phasemask_validate(const oc_phase_mask* m) {
     if (!m) return OC_ERR_INVALID_ARGUMENT;

@@ -99,6 +99,9 @@ EXPORTS = {
     "oc_fd_gradient_dev_c64": ([vp, P(T.BandGeom), P(T.DesignStep), c_sz, c_sz, vp, vp, vp, vp, P(T.AsmConfig), vp,
                                 vp, vp, vp, P(c_sz)], C.c_int),
     "oc_adam_step_dev": ([vp, c_sz, vp, vp, vp, vp, vp, vp, C.c_uint64, P(T.AdamConfig)], C.c_int),
+    "oc_propagate_batch_submit_dev_c64": ([vp, P(T.PropDesc), c_sz], C.c_int),
+    "oc_propagate_batch_submit_dev_c128": ([vp, P(T.PropDesc), c_sz], C.c_int),
+    "oc_context_wait": ([vp], C.c_int),
     "oc_render": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz, P(T.AsmConfig), vp,
                    vp], C.c_int),
     "oc_render_dev_c128": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz,
@@ -436,6 +439,21 @@ def propagate_batch_dev(descs, double=False, ctx=None):
     _torch().cuda.synchronize()
     fn = lib.oc_propagate_batch_dev_c128 if double else lib.oc_propagate_batch_dev_c64
     return fn(ctx.h, descs, len(descs))
+
+
+def propagate_batch_submit_dev(descs, double=False, ctx=None):
+    """Queues a batch without waiting (descs must stay alive until wait_dev);
+    returns the first host-detected status."""
+    ctx = ctx or context()
+    _torch().cuda.synchronize()
+    fn = lib.oc_propagate_batch_submit_dev_c128 if double else lib.oc_propagate_batch_submit_dev_c64
+    return fn(ctx.h, descs, len(descs))
+
+
+def wait_dev(ctx=None):
+    """Resolves every queued batch of the context; returns the first non-OK status."""
+    ctx = ctx or context()
+    return lib.oc_context_wait(ctx.h)
 
 
 class DevEngine:
