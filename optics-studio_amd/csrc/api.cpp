@@ -1,0 +1,573 @@
+// C entry points of the propagation engines, phase masks, batches,
+// contexts and the layout autotuner (include/optics_gpu.h).
+#include "runtime.hpp"
+
+using namespace ocr;
+
+extern "C" {
+
+const char* oc_version(void) { return "optics-gpu 0.1.0 (gfx950)"; }
+
+const char* oc_status_name(int s) {
+    switch (s) {
+        case OC_OK: return "OC_OK";
+        case OC_INVALID_DISTANCE: return "OC_INVALID_DISTANCE";
+        case OC_INVALID_WAVELENGTH: return "OC_INVALID_WAVELENGTH";
+        case OC_INVALID_MEDIUM: return "OC_INVALID_MEDIUM";
+        case OC_EMPTY_GRID: return "OC_EMPTY_GRID";
+        case OC_NON_FINITE_SAMPLE: return "OC_NON_FINITE_SAMPLE";
+        case OC_ALIASING_RISK: return "OC_ALIASING_RISK";
+        case OC_REGIME_VIOLATION: return "OC_REGIME_VIOLATION";
+        case OC_INVALID_APERTURE: return "OC_INVALID_APERTURE";
+        case OC_OUTSIDE_FIELD: return "OC_OUTSIDE_FIELD";
+        case OC_INVALID_AXIS: return "OC_INVALID_AXIS";
+        case OC_INVALID_PHASE: return "OC_INVALID_PHASE";
+        case OC_INVALID_ORIENTATION: return "OC_INVALID_ORIENTATION";
+        case OC_INVALID_GRID: return "OC_INVALID_GRID";
+        case OC_INVALID_GRID_PITCH: return "OC_INVALID_GRID_PITCH";
+        case OC_INV_NOT_FINITE: return "OC_INV_NOT_FINITE";
+        case OC_INV_OUT_OF_BOUNDS: return "OC_INV_OUT_OF_BOUNDS";
+        case OC_INV_WRONG_DIMENSION: return "OC_INV_WRONG_DIMENSION";
+        case OC_INV_CARRIER: return "OC_INV_CARRIER";
+        case OC_INV_RENDER: return "OC_INV_RENDER";
+        case OC_INV_ADJOINT: return "OC_INV_ADJOINT";
+        case OC_ERR_INVALID_ARGUMENT: return "OC_ERR_INVALID_ARGUMENT";
+        case OC_ERR_UNSUPPORTED: return "OC_ERR_UNSUPPORTED";
+        case OC_ERR_DEVICE: return "OC_ERR_DEVICE";
+        case OC_ERR_OUT_OF_MEMORY: return "OC_ERR_OUT_OF_MEMORY";
+        default: return "OC_UNKNOWN";
+    }
+}
+
+int oc_context_create(int device, oc_context** out) {
+    if (!out) return OC_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return OC_ERR_DEVICE;
+    if (device < 0 || device >= ndev) return OC_ERR_INVALID_ARGUMENT;
+    std::unique_ptr<oc_context> c(new oc_context());
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) return OC_ERR_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) return OC_ERR_DEVICE;
+    c->stream = c->own;
+    if (ocl::set_lds_limits() != hipSuccess) return OC_ERR_DEVICE;
+    if (ocl::fast_set_lds_limits_c64() != hipSuccess || ocl::fast_set_lds_limits_c128() != hipSuccess)
+        return OC_ERR_DEVICE;
+    *out = c.release();
+    return OC_OK;
+}
+
+void oc_context_destroy(oc_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->bufs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    for (auto& kv : ctx->twiddles) (void)hipFree(kv.second);
+    for (auto& kv : ctx->tables) (void)hipFree(kv.second.dev);
+    collect_timing(ctx);
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->pinned_done) (void)hipEventDestroy(ctx->pinned_done);
+    release_queued(ctx);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+}
+
+int oc_context_set_stream(oc_context* ctx, void* stream) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own;
+    return OC_OK;
+}
+
+void* oc_context_stream(oc_context* ctx) { return ctx ? ctx->stream : nullptr; }
+
+int oc_context_synchronize(oc_context* ctx) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return OC_OK;
+}
+
+int oc_context_enable_timing(oc_context* ctx, int enable) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    ctx->timing = enable != 0;
+    return OC_OK;
+}
+
+int oc_context_pass_timing(oc_context* ctx, double* ms, uint64_t* launches, int reset) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timing(ctx);
+    for (int i = 0; i < OC_NUM_PASS_KINDS; ++i) {
+        if (ms) ms[i] = ctx->pass_ms[i];
+        if (launches) launches[i] = ctx->pass_n[i];
+        if (reset) {
+            ctx->pass_ms[i] = 0.0;
+            ctx->pass_n[i] = 0;
+        }
+    }
+    return OC_OK;
+}
+
+const char* oc_last_error(oc_context* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int oc_asm_sampling_report(const oc_band_geom* g, double z, const oc_asm_config* cfg, oc_asm_report* rep) {
+    if (!g || !cfg || !rep) return OC_ERR_INVALID_ARGUMENT;
+    *rep = och::asm_analyze(*g, z, *cfg, cfg->padding);
+    return OC_OK;
+}
+
+int oc_fresnel_sampling_report(const oc_band_geom* g, double z, const oc_fresnel_config* cfg,
+                               oc_fresnel_report* rep) {
+    if (!g || !cfg || !rep) return OC_ERR_INVALID_ARGUMENT;
+    *rep = och::fresnel_analyze(*g, z, *cfg, cfg->padding);
+    return OC_OK;
+}
+
+int oc_rs_sampling_report(const oc_band_geom* g, double z, const oc_rs_config* cfg, oc_rs_report* rep) {
+    if (!g || !cfg || !rep) return OC_ERR_INVALID_ARGUMENT;
+    *rep = och::rs_analyze(*g, z, cfg->padding);
+    return OC_OK;
+}
+
+static void fill_asm_report(const Prob& p, oc_asm_report* rep) {
+    if (rep) *rep = p.asm_rep;
+}
+
+int oc_asm_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z, const oc_asm_config* cfg,
+                     double* out, oc_asm_report* rep) {
+    if (!ctx || !g || !cfg || (!in && g->nx * g->ny) || (!out && g->nx * g->ny)) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    Prob done;
+    Prob p = make_prob(g, z, ENG_ASM, cfg->band_limited, cfg->strict, cfg->padding, cfg->max_padding, cfg->medium,
+                       nullptr, nullptr, nullptr);
+    int st = run_host(ctx, p, in, out, nullptr, &done);
+    fill_asm_report(done, rep);
+    return st;
+}
+
+static int dev_asm(oc_context* ctx, bool dbl, const oc_band_geom* g, const void* in, double z,
+                   const oc_asm_config* cfg, const oc_phase_mask* mask, void* out, oc_asm_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    std::vector<Prob> v{make_prob(g, z, ENG_ASM, cfg->band_limited, cfg->strict, cfg->padding, cfg->max_padding,
+                                  cfg->medium, mask, in, out)};
+    int st = run_batch(ctx, dbl, v);
+    fill_asm_report(v[0], rep);
+    return st;
+}
+
+int oc_asm_propagate_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                              const oc_asm_config* cfg, const oc_phase_mask* mask, double* out, oc_asm_report* rep) {
+    return dev_asm(ctx, true, g, in, z, cfg, mask, out, rep);
+}
+int oc_asm_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
+                             const oc_asm_config* cfg, const oc_phase_mask* mask, float* out, oc_asm_report* rep) {
+    return dev_asm(ctx, false, g, in, z, cfg, mask, out, rep);
+}
+
+int oc_fresnel_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                         const oc_fresnel_config* cfg, double* out, oc_fresnel_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    Prob done;
+    Prob p = make_prob(g, z, ENG_FRESNEL, 0, cfg->strict, cfg->padding, cfg->max_padding, 1.0, nullptr, nullptr,
+                       nullptr);
+    int st = run_host(ctx, p, in, out, nullptr, &done);
+    if (rep) *rep = done.fr_rep;
+    return st;
+}
+
+static int dev_fresnel(oc_context* ctx, bool dbl, const oc_band_geom* g, const void* in, double z,
+                       const oc_fresnel_config* cfg, const oc_phase_mask* mask, void* out, oc_fresnel_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    std::vector<Prob> v{make_prob(g, z, ENG_FRESNEL, 0, cfg->strict, cfg->padding, cfg->max_padding, 1.0, mask, in,
+                                  out)};
+    int st = run_batch(ctx, dbl, v);
+    if (rep) *rep = v[0].fr_rep;
+    return st;
+}
+
+int oc_fresnel_propagate_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                                  const oc_fresnel_config* cfg, const oc_phase_mask* mask, double* out,
+                                  oc_fresnel_report* rep) {
+    return dev_fresnel(ctx, true, g, in, z, cfg, mask, out, rep);
+}
+int oc_fresnel_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
+                                 const oc_fresnel_config* cfg, const oc_phase_mask* mask, float* out,
+                                 oc_fresnel_report* rep) {
+    return dev_fresnel(ctx, false, g, in, z, cfg, mask, out, rep);
+}
+
+int oc_fresnel_propagate_convolution(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                                     const oc_fresnel_config* cfg, double* out, oc_fresnel_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    Prob done;
+    Prob p = make_prob(g, z, ENG_FRESNEL_CONV, 0, cfg->strict, cfg->padding, cfg->max_padding, 1.0, nullptr, nullptr,
+                       nullptr);
+    int st = run_host(ctx, p, in, out, nullptr, &done);
+    if (rep) *rep = done.fr_rep;
+    return st;
+}
+
+static int dev_fresnel_conv(oc_context* ctx, bool dbl, const oc_band_geom* g, const void* in, double z,
+                            const oc_fresnel_config* cfg, void* out, oc_fresnel_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    std::vector<Prob> v{
+        make_prob(g, z, ENG_FRESNEL_CONV, 0, cfg->strict, cfg->padding, cfg->max_padding, 1.0, nullptr, in, out)};
+    int st = run_batch(ctx, dbl, v);
+    if (rep) *rep = v[0].fr_rep;
+    return st;
+}
+
+int oc_fresnel_propagate_convolution_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                                              const oc_fresnel_config* cfg, double* out, oc_fresnel_report* rep) {
+    return dev_fresnel_conv(ctx, true, g, in, z, cfg, out, rep);
+}
+int oc_fresnel_propagate_convolution_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
+                                             const oc_fresnel_config* cfg, float* out, oc_fresnel_report* rep) {
+    return dev_fresnel_conv(ctx, false, g, in, z, cfg, out, rep);
+}
+
+int oc_fraunhofer_regime_report(const oc_band_geom* g, double z, const oc_fraunhofer_config* cfg,
+                                oc_fraunhofer_report* rep) {
+    if (!g || !cfg || !rep) return OC_ERR_INVALID_ARGUMENT;
+    *rep = och::fraunhofer_report(*g, z, *cfg);
+    return OC_OK;
+}
+
+int oc_fraunhofer_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                            const oc_fraunhofer_config* cfg, double* out, oc_fraunhofer_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    Prob done;
+    Prob p = make_prob(g, z, ENG_FRAUNHOFER, 0, cfg->strict, 1.0, 1.0, 1.0, nullptr, nullptr, nullptr);
+    p.fh_cfg = *cfg;
+    int st = run_host(ctx, p, in, out, nullptr, &done);
+    if (rep) *rep = done.fh_rep;
+    return st;
+}
+
+static int dev_fraunhofer(oc_context* ctx, bool dbl, const oc_band_geom* g, const void* in, double z,
+                          const oc_fraunhofer_config* cfg, void* out, oc_fraunhofer_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    std::vector<Prob> v{make_prob(g, z, ENG_FRAUNHOFER, 0, cfg->strict, 1.0, 1.0, 1.0, nullptr, in, out)};
+    v[0].fh_cfg = *cfg;
+    int st = run_batch(ctx, dbl, v);
+    if (rep) *rep = v[0].fh_rep;
+    return st;
+}
+
+int oc_fraunhofer_propagate_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                                     const oc_fraunhofer_config* cfg, double* out, oc_fraunhofer_report* rep) {
+    return dev_fraunhofer(ctx, true, g, in, z, cfg, out, rep);
+}
+int oc_fraunhofer_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
+                                    const oc_fraunhofer_config* cfg, float* out, oc_fraunhofer_report* rep) {
+    return dev_fraunhofer(ctx, false, g, in, z, cfg, out, rep);
+}
+
+int oc_rs_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z, const oc_rs_config* cfg,
+                    double* out, oc_rs_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    Prob done;
+    Prob p = make_prob(g, z, ENG_RS, 0, cfg->strict, cfg->padding, cfg->padding, 1.0, nullptr, nullptr, nullptr);
+    int st = run_host(ctx, p, in, out, nullptr, &done);
+    if (rep) *rep = done.rs_rep;
+    return st;
+}
+
+static int dev_rs(oc_context* ctx, bool dbl, const oc_band_geom* g, const void* in, double z,
+                  const oc_rs_config* cfg, void* out, oc_rs_report* rep) {
+    if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
+    if (rep) std::memset(rep, 0, sizeof *rep);
+    (void)hipSetDevice(ctx->device);
+    std::vector<Prob> v{make_prob(g, z, ENG_RS, 0, cfg->strict, cfg->padding, cfg->padding, 1.0, nullptr, in, out)};
+    int st = run_batch(ctx, dbl, v);
+    if (rep) *rep = v[0].rs_rep;
+    return st;
+}
+
+int oc_rs_propagate_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
+                             const oc_rs_config* cfg, double* out, oc_rs_report* rep) {
+    return dev_rs(ctx, true, g, in, z, cfg, out, rep);
+}
+int oc_rs_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
+                            const oc_rs_config* cfg, float* out, oc_rs_report* rep) {
+    return dev_rs(ctx, false, g, in, z, cfg, out, rep);
+}
+
+static int batch_probs(const oc_prop_desc* d, size_t count, std::vector<Prob>& v) {
+    v.reserve(count);
+    for (size_t i = 0; i < count; ++i) {
+        const oc_prop_desc& x = d[i];
+        if ((x.engine < OC_ENGINE_ASM || x.engine > OC_ENGINE_RS) && x.engine != OC_ENGINE_FRESNEL_CONV)
+            return OC_ERR_INVALID_ARGUMENT;
+        if ((x.engine == OC_ENGINE_RS || x.engine == OC_ENGINE_FRESNEL_CONV) && x.mask) return OC_ERR_INVALID_ARGUMENT;
+        v.push_back(make_prob(&x.geom, x.z, x.engine, x.band_limited, x.strict, x.padding, x.max_padding,
+                              x.engine == OC_ENGINE_ASM ? x.medium : 1.0, x.mask, x.in, x.out));
+    }
+    return OC_OK;
+}
+
+static int batch(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
+    if (!ctx || (!d && count)) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    std::vector<Prob> v;
+    int st = batch_probs(d, count, v);
+    if (st) return st;
+    st = run_batch(ctx, dbl, v);
+    for (size_t i = 0; i < count; ++i) d[i].status = v[i].status;
+    return st;
+}
+
+// Submit / wait: batches queued on the context stream with their flags in
+// buffers of their own, resolved together by oc_context_wait.
+static int submit(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
+    if (!ctx || (!d && count)) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    QueuedSet& qs = queued_set(ctx);
+    Queued q;
+    int st = batch_probs(d, count, q.probs);
+    if (st) return st;
+    void* f;
+    st = ensure_buf(ctx, "queued_flags_" + std::to_string(qs.batches.size()), sizeof(int) * std::max<size_t>(count, 1),
+                    &f);
+    if (st) return st;
+    q.dflags = static_cast<int*>(f);
+    q.descs = d;
+    qs.batches.push_back(std::move(q));
+    Queued& b = qs.batches.back();
+    st = launch_problems(ctx, dbl, b.probs, b.dflags);
+    if (st) return st;  // a device error: the queued batches are resolved by the next wait
+    int first = OC_OK;
+    for (size_t i = 0; i < count; ++i) {
+        d[i].status = b.probs[i].status;  // host-detected statuses now, the rest at the wait
+        if (first == OC_OK && b.probs[i].status != OC_OK) first = b.probs[i].status;
+    }
+    return first;
+}
+
+int oc_propagate_batch_dev_c64(oc_context* ctx, oc_prop_desc* d, size_t count) { return batch(ctx, false, d, count); }
+int oc_propagate_batch_dev_c128(oc_context* ctx, oc_prop_desc* d, size_t count) { return batch(ctx, true, d, count); }
+int oc_propagate_batch_submit_dev_c64(oc_context* ctx, oc_prop_desc* d, size_t count) {
+    return submit(ctx, false, d, count);
+}
+int oc_propagate_batch_submit_dev_c128(oc_context* ctx, oc_prop_desc* d, size_t count) {
+    return submit(ctx, true, d, count);
+}
+
+int oc_context_wait(oc_context* ctx) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (!ctx->queued) return OC_OK;
+    QueuedSet& qs = queued_set(ctx);
+    int first = OC_OK;
+    for (Queued& q : qs.batches) {
+        const int st = finish_problems(ctx, q.probs, q.dflags);
+        for (size_t i = 0; i < q.probs.size(); ++i) q.descs[i].status = q.probs[i].status;
+        if (first == OC_OK && st != OC_OK) first = st;
+    }
+    qs.batches.clear();
+    return first;
+}
+
+int oc_phasemask_validate(const oc_phase_mask* m) {
+    if (!m) return OC_ERR_INVALID_ARGUMENT;
+    if (!std::isfinite(och::wrap_angle(m->orientation))) return OC_INVALID_ORIENTATION;
+    if (m->nx == 0 || m->ny == 0 || !m->values) return OC_INVALID_GRID;
+    if (!std::isfinite(m->dx) || m->dx <= 0.0 || !std::isfinite(m->dy) || m->dy <= 0.0) return OC_INVALID_GRID_PITCH;
+    for (size_t k = 0; k < m->nx * m->ny; ++k)
+        if (!std::isfinite(m->values[k])) return OC_INVALID_PHASE;
+    auto fp = [&](double hx, double hy) {
+        return (!std::isfinite(hx) || hx <= 0.0 || !std::isfinite(hy) || hy <= 0.0 || !std::isfinite(m->cx) ||
+                !std::isfinite(m->cy))
+                   ? OC_INVALID_APERTURE
+                   : OC_OK;
+    };
+    switch (m->aperture.kind) {
+        case OC_APERTURE_FIELD: return (std::isfinite(m->cx) && std::isfinite(m->cy)) ? OC_OK : OC_INVALID_AXIS;
+        case OC_APERTURE_CIRCLE: return fp(m->aperture.a, m->aperture.a);
+        case OC_APERTURE_ELLIPSE:
+        case OC_APERTURE_RECTANGLE: return fp(m->aperture.a, m->aperture.b);
+        default: return OC_ERR_INVALID_ARGUMENT;
+    }
+}
+
+static int mask_dev(oc_context* ctx, bool dbl, const oc_band_geom* g, const void* in, const oc_phase_mask* m,
+                    void* out) {
+    if (!ctx || !g || !m) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    int st = mask_footprint(*g, *m);
+    if (st) return st;
+    MaskDev md;
+    st = mask_tables(ctx, *g, *m, m->values, &md);
+    if (st) return st;
+    void* f;
+    st = ensure_buf(ctx, "flags", sizeof(int), &f);
+    if (st) return st;
+    HIP_TRY(ctx, ocl::launch_fill_int(static_cast<int*>(f), 0, 1, ctx->stream));
+    hipError_t e = dbl ? ocl::launch_mask_apply_c128(in, out, md, (int)g->nx, (int)g->ny, nullptr, ctx->stream)
+                       : ocl::launch_mask_apply_c64(in, out, md, (int)g->nx, (int)g->ny, nullptr, ctx->stream);
+    if (e != hipSuccess) return fail(ctx, OC_ERR_DEVICE, hipGetErrorString(e));
+    return OC_OK;
+}
+
+int oc_phasemask_apply_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, const oc_phase_mask* m,
+                                double* out) {
+    return mask_dev(ctx, true, g, in, m, out);
+}
+int oc_phasemask_apply_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, const oc_phase_mask* m,
+                               float* out) {
+    return mask_dev(ctx, false, g, in, m, out);
+}
+
+int oc_phasemask_apply(oc_context* ctx, const oc_band_geom* g, const double* in, const oc_phase_mask* m, double* out) {
+    if (!ctx || !g || !m) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    int st = mask_footprint(*g, *m);
+    if (st) return st;
+    // PhaseMask::sampled validated its lattice at construction; a non-finite
+    // lattice value would surface as InvalidPhase at apply time.
+    for (size_t k = 0; k < m->nx * m->ny; ++k)
+        if (!std::isfinite(m->values[k])) return OC_INVALID_PHASE;
+    const size_t n = g->nx * g->ny;
+    if (n == 0) return OC_OK;
+    void *din, *dout, *dmask;
+    st = ensure_buf(ctx, "host_in", n * 16, &din);
+    if (st) return st;
+    st = ensure_buf(ctx, "host_out", n * 16, &dout);
+    if (st) return st;
+    st = ensure_buf(ctx, "host_mask", std::max<size_t>(m->nx * m->ny, 1) * 8, &dmask);
+    if (st) return st;
+    HIP_TRY(ctx, hipMemcpyAsync(din, in, n * 16, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(dmask, m->values, m->nx * m->ny * 8, hipMemcpyHostToDevice, ctx->stream));
+    oc_phase_mask dm = *m;
+    dm.values = static_cast<const double*>(dmask);
+    st = mask_dev(ctx, true, g, din, &dm, dout);
+    if (st) return st;
+    HIP_TRY(ctx, hipMemcpyAsync(out, dout, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return OC_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int oc_context_set_fft_variant(oc_context* ctx, uint64_t n, int dbl, int variant) {
+    if (!ctx || variant < 0 || variant >= std::max(1, ocl::fast_variants((int)n))) return OC_ERR_INVALID_ARGUMENT;
+    ctx->fft_variant[std::make_pair((int)n, dbl ? 1 : 0)] = variant;
+    return OC_OK;
+}
+
+int oc_context_fft_variant(oc_context* ctx, uint64_t n, int dbl) {
+    if (!ctx || ocl::fast_variants((int)n) == 0) return -1;
+    return ctx->variant_for((int)n, dbl != 0);
+}
+
+int oc_context_force_generic(oc_context* ctx, int enable) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    ctx->force_generic = enable != 0;
+    return OC_OK;
+}
+
+// Times every layout variant of the specialised kernels at padded length n on
+// a synthetic round of masked ASM then Fresnel propagations (up to 16 fields
+// of n/2 x n/2, padding 2) and keeps the fastest.
+int oc_autotune_fft(oc_context* ctx, uint64_t n, int dbl, int* variant_out, double* ms_out) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    const int nv = ocl::fast_variants((int)n);
+    if (ms_out)
+        for (int v = 0; v < OC_FFT_VARIANTS; ++v) ms_out[v] = -1.0;
+    if (nv == 0) {
+        if (variant_out) *variant_out = -1;
+        return OC_OK;
+    }
+    const size_t half = n / 2, esz = dbl ? 16 : 8;
+    const size_t per = half * half;
+    const size_t nb = std::max<size_t>(1, std::min<size_t>(16, ((size_t)1 << 27) / per));
+    Scoped mem(ctx);
+    void *in, *out, *mv;
+    int st;
+    if ((st = mem.alloc(nb * per * esz, &in)) || (st = mem.alloc(nb * per * esz, &out)) ||
+        (st = mem.alloc(per * 8, &mv)))
+        return st;
+    HIP_TRY(ctx, hipMemsetAsync(in, 0, nb * per * esz, ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(mv, 0, per * 8, ctx->stream));
+    oc_phase_mask m;
+    std::memset(&m, 0, sizeof m);
+    m.values = static_cast<const double*>(mv);
+    m.nx = m.ny = half;
+    m.dx = m.dy = 2e-6;
+    oc_band_geom g{half, half, 2e-6, 2e-6, 550e-9};
+    std::vector<Prob> probs, probs_f;
+    for (size_t b = 0; b < nb; ++b) {
+        unsigned char* bi = static_cast<unsigned char*>(in) + b * per * esz;
+        unsigned char* bo = static_cast<unsigned char*>(out) + b * per * esz;
+        probs.push_back(make_prob(&g, 5e-3, ENG_ASM, 1, 0, 2.0, 2.0, 1.0, &m, bi, bo));
+        probs_f.push_back(make_prob(&g, 0.2, ENG_FRESNEL, 0, 0, 2.0, 2.0, 1.0, nullptr, bo, bi));
+    }
+    const auto key = std::make_pair((int)n, dbl ? 1 : 0);
+    const bool had = ctx->fft_variant.count(key) != 0;
+    const int prev = had ? ctx->fft_variant[key] : 0;
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    int best = 0;
+    double best_ms = 1e300;
+    for (int v = 0; v < nv; ++v) {
+        ctx->fft_variant[key] = v;
+        double ms = 1e300;
+        for (int rep = 0; rep < 4; ++rep) {
+            std::vector<Prob> pv = probs, pf = probs_f;
+            hipEvent_t a = pool_event(ctx), e = pool_event(ctx);
+            (void)hipEventRecord(a, ctx->stream);
+            st = run_batch(ctx, dbl, pv);
+            if (!st) st = run_batch(ctx, dbl, pf);
+            (void)hipEventRecord(e, ctx->stream);
+            (void)hipEventSynchronize(e);
+            float f = 0.f;
+            (void)hipEventElapsedTime(&f, a, e);
+            ctx->event_pool.push_back(a);
+            ctx->event_pool.push_back(e);
+            if (st) break;
+            if (rep > 0) ms = std::min(ms, (double)f);
+        }
+        if (st) break;
+        if (ms_out) ms_out[v] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    ctx->timing = timing;
+    if (st) {
+        if (had) ctx->fft_variant[key] = prev;
+        else ctx->fft_variant.erase(key);
+        return st;
+    }
+    ctx->fft_variant[key] = best;
+    if (variant_out) *variant_out = best;
+    return OC_OK;
+}
+
+}  // extern "C"

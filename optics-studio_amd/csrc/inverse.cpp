@@ -1,0 +1,561 @@
+// Gerchberg-Saxton (drop-in and FFT form) and the adjoint gradient.
+#include "runtime.hpp"
+
+using namespace ocr;
+
+namespace {
+
+
+inline double rem_euclid_h(double x, double m) {
+    const double r = std::fmod(x, m);
+    return r < 0.0 ? r + std::fabs(m) : r;
+}
+inline double encode_bounds_h(double phi, double lo, double hi) {  // gs.rs:115-121
+    if (hi - lo >= 2.0 * och::PI) return lo + rem_euclid_h(phi - lo, 2.0 * och::PI);
+    return phi < lo ? lo : (phi > hi ? hi : phi);
+}
+inline bool contains_h(const oc_clear_aperture& ap, double x, double y) {  // element.rs:476-486
+    switch (ap.kind) {
+        case OC_APERTURE_FIELD: return true;
+        case OC_APERTURE_CIRCLE: return x * x + y * y <= ap.a * ap.a;
+        case OC_APERTURE_ELLIPSE: {
+            const double a2 = ap.a * ap.a, b2 = ap.b * ap.b;
+            return b2 * x * x + a2 * y * y <= a2 * b2;
+        }
+        default: return std::fabs(x) <= ap.a && std::fabs(y) <= ap.b;
+    }
+}
+
+// Runs the already prepared persistent batch `b` and reads `nflags` flags.
+int run_persistent(oc_context* ctx, Batch& b, int* dflags, int nflags, unsigned passes, std::vector<int>* flags_out) {
+    HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, nflags, ctx->stream));
+    int st = launch_batch(ctx, b, passes);
+    if (st) return st;
+    if (flags_out) {
+        flags_out->assign(nflags, 0);
+        HIP_TRY(ctx, hipMemcpyAsync(flags_out->data(), dflags, sizeof(int) * nflags, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    }
+    return OC_OK;
+}
+
+int gs_fft_impl(oc_context* ctx, bool dbl, uint64_t n, uint64_t R, const void* A, const void* Tamp, void* phases,
+                uint64_t iterations, uint64_t chunk, double* err_out) {
+    if (!ctx || !A || !Tamp || !phases || n == 0 || R == 0) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    evict_tables(ctx);
+    const size_t esz_r = dbl ? 8 : 4, esz = 2 * esz_r;
+    const size_t nn = (size_t)n * n;
+    Scoped mem(ctx);
+    // layout (same plan on both axes)
+    FftPlanDev pl;
+    int st = make_plan(ctx, (int)n, dbl, &pl);
+    if (st) return st;
+    Layout L = choose_layout(pl, pl, esz, n, n, false);
+    if (!L.ok) return fail(ctx, OC_ERR_UNSUPPORTED, "grid side exceeds the on-chip transform size");
+    void *tblk, *sums, *flags;
+    if ((st = mem.alloc(nn * esz_r, &tblk))) return st;
+    if ((st = mem.alloc(sizeof(double) * 4 * R, &sums))) return st;
+    if ((st = mem.alloc(sizeof(int) * R, &flags))) return st;
+    oc_band_geom g{n, n, 1.0, 1.0, 1.0};
+    std::vector<Prob> it(R), me(R);
+    for (uint64_t r = 0; r < R; ++r) {
+        Prob& p = it[r];
+        p.g = g;
+        p.engine = ENG_GS_PROJECT;
+        p.mask = nullptr;
+        p.in = nullptr;
+        p.out = nullptr;
+        p.ops.pre = PRE_POLAR;
+        p.ops.amp = A;
+        p.ops.phase = static_cast<unsigned char*>(phases) + r * nn * esz_r;
+        p.ops.open = nullptr;
+        p.ops.post = POST_GS_PHASE;
+        p.ops.phase_io = static_cast<unsigned char*>(phases) + r * nn * esz_r;
+        p.ops.beta_p = 1.0;
+        p.ops.lo = 0.0;
+        p.ops.hi = 2.0 * och::PI;
+        p.ops.tamp_blk = tblk;
+        p.ops.sums = nullptr;
+        prepare(p);
+        me[r] = p;
+        me[r].engine = ENG_MEASURE;
+        me[r].ops.post = POST_NONE;
+        me[r].ops.sums = static_cast<double*>(sums) + 4 * r;
+    }
+    Batch bi, bm;
+    st = build_batch(ctx, dbl, it, static_cast<int*>(flags), bi, true);
+    if (st) { free_batch(ctx, bi); return st; }
+    st = build_batch(ctx, dbl, me, static_cast<int*>(flags), bm, true);
+    if (st) { free_batch(ctx, bi); free_batch(ctx, bm); return st; }
+    auto cleanup = [&]() {
+        free_batch(ctx, bi);
+        free_batch(ctx, bm);
+    };
+    if (bi.pd.size() != R || bm.pd.size() != R) { cleanup(); return it[0].status ? it[0].status : OC_ERR_UNSUPPORTED; }
+    // the target amplitude in the column pass's blocked order, with the block
+    // width of the kernels the batches were built for
+    if (bi.pd[0].B != bm.pd[0].B ||
+        ocl::launch_block_reorder(dbl, Tamp, tblk, (int)n, (int)n, bi.pd[0].B, ctx->stream) != hipSuccess) {
+        cleanup();
+        return fail(ctx, OC_ERR_DEVICE, "target reorder failed");
+    }
+    const unsigned passes3 = (1u << 0) | (1u << 1) | (1u << 2);
+    uint64_t done = 0;
+    if (bi.fast && iterations > 0) {
+        // power-of-two grids: row_fwd once, then per iteration the column
+        // pass and the fused row pass (inverse rows + phase update + next
+        // forward rows); the last iteration ends with the plain inverse
+        // rows.  Graph-captured in chunks of [col, fused row].
+        // beta = 1 over [0, 2 pi) with no per-iteration outputs: amplitude
+        // replacement in the fused pass, phases from the last inverse pass
+        const bool amp_only = it[0].ops.beta_p == 1.0 && it[0].ops.hi - it[0].ops.lo >= 2.0 * och::PI &&
+                              !it[0].ops.sums && !it[0].ops.dphi_out && !it[0].out;
+        const unsigned cf = (1u << 1) | (1u << (amp_only ? PASS_GS_AMP : PASS_GS_ROW));
+        st = launch_batch(ctx, bi, 1u << 0);
+        if (st) { cleanup(); return st; }
+        const uint64_t body = iterations - 1;  // iterations with a fused row pass
+        if (chunk > 0 && body >= chunk) {
+            const bool was = ctx->timing;
+            ctx->timing = false;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+            if (e != hipSuccess) { ctx->timing = was; cleanup(); return fail(ctx, OC_ERR_DEVICE, "begin capture"); }
+            for (uint64_t k = 0; k < chunk && st == OC_OK; ++k) st = launch_batch(ctx, bi, cf);
+            e = hipStreamEndCapture(ctx->stream, &graph);
+            ctx->timing = was;
+            if (st || e != hipSuccess) { cleanup(); return st ? st : fail(ctx, OC_ERR_DEVICE, "end capture"); }
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) { (void)hipGraphDestroy(graph); cleanup(); return fail(ctx, OC_ERR_DEVICE, "instantiate"); }
+            while (body - done >= chunk) {
+                e = timed(ctx, 4, [&] { return hipGraphLaunch(exec, ctx->stream); });
+                if (e != hipSuccess) break;
+                done += chunk;
+            }
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) { cleanup(); return fail(ctx, OC_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e)); }
+        }
+        for (; done < body; ++done) {
+            st = launch_batch(ctx, bi, cf);
+            if (st) { cleanup(); return st; }
+        }
+        st = launch_batch(ctx, bi, (1u << 1) | (1u << 2));
+        if (st) { cleanup(); return st; }
+        done = iterations;
+    }
+    if (chunk > 0 && iterations - done >= chunk) {
+        // capture `chunk` iterations once, replay
+        const bool was = ctx->timing;
+        ctx->timing = false;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+        if (e != hipSuccess) { ctx->timing = was; cleanup(); return fail(ctx, OC_ERR_DEVICE, "begin capture"); }
+        for (uint64_t k = 0; k < chunk && st == OC_OK; ++k) st = launch_batch(ctx, bi, passes3);
+        e = hipStreamEndCapture(ctx->stream, &graph);
+        ctx->timing = was;
+        if (st || e != hipSuccess) { cleanup(); return st ? st : fail(ctx, OC_ERR_DEVICE, "end capture"); }
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) { (void)hipGraphDestroy(graph); cleanup(); return fail(ctx, OC_ERR_DEVICE, "instantiate"); }
+        while (iterations - done >= chunk) {
+            e = timed(ctx, 4, [&] { return hipGraphLaunch(exec, ctx->stream); });
+            if (e != hipSuccess) break;
+            done += chunk;
+        }
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) { cleanup(); return fail(ctx, OC_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e)); }
+    }
+    for (; done < iterations; ++done) {
+        st = launch_batch(ctx, bi, passes3);
+        if (st) { cleanup(); return st; }
+    }
+    // final error: spectrum statistics of the final phases
+    HIP_TRY(ctx, hipMemsetAsync(sums, 0, sizeof(double) * 4 * R, ctx->stream));
+    std::vector<int> fl;
+    st = run_persistent(ctx, bm, static_cast<int*>(flags), (int)R, (1u << 0) | (1u << 1), &fl);
+    if (st) { cleanup(); return st; }
+    std::vector<double> hs(4 * R);
+    std::vector<unsigned char> th(nn * esz_r);
+    HIP_TRY(ctx, hipMemcpyAsync(hs.data(), sums, sizeof(double) * 4 * R, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipMemcpyAsync(th.data(), Tamp, nn * esz_r, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    collect_timing(ctx);
+    cleanup();
+    double st1 = 0.0, st2 = 0.0;
+    for (size_t k = 0; k < nn; ++k) {
+        const double t = dbl ? reinterpret_cast<const double*>(th.data())[k] : reinterpret_cast<const float*>(th.data())[k];
+        st1 += t * t;
+        st2 += (t * t) * (t * t);
+    }
+    if (err_out) {
+        for (uint64_t r = 0; r < R; ++r) {
+            const double S = hs[4 * r], Q = hs[4 * r + 1], X = hs[4 * r + 2];
+            const double den = st2 / (st1 * st1);
+            const double num = Q / (S * S) - 2.0 * X / (S * st1) + den;
+            err_out[r] = std::sqrt(std::fmax(num, 0.0) / den);
+        }
+    }
+    for (int f : fl)
+        if (f) return OC_NON_FINITE_SAMPLE;
+    return OC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oc_gs_fft_run_dev_c64(oc_context* ctx, uint64_t n, uint64_t restarts, const float* source_amp,
+                          const float* target_amp, float* phases, uint64_t iterations, uint64_t graph_chunk,
+                          double* err_out) {
+    return gs_fft_impl(ctx, false, n, restarts, source_amp, target_amp, phases, iterations, graph_chunk, err_out);
+}
+int oc_gs_fft_run_dev_c128(oc_context* ctx, uint64_t n, uint64_t restarts, const double* source_amp,
+                           const double* target_amp, double* phases, uint64_t iterations, uint64_t graph_chunk,
+                           double* err_out) {
+    return gs_fft_impl(ctx, true, n, restarts, source_amp, target_amp, phases, iterations, graph_chunk, err_out);
+}
+
+// gs.rs:347-597
+int oc_gs_run(oc_context* ctx, const oc_gs_problem* pb, oc_gs_solution* sol) {
+    if (!ctx || !pb || !sol) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    evict_tables(ctx);
+    const oc_band_geom g = pb->geom;
+    const size_t nx = g.nx, ny = g.ny, n = nx * ny;
+    if (n == 0 || !pb->input || !pb->target_amplitude || !pb->merit_target) return OC_ERR_INVALID_ARGUMENT;
+    for (size_t k = 0; k < n; ++k) {
+        const double t = pb->target_amplitude[k];
+        if (!std::isfinite(t)) return OC_INV_NOT_FINITE;
+        if (t < 0.0) return OC_INV_OUT_OF_BOUNDS;
+    }
+    if (!(std::isfinite(pb->z_total) && pb->z_total > 0.0)) return OC_INV_RENDER;
+    if (pb->variant == 1 && !(std::isfinite(pb->beta) && pb->beta > 0.0 && pb->beta < 2.0)) return OC_INV_RENDER;
+    if (pb->variant == 2 && !(std::isfinite(pb->beta) && pb->beta > 0.0 && pb->beta <= 1.0)) return OC_INV_RENDER;
+    if (pb->max_iterations == 0) return OC_INV_RENDER;
+    if (pb->init_kind != 0 && pb->init_kind != 2) return OC_ERR_UNSUPPORTED;
+    if (pb->init_kind == 0 && !pb->carrier_values) return OC_ERR_INVALID_ARGUMENT;
+    if (pb->init_kind == 2 && !pb->init_phase) return OC_ERR_INVALID_ARGUMENT;
+    const double half_z = pb->z_total / 2.0;
+    const double lo = pb->lo, hi = pb->hi;
+    const double beta_f = pb->variant == 2 ? pb->beta : 1.0;
+    const double beta_p = pb->variant == 1 ? pb->beta : 1.0;
+    Scoped mem(ctx);
+    void *d_in, *d_arr, *d_illum, *d_phase, *d_best, *d_open, *d_tamp, *d_mt, *d_proj, *d_err, *d_dphi, *d_sums,
+        *d_flags;
+    int st;
+    if ((st = mem.alloc(n * 16, &d_in)) || (st = mem.alloc(n * 16, &d_arr)) || (st = mem.alloc(n * 8, &d_illum)) ||
+        (st = mem.alloc(n * 8, &d_phase)) || (st = mem.alloc(n * 8, &d_best)) || (st = mem.alloc(n, &d_open)) ||
+        (st = mem.alloc(n * 8, &d_tamp)) || (st = mem.alloc(n * 8, &d_mt)) || (st = mem.alloc(n * 16, &d_proj)) ||
+        (st = mem.alloc(n * 8, &d_err)) || (st = mem.alloc(n * 8, &d_dphi)) || (st = mem.alloc(64, &d_sums)) ||
+        (st = mem.alloc(64, &d_flags)))
+        return st;
+    HIP_TRY(ctx, hipMemcpy(d_in, pb->input, n * 16, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(d_tamp, pb->target_amplitude, n * 8, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(d_mt, pb->merit_target, n * 8, hipMemcpyHostToDevice));
+    // aperture indicator (gs.rs:427-433)
+    std::vector<unsigned char> open(n);
+    for (size_t iy = 0; iy < ny; ++iy)
+        for (size_t ix = 0; ix < nx; ++ix) {
+            const double x = ((double)ix - ((double)nx - 1.0) / 2.0) * g.dx;
+            const double y = ((double)iy - ((double)ny - 1.0) / 2.0) * g.dy;
+            open[iy * nx + ix] = contains_h(pb->aperture, x - pb->cx, y - pb->cy) ? 1 : 0;
+        }
+    HIP_TRY(ctx, hipMemcpy(d_open, open.data(), n, hipMemcpyHostToDevice));
+    // illumination arriving at the DOE plane (gs.rs:436-444)
+    const oc_asm_config& c = pb->asm_cfg;
+    {
+        std::vector<Prob> v{make_prob(&g, half_z, ENG_ASM, c.band_limited, c.strict, c.padding, c.max_padding,
+                                      c.medium, nullptr, d_in, d_arr)};
+        if (run_batch(ctx, true, v) != OC_OK) return OC_INV_RENDER;
+    }
+    std::vector<double> arr(2 * n), illum(n);
+    HIP_TRY(ctx, hipMemcpy(arr.data(), d_arr, n * 16, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < n; ++k) illum[k] = std::hypot(arr[2 * k], arr[2 * k + 1]);
+    HIP_TRY(ctx, hipMemcpy(d_illum, illum.data(), n * 8, hipMemcpyHostToDevice));
+    // initial phase (gs.rs:447-475)
+    std::vector<double> phase(n);
+    const double* src = pb->init_kind == 0 ? pb->carrier_values : pb->init_phase;
+    for (size_t k = 0; k < n; ++k) phase[k] = encode_bounds_h(src[k], lo, hi);
+    HIP_TRY(ctx, hipMemcpy(d_phase, phase.data(), n * 8, hipMemcpyHostToDevice));
+    // fused forward / backward half-transforms
+    std::vector<Prob> fw{make_prob(&g, half_z, ENG_ASM, c.band_limited, c.strict, c.padding, c.max_padding, c.medium,
+                                   nullptr, nullptr, d_proj)};
+    fw[0].ops.pre = PRE_POLAR;
+    fw[0].ops.amp = d_illum;
+    fw[0].ops.phase = d_phase;
+    fw[0].ops.open = static_cast<const unsigned char*>(d_open);
+    fw[0].ops.post = POST_GS_TARGET;
+    fw[0].ops.tamp = static_cast<const double*>(d_tamp);
+    fw[0].ops.merit_target = static_cast<const double*>(d_mt);
+    fw[0].ops.beta_f = beta_f;
+    fw[0].ops.err_out = static_cast<double*>(d_err);
+    fw[0].ops.sums = static_cast<double*>(d_sums);
+    std::vector<Prob> bw{make_prob(&g, -half_z, ENG_ASM, c.band_limited, c.strict, c.padding, c.max_padding, c.medium,
+                                   nullptr, d_proj, nullptr)};
+    bw[0].ops.post = POST_GS_PHASE;
+    bw[0].ops.phase_io = d_phase;
+    bw[0].ops.beta_p = beta_p;
+    bw[0].ops.lo = lo;
+    bw[0].ops.hi = hi;
+    bw[0].ops.dphi_out = static_cast<double*>(d_dphi);
+    bw[0].ops.sums = static_cast<double*>(d_sums);
+    prepare(fw[0]);
+    prepare(bw[0]);
+    if (fw[0].status || bw[0].status || fw[0].check_only || bw[0].check_only) return OC_INV_RENDER;
+    Batch bf, bb;
+    st = build_batch(ctx, true, fw, static_cast<int*>(d_flags), bf, true);
+    if (!st) st = build_batch(ctx, true, bw, static_cast<int*>(d_flags) + 1, bb, true);
+    if (st || bf.pd.empty() || bb.pd.empty()) {
+        free_batch(ctx, bf);
+        free_batch(ctx, bb);
+        return st ? st : OC_INV_RENDER;
+    }
+    bool have_best = false;
+    double best_merit = 0.0;
+    uint64_t best_it = 0;
+    std::vector<oc_gs_record> hist;
+    bool converged = false;
+    double hsum[4];
+    int hfl[2];
+    for (uint64_t it = 0; it < pb->max_iterations; ++it) {
+        HIP_TRY(ctx, hipMemsetAsync(d_sums, 0, 64, ctx->stream));
+        HIP_TRY(ctx, ocl::launch_fill_int(static_cast<int*>(d_flags), 0, 2, ctx->stream));
+        if ((st = launch_batch(ctx, bf))) break;
+        HIP_TRY(ctx, hipMemcpyAsync(hsum, d_sums, 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(hfl, d_flags, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (hfl[0]) { st = OC_INV_RENDER; break; }
+        const double amp_mse = hsum[0] / (double)n;
+        const double merit = hsum[1] / (double)n;
+        const bool improved = !have_best || merit < best_merit;
+        if (improved) {
+            have_best = true;
+            best_merit = merit;
+            best_it = it;
+            HIP_TRY(ctx, hipMemcpyAsync(d_best, d_phase, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        hist.push_back({it, merit, best_merit, amp_mse, std::sqrt(amp_mse), 0.0});
+        if ((st = launch_batch(ctx, bb))) break;
+        HIP_TRY(ctx, hipMemcpyAsync(hsum, d_sums, 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(hfl, d_flags, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (hfl[1]) { st = OC_INV_RENDER; break; }
+        hist.back().phase_update_rms = std::sqrt(hsum[2]) / std::sqrt((double)n);
+        if (it + 1 > pb->patience) {
+            const double earlier = hist[hist.size() - 1 - pb->patience].best_merit;
+            const double scale = std::fmax(std::fabs(earlier), 1e-30);
+            if ((earlier - best_merit) / scale < pb->merit_tolerance) {
+                converged = true;
+                break;
+            }
+        }
+    }
+    collect_timing(ctx);
+    free_batch(ctx, bf);
+    free_batch(ctx, bb);
+    if (st) return st;
+    // outputs
+    HIP_TRY(ctx, hipMemcpy(sol->phase, d_best, n * 8, hipMemcpyDeviceToHost));
+    if (sol->target_amplitude_error)
+        HIP_TRY(ctx, hipMemcpy(sol->target_amplitude_error, d_err, n * 8, hipMemcpyDeviceToHost));
+    if (sol->phase_error) HIP_TRY(ctx, hipMemcpy(sol->phase_error, d_dphi, n * 8, hipMemcpyDeviceToHost));
+    if (sol->illumination_amplitude) std::memcpy(sol->illumination_amplitude, illum.data(), n * 8);
+    if (sol->history) std::memcpy(sol->history, hist.data(), hist.size() * sizeof(oc_gs_record));
+    sol->merit = best_merit;
+    sol->best_iteration = best_it;
+    sol->amplitude_mse = hist[best_it].amplitude_mse;
+    sol->converged = converged ? 1 : 0;
+    sol->iterations = hist.size();
+    if (sol->target_intensity) {  // |U_t|^2 of the best phase (same render as its iteration)
+        std::vector<Prob> v{make_prob(&g, half_z, ENG_ASM, c.band_limited, c.strict, c.padding, c.max_padding,
+                                      c.medium, nullptr, nullptr, d_proj)};
+        v[0].ops.pre = PRE_POLAR;
+        v[0].ops.amp = d_illum;
+        v[0].ops.phase = d_best;
+        v[0].ops.open = static_cast<const unsigned char*>(d_open);
+        if (run_batch(ctx, true, v) != OC_OK) return OC_INV_RENDER;
+        HIP_TRY(ctx, hipMemcpy(arr.data(), d_proj, n * 16, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; ++k) {
+            const double a = std::hypot(arr[2 * k], arr[2 * k + 1]);
+            sol->target_intensity[k] = a * a;
+        }
+    }
+    return OC_OK;
+}
+
+int oc_adjoint_asm(oc_context* ctx, const oc_band_geom* g, const double* in, double z, const oc_asm_config* cfg,
+                   double* out, oc_asm_report* rep) {
+    int st = oc_asm_propagate(ctx, g, in, -z, cfg, out, rep);
+    return st == OC_OK ? OC_OK : OC_INV_ADJOINT;
+}
+
+int oc_adjoint_transmission(oc_context* ctx, const oc_band_geom* g, const oc_phase_mask* m, double* t_out) {
+    if (!ctx || !g || !m || !t_out) return OC_ERR_INVALID_ARGUMENT;
+    const size_t n = g->nx * g->ny;
+    std::vector<double> ones(2 * n, 0.0);
+    for (size_t k = 0; k < n; ++k) ones[2 * k] = 1.0;
+    int st = oc_phasemask_apply(ctx, g, ones.data(), m, t_out);
+    return st == OC_OK ? OC_OK : OC_INV_ADJOINT;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Forward tape + reverse sweep on device (adjoint.rs:140-246).  Mask values
+// are device pointers.  `dmdi` (device, f64) seeds w = dmdi * U_det; when
+// null, the MseToTarget seed 2(|U|^2 - target)/n is formed from `target` and
+// the merit is returned in *merit.  grads: n_elements * n (device, zeroed
+// here); may be null for a forward render + merit only.
+int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, const oc_band_geom& g, const void* d_input,
+                 const oc_asm_config& cfg, const double* dmdi, const double* target, double* grads, double* merit) {
+    const size_t n = g.nx * g.ny;
+    size_t ne = 0;
+    for (size_t s = 0; s < nsteps; ++s) ne += steps[s].has_element ? 1 : 0;
+    Scoped mem(ctx);
+    void *cur, *nxt, *ckpt = nullptr, *sums, *flags;
+    int st;
+    if ((st = mem.alloc(n * 16, &cur)) || (st = mem.alloc(n * 16, &nxt)) || (st = mem.alloc(64, &sums)) ||
+        (st = mem.alloc(sizeof(int) * (2 * nsteps + 1), &flags)))
+        return st;
+    if (ne && (st = mem.alloc(ne * n * 16, &ckpt))) return st;
+    // Every step is queued without waiting; the finiteness flags of all
+    // propagations and the merit are read back once at the end.
+    std::vector<std::vector<Prob>> queued;
+    queued.reserve(2 * nsteps);
+    int* dflags = static_cast<int*>(flags);
+    auto propagate = [&](Prob p) -> int {
+        queued.push_back({p});
+        int s2 = launch_problems(ctx, true, queued.back(), dflags + (queued.size() - 1));
+        if (s2 == OC_OK) s2 = queued.back()[0].status;
+        return s2;
+    };
+    HIP_TRY(ctx, hipMemcpyAsync(cur, d_input, n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+    // forward: store each element's incoming field, fuse the mask into the propagation
+    size_t e = 0;
+    for (size_t s = 0; s < nsteps; ++s) {
+        const oc_design_step& stp = steps[s];
+        if (stp.has_element) {
+            HIP_TRY(ctx, hipMemcpyAsync(static_cast<unsigned char*>(ckpt) + e * n * 16, cur, n * 16,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+            ++e;
+        }
+        if (stp.z_after != 0.0) {
+            if (propagate(make_prob(&g, stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
+                                    cfg.max_padding, stp.medium, stp.has_element ? &stp.mask : nullptr, cur,
+                                    nxt)) != OC_OK)
+                return fail(ctx, OC_INV_ADJOINT, "forward propagation failed");
+            std::swap(cur, nxt);
+        } else if (stp.has_element) {
+            const int fs = mask_footprint(g, stp.mask);
+            if (fs) return OC_INV_ADJOINT;
+            MaskDev md;
+            if ((st = mask_tables(ctx, g, stp.mask, stp.mask.values, &md))) return st;
+            HIP_TRY(ctx, ocl::launch_mask_apply_c128(cur, nxt, md, (int)g.nx, (int)g.ny, nullptr, ctx->stream));
+            std::swap(cur, nxt);
+        }
+    }
+    // seed at the detector
+    HIP_TRY(ctx, hipMemsetAsync(sums, 0, 64, ctx->stream));
+    if (dmdi) HIP_TRY(ctx, ocl::launch_seed(true, cur, dmdi, nxt, n, ctx->stream));
+    else HIP_TRY(ctx, ocl::launch_seed_mse(true, cur, target, nxt, n, static_cast<double*>(sums), ctx->stream));
+    std::swap(cur, nxt);
+    auto finish = [&]() -> int {
+        double hs[4];
+        std::vector<int> hf(queued.size() + 1, 0);
+        HIP_TRY(ctx, hipMemcpyAsync(hs, sums, 32, hipMemcpyDeviceToHost, ctx->stream));
+        if (!queued.empty())
+            HIP_TRY(ctx, hipMemcpyAsync(hf.data(), dflags, sizeof(int) * queued.size(), hipMemcpyDeviceToHost,
+                                        ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        collect_timing(ctx);
+        if (merit) *merit = hs[0] / (double)n;
+        for (size_t i = 0; i < queued.size(); ++i)
+            if (hf[i]) return fail(ctx, OC_INV_ADJOINT, "non-finite samples in a propagation");
+        return OC_OK;
+    };
+    if (!grads) return finish();
+    HIP_TRY(ctx, hipMemsetAsync(grads, 0, ne * n * 8, ctx->stream));
+    // reverse sweep
+    e = ne;
+    for (size_t s = nsteps; s-- > 0;) {
+        const oc_design_step& stp = steps[s];
+        if (stp.has_element) --e;
+        void* ck = stp.has_element ? static_cast<unsigned char*>(ckpt) + e * n * 16 : nullptr;
+        double* gr = stp.has_element ? grads + e * n : nullptr;
+        if (stp.z_after != 0.0) {
+            Prob pb = make_prob(&g, -stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding, cfg.max_padding,
+                                stp.medium, nullptr, cur, nxt);
+            if (stp.has_element) {
+                pb.mask = &stp.mask;  // mask tables for the epilogue (no pre-op)
+                pb.ops.pre = PRE_NONE;
+                pb.ops.post = POST_ADJ_GRAD;
+                pb.ops.ckpt = ck;
+                pb.ops.grad = gr;
+            }
+            if (propagate(pb) != OC_OK) return fail(ctx, OC_INV_ADJOINT, "adjoint propagation failed");
+            std::swap(cur, nxt);
+        } else if (stp.has_element) {
+            MaskDev md;
+            if ((st = mask_tables(ctx, g, stp.mask, stp.mask.values, &md))) return st;
+            HIP_TRY(ctx, ocl::launch_adj_elem(true, cur, ck, gr, md, (int)g.nx, (int)g.ny, ctx->stream));
+        }
+    }
+    return finish();
+}
+
+}  // namespace
+
+extern "C" {
+
+int oc_adjoint_push_back(oc_context* ctx, const oc_design_step* steps, size_t nsteps, const oc_band_geom* g,
+                         const double* input, const oc_asm_config* cfg, const double* dmdi, double* dphi_out) {
+    if (!ctx || !g || !cfg || !input || !dmdi || (!steps && nsteps)) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    evict_tables(ctx);
+    const size_t n = g->nx * g->ny;
+    if (n == 0) return OC_INV_ADJOINT;
+    size_t ne = 0;
+    for (size_t s = 0; s < nsteps; ++s) ne += steps[s].has_element ? 1 : 0;
+    Scoped mem(ctx);
+    void *din, *ddm, *dgr = nullptr;
+    int st;
+    if ((st = mem.alloc(n * 16, &din)) || (st = mem.alloc(n * 8, &ddm))) return st;
+    if (ne && (st = mem.alloc(ne * n * 8, &dgr))) return st;
+    HIP_TRY(ctx, hipMemcpy(din, input, n * 16, hipMemcpyHostToDevice));
+    HIP_TRY(ctx, hipMemcpy(ddm, dmdi, n * 8, hipMemcpyHostToDevice));
+    std::vector<oc_design_step> ds(steps, steps + nsteps);
+    for (auto& s : ds) {
+        if (!s.has_element) continue;
+        int vs = oc_phasemask_validate(&s.mask);
+        if (vs) return OC_INV_ADJOINT;
+        void* mv;
+        const size_t mn = s.mask.nx * s.mask.ny;
+        if ((st = mem.alloc(mn * 8, &mv))) return st;
+        HIP_TRY(ctx, hipMemcpy(mv, s.mask.values, mn * 8, hipMemcpyHostToDevice));
+        s.mask.values = static_cast<const double*>(mv);
+    }
+    st = adjoint_impl(ctx, ds.data(), nsteps, *g, din, *cfg, static_cast<double*>(ddm), nullptr,
+                      static_cast<double*>(dgr), nullptr);
+    if (st) return st;
+    if (ne) HIP_TRY(ctx, hipMemcpy(dphi_out, dgr, ne * n * 8, hipMemcpyDeviceToHost));
+    return OC_OK;
+}
+
+int oc_adjoint_grid_gradient_dev_c128(oc_context* ctx, const oc_design_step* steps, size_t nsteps,
+                                      const oc_band_geom* g, const double* input, const oc_asm_config* cfg,
+                                      const double* target, double* dphi_out, double* merit_out) {
+    if (!ctx || !g || !cfg || !input || !target || (!steps && nsteps)) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    evict_tables(ctx);
+    if (g->nx * g->ny == 0) return OC_INV_ADJOINT;
+    double m = 0.0;
+    int st = adjoint_impl(ctx, steps, nsteps, *g, input, *cfg, nullptr, target, dphi_out, &m);
+    if (merit_out) *merit_out = m;
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,285 @@
+// Internal runtime of the propagation library: the context, its buffer
+// and table caches, the problem descriptors and the batch builder shared by
+// the C entry points (api.cpp, inverse.cpp, observe.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/optics_gpu.h"
+#include "host_guards.hpp"
+#include "prop_kernels.hpp"
+
+namespace ocl {
+hipError_t launch_pass_c64(int pass, const ocd::ProbDev* d, int np, int total, int block, size_t lds, int grid,
+                           hipStream_t st);
+hipError_t launch_pass_c128(int pass, const ocd::ProbDev* d, int np, int total, int block, size_t lds, int grid,
+                            hipStream_t st);
+hipError_t set_lds_limits();
+hipError_t launch_mask_apply_c64(const void* in, void* out, const ocd::MaskDev& m, int nx, int ny, int* flag,
+                                 hipStream_t st);
+hipError_t launch_mask_trans(bool dbl, void* out, const ocd::MaskDev& m, int nx, int ny, hipStream_t st);
+hipError_t launch_mask_apply_c128(const void* in, void* out, const ocd::MaskDev& m, int nx, int ny, int* flag,
+                                  hipStream_t st);
+hipError_t launch_check_finite(bool dbl, const void* in, size_t n, int* flag, hipStream_t st);
+hipError_t launch_copy(bool dbl, const void* in, void* out, size_t n, hipStream_t st);
+hipError_t launch_fill_int(int* p, int v, int n, hipStream_t st);
+hipError_t fast_launch_c64(const ocd::FastSel& s, const ocd::ProbDev* d, int np, int total, int grid, hipStream_t st);
+hipError_t fast_launch_c128(const ocd::FastSel& s, const ocd::ProbDev* d, int np, int total, int grid, hipStream_t st);
+void fast_layout_c64(int n, int v, int* G, int* B, int* CB);
+void fast_layout_c128(int n, int v, int* G, int* B, int* CB);
+hipError_t fast_set_lds_limits_c64();
+hipError_t fast_set_lds_limits_c128();
+
+// Number of layout variants of the power-of-two kernels at length n (0: none).
+inline int fast_variants(int n) {
+    switch (n) {
+        case 64: case 128: case 256: case 512: return 1;
+        case 1024: case 2048: case 4096: case 8192: return 4;
+        default: return 0;
+    }
+}
+
+// true when n has a specialised kernel; fills its workgroup layout.
+inline bool fast_supported(bool dbl, int n, int v, int* G, int* B, int* CB) {
+    if (fast_variants(n) == 0) return false;
+    if (dbl) fast_layout_c128(n, v, G, B, CB);
+    else fast_layout_c64(n, v, G, B, CB);
+    return *G > 0;
+}
+hipError_t launch_seed(bool dbl, const void* u, const double* dmdi, void* out, size_t n, hipStream_t st);
+hipError_t launch_adj_elem(bool dbl, void* co, const void* ckpt, double* grad, const ocd::MaskDev& m, int nx, int ny,
+                           hipStream_t st);
+hipError_t launch_block_reorder(bool dbl, const void* src, void* dst, int nx, int ny, int B, hipStream_t st);
+hipError_t launch_seed_mse(bool dbl, const void* u, const double* target, void* out, size_t n, double* sums,
+                           hipStream_t st);
+hipError_t launch_render_obs(bool dbl, const void* u, double* obs, size_t n, int frames, int nr, int cx, int cy,
+                             int cl, double hs, double hl, int* flag, hipStream_t st);
+hipError_t launch_mse(bool dbl, const void* u, const double* target, size_t n, int nr, double* part, double* merit,
+                      hipStream_t st);
+hipError_t launch_adam(double* x, const double* g, double* m, double* v, const double* lo, const double* hi, size_t n,
+                       double* part, double* scale, int has_clip, double cn, double lr, double b1, double b2,
+                       double eps, double bc1, double bc2, hipStream_t st);
+}  // namespace ocl
+
+namespace ocr {
+using namespace ocd;
+
+
+constexpr size_t kLdsBudget = 140 * 1024;  // per workgroup (gfx950: 160 KB per CU)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct TableEntry {
+    void* dev = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace ocr
+
+struct oc_context {
+    int device = 0;
+    int num_cu = 256;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, ocr::DevBuf> bufs;
+    std::map<std::pair<int, int>, void*> twiddles;  // (n, double?) -> table
+    std::map<std::string, ocr::TableEntry> tables;       // per-parameter axis tables
+    size_t tables_bytes = 0;
+    void* pinned = nullptr;  // host staging for descriptors / flags
+    size_t pinned_cap = 0;
+    hipEvent_t pinned_done = nullptr;  // recorded after the last copy out of `pinned`
+    // per-pass timing
+    bool timing = false;
+    struct Pending {
+        int pass;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    double pass_ms[OC_NUM_PASS_KINDS] = {0};
+    uint64_t pass_n[OC_NUM_PASS_KINDS] = {0};
+    bool force_generic = false;  // route every batch through the runtime-plan kernels
+    void* queued = nullptr;      // batches submitted without waiting (QueuedSet, created on first use)
+    std::map<std::pair<int, int>, int> fft_variant;  // (n, double?) -> fast-kernel layout variant
+    // The autotuned layout if there is one, else the layout that timed
+    // fastest for a masked-ASM + Fresnel round on MI355X (oc_autotune_fft
+    // over 16 problems, both precisions; 1024/2048 complex128 are within 1%
+    // across layouts).
+    int variant_for(int n, bool dbl) const {
+        auto it = fft_variant.find(std::make_pair(n, dbl ? 1 : 0));
+        if (it != fft_variant.end()) return it->second;
+        switch (n) {
+            case 1024: case 2048: return 1;
+            case 4096: case 8192: return dbl ? 3 : 2;
+            default: return 0;
+        }
+    }
+};
+
+namespace ocr {
+
+int fail(oc_context* ctx, int code, const std::string& msg);
+hipEvent_t pool_event(oc_context* ctx);
+// Records a timing bracket around `launch` when timing is enabled.
+template <typename F>
+hipError_t timed(oc_context* ctx, int pass, F&& launch) {
+    if (!ctx->timing) return launch();
+    hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
+    if (!a || !b) return launch();
+    (void)hipEventRecord(a, ctx->stream);
+    hipError_t e = launch();
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back({pass, a, b});
+    return e;
+}
+
+void collect_timing(oc_context* ctx);
+#define HIP_TRY(ctx, expr)                                                                          \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? OC_ERR_OUT_OF_MEMORY : OC_ERR_DEVICE,       \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+    } while (0)
+
+int ensure_buf(oc_context* ctx, const std::string& name, size_t bytes, void** out);
+int ensure_pinned(oc_context* ctx, size_t bytes);
+int get_twiddles(oc_context* ctx, int n, bool dbl, const void** out);
+bool make_radices(int n, std::vector<int>& rad, bool& generic);
+int make_plan(oc_context* ctx, int n, bool dbl, FftPlanDev* pl);
+size_t plan_lds_elems(const FftPlanDev& pl);
+// Internal description of one propagation problem.
+struct Prob {
+    oc_band_geom g;
+    double z;
+    int engine;
+    int band_limited, strict;
+    double padding, max_padding, medium;
+    const oc_phase_mask* mask;
+    const void* in;
+    void* out;
+    int status = OC_OK;
+    bool launch = false;     // goes through the FFT passes
+    bool check_only = false; // finiteness check first (z == 0 copy / guard failure)
+    int pending = OC_OK;     // status to report if the input is finite
+    size_t nxp = 0, nyp = 0;
+    oc_asm_report asm_rep{};
+    oc_fresnel_report fr_rep{};
+    oc_rs_report rs_rep{};
+    oc_fraunhofer_config fh_cfg{};
+    oc_fraunhofer_report fh_rep{};
+    OpsDev ops{};
+};
+
+std::string fmtkey(const char* tag, std::initializer_list<double> vals);
+int fresnel_factors(oc_context* ctx, const Prob& p, EngineDev* e);
+int engine_tables(oc_context* ctx, const Prob& p, bool dbl, EngineDev* e);
+int mask_footprint(const oc_band_geom& g, const oc_phase_mask& m);
+int mask_tables(oc_context* ctx, const oc_band_geom& g, const oc_phase_mask& m, const double* values_dev,
+                MaskDev* md);
+int pick_pow2_divisor(size_t n, int cap);
+void prepare(Prob& p);
+struct PassCfg {
+    int block = 64;
+    size_t lds = 0;
+    int total = 0;
+};
+
+// Workgroup shape of one problem: transforms per row-pass workgroup (cnt) and
+// the column block width B of the blocked intermediate.
+struct Layout {
+    int cnt = 1, B = 1;
+    int ccnt = 1;  // columns per column-pass workgroup (divides B)
+    size_t row_bytes = 0, col_bytes = 0;
+    bool ok = false;
+};
+
+Layout choose_layout(const FftPlanDev& prow, const FftPlanDev& pcol, size_t esz, size_t ny, size_t nyp, bool rs);
+struct Batch {
+    bool dbl = false;
+    bool fast = false;  // power-of-two kernels (all problems share P_x = P_y = fast_n)
+    int fast_n = 0;
+    int fast_v = 0;
+    int post_kind = -2;      // post-op shared by every problem (-1: mixed)
+    int col_kind = -2;       // engine shared by every problem (-1: mixed)
+    int pre_kind = -2;       // pre-op shared by every problem (-1: mixed)
+    bool sep = false;        // separable complex64 Fresnel: two passes
+    bool rs = false;
+    std::vector<ProbDev> pd;
+    std::vector<int> idx;  // index into the problem vector of each descriptor
+    void* dprobs = nullptr;
+    PassCfg pc[4];
+    std::vector<void*> owned;  // allocations of a persistent batch
+};
+
+void free_batch(oc_context* ctx, Batch& b);
+// Batches of oc_propagate_batch_submit_dev_* not yet waited for.
+struct Queued {
+    std::vector<Prob> probs;
+    int* dflags = nullptr;
+    oc_prop_desc* descs = nullptr;
+};
+
+struct QueuedSet {
+    std::vector<Queued> batches;
+};
+
+QueuedSet& queued_set(oc_context* ctx);
+void release_queued(oc_context* ctx);
+int batch_alloc(oc_context* ctx, Batch& b, bool persistent, const std::string& name, size_t bytes, void** out);
+int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags, Batch& b, bool persistent,
+                const std::vector<int>* subset = nullptr, const std::string& tag = "");
+constexpr unsigned kAllPasses = 0xF;
+// pass 4 (power-of-two batches with a GS phase post-op): one iteration's
+// inverse row pass fused with the next one's forward row pass
+constexpr int PASS_GS_ROW = 4;
+// pass 5: the same with beta = 1 and 2 pi wide bounds (amplitude replacement;
+// the phases are left to the last iteration's inverse pass)
+constexpr int PASS_GS_AMP = 5;
+// timing slots of the separable Fresnel passes (see OC_NUM_PASS_KINDS)
+constexpr int PASS_T_SEP_COL = 6, PASS_T_SEP_ROW = 7;
+int launch_batch(oc_context* ctx, Batch& b, unsigned mask = kAllPasses);
+void evict_tables(oc_context* ctx);
+int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags);
+int finish_problems(oc_context* ctx, std::vector<Prob>& probs, const int* dflags);
+int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs);
+Prob make_prob(const oc_band_geom* g, double z, int engine, int band_limited, int strict, double padding,
+               double max_padding, double medium, const oc_phase_mask* mask, const void* in, void* out);
+bool geom_ok(const oc_band_geom* g);
+// Host-memory wrapper: H2D, run, D2H.
+int run_host(oc_context* ctx, Prob p, const double* in, double* out, const oc_phase_mask* host_mask, Prob* done);
+
+// Device allocation released at scope exit (after a stream sync).
+struct Scoped {
+    oc_context* ctx;
+    std::vector<void*> ptrs;
+    explicit Scoped(oc_context* c) : ctx(c) {}
+    ~Scoped() {
+        if (!ptrs.empty()) (void)hipStreamSynchronize(ctx->stream);
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+    int alloc(size_t bytes, void** out) {
+        hipError_t e = hipMalloc(out, std::max<size_t>(bytes, 256));
+        if (e != hipSuccess) return fail(ctx, e == hipErrorOutOfMemory ? OC_ERR_OUT_OF_MEMORY : OC_ERR_DEVICE,
+                                         std::string("hipMalloc: ") + hipGetErrorString(e));
+        ptrs.push_back(*out);
+        return OC_OK;
+    }
+};
+
+}  // namespace ocr
