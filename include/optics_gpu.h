@@ -235,6 +235,10 @@ const char* oc_version(void);
    Fresnel form. */
 #define OC_NUM_PASS_KINDS 8
 int oc_context_enable_timing(oc_context* ctx, int enable);
+/* Restricts the timing brackets to the pass kinds whose bit is set in mask
+   (bit k = slot k above; all by default), so that a caller timing one pass
+   records two events per launch of that pass only. */
+int oc_context_set_timing_mask(oc_context* ctx, uint32_t mask);
 int oc_context_pass_timing(oc_context* ctx, double* ms, uint64_t* launches,
                            int reset);
 

@@ -98,6 +98,12 @@ int oc_context_enable_timing(oc_context* ctx, int enable) {
     return OC_OK;
 }
 
+int oc_context_set_timing_mask(oc_context* ctx, uint32_t mask) {
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    ctx->timing_mask = mask;
+    return OC_OK;
+}
+
 int oc_context_pass_timing(oc_context* ctx, double* ms, uint64_t* launches, int reset) {
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));

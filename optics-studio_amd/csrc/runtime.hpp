@@ -105,6 +105,7 @@ struct oc_context {
     hipEvent_t pinned_done = nullptr;  // recorded after the last copy out of `pinned`
     // per-pass timing
     bool timing = false;
+    uint32_t timing_mask = 0xFFFFFFFFu;  // pass kinds that get timing brackets
     struct Pending {
         int pass;
         hipEvent_t a, b;
@@ -138,7 +139,7 @@ hipEvent_t pool_event(oc_context* ctx);
 // Records a timing bracket around `launch` when timing is enabled.
 template <typename F>
 hipError_t timed(oc_context* ctx, int pass, F&& launch) {
-    if (!ctx->timing) return launch();
+    if (!ctx->timing || !((ctx->timing_mask >> pass) & 1u)) return launch();
     hipEvent_t a = pool_event(ctx), b = pool_event(ctx);
     if (!a || !b) return launch();
     (void)hipEventRecord(a, ctx->stream);

@@ -45,6 +45,7 @@ EXPORTS = {
     "oc_version": ([], C.c_char_p),
     "oc_context_enable_timing": ([vp, C.c_int], C.c_int),
     "oc_context_pass_timing": ([vp, P(c_f64), P(c_u64), C.c_int], C.c_int),
+    "oc_context_set_timing_mask": ([vp, C.c_uint32], C.c_int),
     "oc_autotune_fft": ([vp, c_u64, C.c_int, P(C.c_int), P(c_f64)], C.c_int),
     "oc_context_set_fft_variant": ([vp, c_u64, C.c_int, C.c_int], C.c_int),
     "oc_context_fft_variant": ([vp, c_u64, C.c_int], C.c_int),
@@ -158,6 +159,13 @@ class Context:
 
     def enable_timing(self, on=True):
         lib.oc_context_enable_timing(self.h, 1 if on else 0)
+
+    PASS_NAMES = ["row_fwd", "col", "row_inv", "rs_kgen", "gs", "other", "sep_col", "sep_row"]
+
+    def set_timing_passes(self, names=None):
+        """Time only the named pass kinds (None: all)."""
+        mask = 0xFFFFFFFF if names is None else sum(1 << self.PASS_NAMES.index(x) for x in names)
+        return lib.oc_context_set_timing_mask(self.h, mask)
 
     def pass_timing(self, reset=True):
         """{pass: (total_ms, launches)} for row_fwd, col, row_inv, rs_kgen, gs, other, sep_col, sep_row."""
