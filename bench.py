@@ -465,8 +465,23 @@ def main():
     passes = {k: {"avg_ms": timing[k][0] / max(timing[k][1], 1), "launches": timing[k][1],
                   "alg_GBps": alg[k] / (timing[k][0] / max(timing[k][1], 1) * 1e-3) / 1e9}
               for k in alg if timing[k][1]}
+    # reference point for the HBM figures: a plain 1 GiB device copy
+    # (read + write) on the same GPU, timed with events
+    ca = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    cb = torch.empty_like(ca)
+    for _ in range(3):
+        cb.copy_(ca)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(10):
+        cb.copy_(ca)
+    ev1.record()
+    torch.cuda.synchronize()
+    copy_gbps = 2 * (1 << 30) / (ev0.elapsed_time(ev1) / 10 * 1e-3) / 1e9
+    del ca, cb
     step_bytes = sum(alg[k] * timing[k][1] for k in alg) / nbreak + nb * n * n * 8  # + f64 mask lattice per band
     extra = {"passes": passes, "fft_variant": variant, "autotune_ms": variant_ms, "step_alg_GB": step_bytes / 1e9,
+             "device_copy_GBps": copy_gbps,
              "step_alg_GBps": step_bytes / (elapsed / args.steps) / 1e9}
 
     # parity of this run's last band output against the f64 oracle (and the
