@@ -415,13 +415,15 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
     const size_t n = g.nx * g.ny;
     size_t ne = 0;
     for (size_t s = 0; s < nsteps; ++s) ne += steps[s].has_element ? 1 : 0;
-    Scoped mem(ctx);
+    // the context's cached buffers (a gradient per call in an inverse loop
+    // must not pay an allocation and a free)
     void *cur, *nxt, *ckpt = nullptr, *sums, *flags;
     int st;
-    if ((st = mem.alloc(n * 16, &cur)) || (st = mem.alloc(n * 16, &nxt)) || (st = mem.alloc(64, &sums)) ||
-        (st = mem.alloc(sizeof(int) * (2 * nsteps + 1), &flags)))
+    if ((st = ensure_buf(ctx, "adj_cur", n * 16, &cur)) || (st = ensure_buf(ctx, "adj_nxt", n * 16, &nxt)) ||
+        (st = ensure_buf(ctx, "adj_sums", 64, &sums)) ||
+        (st = ensure_buf(ctx, "adj_flags", sizeof(int) * (2 * nsteps + 1), &flags)))
         return st;
-    if (ne && (st = mem.alloc(ne * n * 16, &ckpt))) return st;
+    if (ne && (st = ensure_buf(ctx, "adj_ckpt", ne * n * 16, &ckpt))) return st;
     // Every step is queued without waiting; the finiteness flags of all
     // propagations and the merit are read back once at the end.
     std::vector<std::vector<Prob>> queued;
