@@ -42,7 +42,7 @@ template <typename R, int N, int V = 0>
 struct FastLayout {
     static constexpr int T = N / 16;
     static constexpr int ESZ = (int)sizeof(cpx<R>);
-    static constexpr int BYTES = Pow2Plan<N>::SLOTS * ESZ;
+    static constexpr int BYTES = Pow2Plan<N>::RSTRIDE * ESZ;  // per transform (the row layout's stride)
     static constexpr int LG = 140 * 1024 / BYTES;
     static constexpr int ROWCAP = (V == 1 || V == 3) ? 512 : 1024;
     static constexpr int COLCAP = V == 0 ? 1024 : 512;
@@ -167,7 +167,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
         if (bad) atomicOr(P.flag, 1);
         if (PK < 0 && P.ops.pre == PRE_SEED_MSE)
             accumulate4(acc, 0.0, 0.0, 0.0, P.ops.sums ? P.ops.sums + 3 : nullptr);
-        fft_regs<R, N, G, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         if (rowok) {
             cpx<R>* work = static_cast<cpx<R>*>(P.work);
             const unsigned ny = P.ny, b0 = ((unsigned)tq * ny + iy) * Q + tb;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
             v[m] = {R(0), R(0)};
             if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
-        fft_regs<R, N, G, +1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         const R s = (R)P.scale;
         bool bad = false;
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
@@ -392,7 +392,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(
             v[m] = {R(0), R(0)};
             if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
-        fft_regs<R, N, G, +1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         const R s = (R)P.scale;
         bool bad = false;
         double acc2 = 0.0;
@@ -452,7 +452,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(
         }
         if (bad) atomicOr(P.flag, 2);
         if (FULL) accumulate4(0.0, 0.0, acc2, 0.0, P.ops.sums);
-        fft_regs<R, N, G, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         if (rowok) {
 #pragma unroll
             for (int m = 0; m < 16; ++m) at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q) = v[m];
@@ -478,7 +478,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             v[m] = rowok ? conv_sample<R>(P.eng, t + m * T, y, N, N) : cpx<R>{R(0), R(0)};
-        fft_regs<R, N, G, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         if (rowok) {
             cpx<R>* kb = static_cast<cpx<R>*>(P.kbuf);
             const unsigned k0 = ((unsigned)tq * N + y) * Q + tb;
@@ -598,7 +598,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
             if (rowok && x >= ox && x < ox + nx) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.prow.tw);
-        fft_regs<R, N, G, -1>(v, lds, g, t, tw);
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, tw);
         const cpx<R>* hx = static_cast<const cpx<R>*>(P.eng.hx);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
@@ -606,7 +606,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
             h = at32(hx, t + m * T);
             v[m] = cmul(v[m], h);
         }
-        fft_regs<R, N, G, +1>(v, lds, g, t, tw);
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw);
         const R s = (R)P.scale;
         bool bad = false;
         if (rowok) {
@@ -644,7 +644,7 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
     using R = Real;
     using LY = FastLayout<R, N, V>;
     constexpr int T = N / 16;
-    const size_t row_lds = (size_t)LY::G * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
+    const size_t row_lds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
     const size_t col_lds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
     const dim3 brow(LY::G * T), bcol(LY::CB * T), g(grid);
     if (s.sep) {  // separable complex64 Fresnel (pass 1: columns, pass 2: rows)

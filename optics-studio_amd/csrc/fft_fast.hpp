@@ -28,6 +28,11 @@ struct Pow2Plan {
     static constexpr int T = N / 16;
     static constexpr int NSTAGES = NS16 + (L > 1 ? 1 : 0);
     static constexpr int SLOTS = N + N / 16;  // LDS elements per transform
+    // Row-major exchange layout (transform g at g * RSTRIDE): the stride is
+    // SLOTS rounded up to 8 mod 32 elements, which puts consecutive
+    // transforms 64 bytes (complex64) apart in the bank space, so the row
+    // kernels' lanes (running along the transform first) hit distinct banks.
+    static constexpr int RSTRIDE = SLOTS + (40 - SLOTS % 32) % 32;
     static_assert((1 << LOG) == N && N >= 32, "power of two >= 32");
 };
 
@@ -40,23 +45,28 @@ __device__ __forceinline__ int fslot(int idx) { return idx + (idx >> 4); }
 // multiple of 16, slot(b + r Ns) = slot(b) + r (Ns + Ns/16); likewise
 // slot(t + m T) = slot(t) + m (T + T/16) when 16 | T.  The accesses then use
 // immediate offsets from a single address register instead of sixteen.
-template <typename R, int N, int G>
+// RM: transform g's slots are contiguous at g * RSTRIDE (row kernels, whose
+// lanes run along a transform first); else interleaved, slot * G + g
+// (column kernels, whose lanes run across transforms first).
+template <typename R, int N, int G, bool RM = false>
 __device__ __forceinline__ void fft_exchange16(cpx<R> (&v)[16], cpx<R>* __restrict__ lds, int g, int t, int Ns) {
     constexpr int T = N / 16;
+    constexpr int SG = RM ? 1 : G;                              // element step of one slot
+    cpx<R>* __restrict__ own = lds + (RM ? g * Pow2Plan<N>::RSTRIDE : g);  // this transform's slot 0
     const int k = t & (Ns - 1);
     const int base = (t - k) * 16 + k;
     const int wstep = Ns == 1 ? 1 : Ns + Ns / 16;
-    cpx<R>* __restrict__ wp = lds + (Ns == 1 ? 17 * t : fslot(base)) * G + g;
+    cpx<R>* __restrict__ wp = own + (Ns == 1 ? 17 * t : fslot(base)) * SG;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) wp[r * wstep * G] = v[r];
+    for (int r = 0; r < 16; ++r) wp[r * wstep * SG] = v[r];
     __syncthreads();
     if constexpr (T % 16 == 0) {
-        const cpx<R>* __restrict__ rp = lds + fslot(t) * G + g;
+        const cpx<R>* __restrict__ rp = own + fslot(t) * SG;
 #pragma unroll
-        for (int m = 0; m < 16; ++m) v[m] = rp[m * (T + T / 16) * G];
+        for (int m = 0; m < 16; ++m) v[m] = rp[m * (T + T / 16) * SG];
     } else {
 #pragma unroll
-        for (int m = 0; m < 16; ++m) v[m] = lds[fslot(t + m * T) * G + g];
+        for (int m = 0; m < 16; ++m) v[m] = own[fslot(t + m * T) * SG];
     }
     __syncthreads();
 }
@@ -71,7 +81,7 @@ __device__ __forceinline__ void dft_small(cpx<R>* w, int L) {
 // G transforms of length N in place in registers (see header).  `active`
 // threads participate in the arithmetic; every thread of the block must call
 // this (barriers).
-template <typename R, int N, int G, int DIR>
+template <typename R, int N, int G, int DIR, bool RM = false>
 __device__ __forceinline__ void fft_regs(cpx<R> (&v)[16], cpx<R>* __restrict__ lds, int g, int t,
                                          const cpx<R>* __restrict__ tw) {
     using PL = Pow2Plan<N>;
@@ -104,7 +114,7 @@ __device__ __forceinline__ void fft_regs(cpx<R> (&v)[16], cpx<R>* __restrict__ l
             v[15] = cmul(v[15], cmul(w7, w8));
         }
         dft16<DIR>(v);
-        if (s + 1 < PL::NSTAGES) fft_exchange16<R, N, G>(v, lds, g, t, Ns);
+        if (s + 1 < PL::NSTAGES) fft_exchange16<R, N, G, RM>(v, lds, g, t, Ns);
     }
     if constexpr (PL::L > 1) {
         constexpr int L = PL::L;
