@@ -78,6 +78,7 @@ void oc_context_destroy(oc_context* ctx) {
 }
 
 int oc_context_set_stream(oc_context* ctx, void* stream) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own;
@@ -87,24 +88,28 @@ int oc_context_set_stream(oc_context* ctx, void* stream) {
 void* oc_context_stream(oc_context* ctx) { return ctx ? ctx->stream : nullptr; }
 
 int oc_context_synchronize(oc_context* ctx) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     return OC_OK;
 }
 
 int oc_context_enable_timing(oc_context* ctx, int enable) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     ctx->timing = enable != 0;
     return OC_OK;
 }
 
 int oc_context_set_timing_mask(oc_context* ctx, uint32_t mask) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     ctx->timing_mask = mask;
     return OC_OK;
 }
 
 int oc_context_pass_timing(oc_context* ctx, double* ms, uint64_t* launches, int reset) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
     collect_timing(ctx);
@@ -146,6 +151,7 @@ static void fill_asm_report(const Prob& p, oc_asm_report* rep) {
 
 int oc_asm_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z, const oc_asm_config* cfg,
                      double* out, oc_asm_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !cfg || (!in && g->nx * g->ny) || (!out && g->nx * g->ny)) return OC_ERR_INVALID_ARGUMENT;
     if (rep) std::memset(rep, 0, sizeof *rep);
     (void)hipSetDevice(ctx->device);
@@ -171,15 +177,18 @@ static int dev_asm(oc_context* ctx, bool dbl, const oc_band_geom* g, const void*
 
 int oc_asm_propagate_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
                               const oc_asm_config* cfg, const oc_phase_mask* mask, double* out, oc_asm_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_asm(ctx, true, g, in, z, cfg, mask, out, rep);
 }
 int oc_asm_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
                              const oc_asm_config* cfg, const oc_phase_mask* mask, float* out, oc_asm_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_asm(ctx, false, g, in, z, cfg, mask, out, rep);
 }
 
 int oc_fresnel_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
                          const oc_fresnel_config* cfg, double* out, oc_fresnel_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
     if (rep) std::memset(rep, 0, sizeof *rep);
     (void)hipSetDevice(ctx->device);
@@ -206,16 +215,19 @@ static int dev_fresnel(oc_context* ctx, bool dbl, const oc_band_geom* g, const v
 int oc_fresnel_propagate_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
                                   const oc_fresnel_config* cfg, const oc_phase_mask* mask, double* out,
                                   oc_fresnel_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_fresnel(ctx, true, g, in, z, cfg, mask, out, rep);
 }
 int oc_fresnel_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
                                  const oc_fresnel_config* cfg, const oc_phase_mask* mask, float* out,
                                  oc_fresnel_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_fresnel(ctx, false, g, in, z, cfg, mask, out, rep);
 }
 
 int oc_fresnel_propagate_convolution(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
                                      const oc_fresnel_config* cfg, double* out, oc_fresnel_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
     if (rep) std::memset(rep, 0, sizeof *rep);
     (void)hipSetDevice(ctx->device);
@@ -241,10 +253,12 @@ static int dev_fresnel_conv(oc_context* ctx, bool dbl, const oc_band_geom* g, co
 
 int oc_fresnel_propagate_convolution_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
                                               const oc_fresnel_config* cfg, double* out, oc_fresnel_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_fresnel_conv(ctx, true, g, in, z, cfg, out, rep);
 }
 int oc_fresnel_propagate_convolution_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
                                              const oc_fresnel_config* cfg, float* out, oc_fresnel_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_fresnel_conv(ctx, false, g, in, z, cfg, out, rep);
 }
 
@@ -257,6 +271,7 @@ int oc_fraunhofer_regime_report(const oc_band_geom* g, double z, const oc_fraunh
 
 int oc_fraunhofer_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
                             const oc_fraunhofer_config* cfg, double* out, oc_fraunhofer_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
     if (rep) std::memset(rep, 0, sizeof *rep);
     (void)hipSetDevice(ctx->device);
@@ -282,15 +297,18 @@ static int dev_fraunhofer(oc_context* ctx, bool dbl, const oc_band_geom* g, cons
 
 int oc_fraunhofer_propagate_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
                                      const oc_fraunhofer_config* cfg, double* out, oc_fraunhofer_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_fraunhofer(ctx, true, g, in, z, cfg, out, rep);
 }
 int oc_fraunhofer_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
                                     const oc_fraunhofer_config* cfg, float* out, oc_fraunhofer_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_fraunhofer(ctx, false, g, in, z, cfg, out, rep);
 }
 
 int oc_rs_propagate(oc_context* ctx, const oc_band_geom* g, const double* in, double z, const oc_rs_config* cfg,
                     double* out, oc_rs_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !cfg) return OC_ERR_INVALID_ARGUMENT;
     if (rep) std::memset(rep, 0, sizeof *rep);
     (void)hipSetDevice(ctx->device);
@@ -314,10 +332,12 @@ static int dev_rs(oc_context* ctx, bool dbl, const oc_band_geom* g, const void* 
 
 int oc_rs_propagate_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, double z,
                              const oc_rs_config* cfg, double* out, oc_rs_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_rs(ctx, true, g, in, z, cfg, out, rep);
 }
 int oc_rs_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, double z,
                             const oc_rs_config* cfg, float* out, oc_rs_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     return dev_rs(ctx, false, g, in, z, cfg, out, rep);
 }
 
@@ -372,16 +392,25 @@ static int submit(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
     return first;
 }
 
-int oc_propagate_batch_dev_c64(oc_context* ctx, oc_prop_desc* d, size_t count) { return batch(ctx, false, d, count); }
-int oc_propagate_batch_dev_c128(oc_context* ctx, oc_prop_desc* d, size_t count) { return batch(ctx, true, d, count); }
+int oc_propagate_batch_dev_c64(oc_context* ctx, oc_prop_desc* d, size_t count) {
+    OC_CONTEXT_LOCK(ctx);
+    return batch(ctx, false, d, count);
+}
+int oc_propagate_batch_dev_c128(oc_context* ctx, oc_prop_desc* d, size_t count) {
+    OC_CONTEXT_LOCK(ctx);
+    return batch(ctx, true, d, count);
+}
 int oc_propagate_batch_submit_dev_c64(oc_context* ctx, oc_prop_desc* d, size_t count) {
+    OC_CONTEXT_LOCK(ctx);
     return submit(ctx, false, d, count);
 }
 int oc_propagate_batch_submit_dev_c128(oc_context* ctx, oc_prop_desc* d, size_t count) {
+    OC_CONTEXT_LOCK(ctx);
     return submit(ctx, true, d, count);
 }
 
 int oc_context_wait(oc_context* ctx) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     if (!ctx->queued) return OC_OK;
@@ -439,14 +468,17 @@ static int mask_dev(oc_context* ctx, bool dbl, const oc_band_geom* g, const void
 
 int oc_phasemask_apply_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* in, const oc_phase_mask* m,
                                 double* out) {
+    OC_CONTEXT_LOCK(ctx);
     return mask_dev(ctx, true, g, in, m, out);
 }
 int oc_phasemask_apply_dev_c64(oc_context* ctx, const oc_band_geom* g, const float* in, const oc_phase_mask* m,
                                float* out) {
+    OC_CONTEXT_LOCK(ctx);
     return mask_dev(ctx, false, g, in, m, out);
 }
 
 int oc_phasemask_apply(oc_context* ctx, const oc_band_geom* g, const double* in, const oc_phase_mask* m, double* out) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !m) return OC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     int st = mask_footprint(*g, *m);
@@ -480,17 +512,20 @@ int oc_phasemask_apply(oc_context* ctx, const oc_band_geom* g, const double* in,
 extern "C" {
 
 int oc_context_set_fft_variant(oc_context* ctx, uint64_t n, int dbl, int variant) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || variant < 0 || variant >= std::max(1, ocl::fast_variants((int)n))) return OC_ERR_INVALID_ARGUMENT;
     ctx->fft_variant[std::make_pair((int)n, dbl ? 1 : 0)] = variant;
     return OC_OK;
 }
 
 int oc_context_fft_variant(oc_context* ctx, uint64_t n, int dbl) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || ocl::fast_variants((int)n) == 0) return -1;
     return ctx->variant_for((int)n, dbl != 0);
 }
 
 int oc_context_force_generic(oc_context* ctx, int enable) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     ctx->force_generic = enable != 0;
     return OC_OK;
@@ -500,6 +535,7 @@ int oc_context_force_generic(oc_context* ctx, int enable) {
 // a synthetic round of masked ASM then Fresnel propagations (up to 16 fields
 // of n/2 x n/2, padding 2) and keeps the fastest.
 int oc_autotune_fft(oc_context* ctx, uint64_t n, int dbl, int* variant_out, double* ms_out) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     const int nv = ocl::fast_variants((int)n);

@@ -210,16 +210,19 @@ extern "C" {
 int oc_gs_fft_run_dev_c64(oc_context* ctx, uint64_t n, uint64_t restarts, const float* source_amp,
                           const float* target_amp, float* phases, uint64_t iterations, uint64_t graph_chunk,
                           double* err_out) {
+    OC_CONTEXT_LOCK(ctx);
     return gs_fft_impl(ctx, false, n, restarts, source_amp, target_amp, phases, iterations, graph_chunk, err_out);
 }
 int oc_gs_fft_run_dev_c128(oc_context* ctx, uint64_t n, uint64_t restarts, const double* source_amp,
                            const double* target_amp, double* phases, uint64_t iterations, uint64_t graph_chunk,
                            double* err_out) {
+    OC_CONTEXT_LOCK(ctx);
     return gs_fft_impl(ctx, true, n, restarts, source_amp, target_amp, phases, iterations, graph_chunk, err_out);
 }
 
 // gs.rs:347-597
 int oc_gs_run(oc_context* ctx, const oc_gs_problem* pb, oc_gs_solution* sol) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !pb || !sol) return OC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     evict_tables(ctx);
@@ -388,11 +391,13 @@ int oc_gs_run(oc_context* ctx, const oc_gs_problem* pb, oc_gs_solution* sol) {
 
 int oc_adjoint_asm(oc_context* ctx, const oc_band_geom* g, const double* in, double z, const oc_asm_config* cfg,
                    double* out, oc_asm_report* rep) {
+    OC_CONTEXT_LOCK(ctx);
     int st = oc_asm_propagate(ctx, g, in, -z, cfg, out, rep);
     return st == OC_OK ? OC_OK : OC_INV_ADJOINT;
 }
 
 int oc_adjoint_transmission(oc_context* ctx, const oc_band_geom* g, const oc_phase_mask* m, double* t_out) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !m || !t_out) return OC_ERR_INVALID_ARGUMENT;
     const size_t n = g->nx * g->ny;
     std::vector<double> ones(2 * n, 0.0);
@@ -515,6 +520,7 @@ extern "C" {
 
 int oc_adjoint_push_back(oc_context* ctx, const oc_design_step* steps, size_t nsteps, const oc_band_geom* g,
                          const double* input, const oc_asm_config* cfg, const double* dmdi, double* dphi_out) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !cfg || !input || !dmdi || (!steps && nsteps)) return OC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     evict_tables(ctx);
@@ -550,6 +556,7 @@ int oc_adjoint_push_back(oc_context* ctx, const oc_design_step* steps, size_t ns
 int oc_adjoint_grid_gradient_dev_c128(oc_context* ctx, const oc_design_step* steps, size_t nsteps,
                                       const oc_band_geom* g, const double* input, const oc_asm_config* cfg,
                                       const double* target, double* dphi_out, double* merit_out) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !cfg || !input || !target || (!steps && nsteps)) return OC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     evict_tables(ctx);

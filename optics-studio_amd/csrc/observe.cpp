@@ -247,6 +247,7 @@ int oc_fd_gradient_dev_c128(oc_context* ctx, const oc_band_geom* g, const oc_des
                             size_t dim, const double* up, const double* dn, const double* lo, const double* hi,
                             const oc_asm_config* cfg, const void* input, const double* target, double* grad_out,
                             double* merits_out, size_t* evaluations_out) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !cfg || (!designs && nsteps && dim) || (dim && (!up || !dn || !lo || !hi || !grad_out)) ||
         !input || !target)
         return OC_ERR_INVALID_ARGUMENT;
@@ -260,6 +261,7 @@ int oc_fd_gradient_dev_c64(oc_context* ctx, const oc_band_geom* g, const oc_desi
                            size_t dim, const double* up, const double* dn, const double* lo, const double* hi,
                            const oc_asm_config* cfg, const void* input, const double* target, double* grad_out,
                            double* merits_out, size_t* evaluations_out) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !cfg || (!designs && nsteps && dim) || (dim && (!up || !dn || !lo || !hi || !grad_out)) ||
         !input || !target)
         return OC_ERR_INVALID_ARGUMENT;
@@ -271,6 +273,7 @@ int oc_fd_gradient_dev_c64(oc_context* ctx, const oc_band_geom* g, const oc_desi
 
 int oc_adam_step_dev(oc_context* ctx, size_t dim, double* x, const double* g, double* m, double* v, const double* lo,
                      const double* hi, uint64_t t, const oc_adam_config* cfg) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !cfg || t == 0 || (dim && (!x || !g || !m || !v || !lo || !hi))) return OC_ERR_INVALID_ARGUMENT;
     // AdamConfig::validate (fd.rs:95-118)
     if (!std::isfinite(cfg->lr_frac) || cfg->lr_frac <= 0.0) return fail(ctx, OC_INV_RENDER, "invalid Adam lr_frac");
@@ -304,6 +307,7 @@ static size_t render_counts(const oc_render_config* rc, size_t nb, size_t ns, si
 int oc_render_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* bands, size_t nb, size_t ns,
                        const oc_render_config* rc, const oc_design_step* designs, size_t nsteps,
                        const oc_asm_config* cfg, const void* inputs, double* obs) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !rc || !cfg) return OC_ERR_INVALID_ARGUMENT;
     int st = render_check(ctx, *g, nb, ns, *rc);
     if (st) return st;
@@ -316,6 +320,7 @@ int oc_render_dev_c128(oc_context* ctx, const oc_band_geom* g, const double* ban
 int oc_render_dev_c64(oc_context* ctx, const oc_band_geom* g, const double* bands, size_t nb, size_t ns,
                       const oc_render_config* rc, const oc_design_step* designs, size_t nsteps,
                       const oc_asm_config* cfg, const void* inputs, double* obs) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !rc || !cfg) return OC_ERR_INVALID_ARGUMENT;
     int st = render_check(ctx, *g, nb, ns, *rc);
     if (st) return st;
@@ -328,6 +333,7 @@ int oc_render_dev_c64(oc_context* ctx, const oc_band_geom* g, const double* band
 int oc_render(oc_context* ctx, const oc_band_geom* g, const double* bands, size_t nb, size_t ns,
               const oc_render_config* rc, const oc_design_step* designs, size_t nsteps, const oc_asm_config* cfg,
               const double* inputs, double* obs) {
+    OC_CONTEXT_LOCK(ctx);
     if (!ctx || !g || !rc || !cfg) return OC_ERR_INVALID_ARGUMENT;
     int st = render_check(ctx, *g, nb, ns, *rc);
     if (st) return st;

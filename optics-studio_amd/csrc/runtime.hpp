@@ -12,6 +12,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -116,6 +117,7 @@ struct oc_context {
     uint64_t pass_n[OC_NUM_PASS_KINDS] = {0};
     bool force_generic = false;  // route every batch through the runtime-plan kernels
     void* queued = nullptr;      // batches submitted without waiting (QueuedSet, created on first use)
+    std::recursive_mutex mu;     // held by every entry point for the duration of its call
     std::map<std::pair<int, int>, int> fft_variant;  // (n, double?) -> fast-kernel layout variant
     // The autotuned layout if there is one, else the layout that timed
     // fastest for a masked-ASM + Fresnel round on MI355X (oc_autotune_fft
@@ -131,6 +133,13 @@ struct oc_context {
         }
     }
 };
+
+// Serialises the calls made on one context (its caches, buffers and stream
+// order); calls on different contexts proceed concurrently.  Recursive, so
+// an entry point may call another.
+#define OC_CONTEXT_LOCK(ctx)                                                                           \
+    std::unique_lock<std::recursive_mutex> oc_context_lock_ =                                        \
+        (ctx) ? std::unique_lock<std::recursive_mutex>((ctx)->mu) : std::unique_lock<std::recursive_mutex>()
 
 namespace ocr {
 

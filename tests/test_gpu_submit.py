@@ -97,3 +97,46 @@ def test_submit_statuses(G):
     assert st == T.CODE["OC_INVALID_DISTANCE"]  # the first non-OK status in submit order
     assert arr[0].status == 0 and arr2[0].status == 0 and arr2[1].status == T.CODE["OC_NON_FINITE_SAMPLE"]
     ctx.close()
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_threads_on_contexts(G, shared):
+    """Calls from several host threads, each on its own context or all on
+    one shared context (serialised by the context's lock), give the results
+    of the same calls made one after another."""
+    import threading
+    import torch
+    n, d = 256, 2e-6
+    lams = [450e-9, 500e-9, 550e-9, 600e-9]
+    u = [torch.as_tensor(field(n, 50 + b)).to(torch.complex64).cuda() for b in range(len(lams))]
+    ref = [torch.empty_like(x) for x in u]
+    got = [torch.empty_like(x) for x in u]
+    seq = G.Context(0)
+    for b, lam in enumerate(lams):
+        assert G.propagate_batch_dev(descs(T.ENGINE_ASM, [lam], n, d, [u[b]], [ref[b]], 1e-3), ctx=seq) == 0
+    torch.cuda.synchronize()
+    one = G.Context(0) if shared else None
+    ctxs = [one if shared else G.Context(0) for _ in lams]
+    errs = []
+
+    def work(b):
+        try:
+            for _ in range(5):
+                arr = descs(T.ENGINE_ASM, [lams[b]], n, d, [u[b]], [got[b]], 1e-3)
+                st = G.lib.oc_propagate_batch_dev_c64(ctxs[b].h, arr, len(arr))
+                if st:
+                    errs.append(st)
+                G.lib.oc_context_synchronize(ctxs[b].h)
+        except Exception as e:  # noqa: BLE001
+            errs.append(repr(e))
+
+    say("threads on", "one shared context" if shared else "own contexts")
+    th = [threading.Thread(target=work, args=(b,)) for b in range(len(lams))]
+    for t_ in th:
+        t_.start()
+    for t_ in th:
+        t_.join()
+    assert not errs, errs
+    torch.cuda.synchronize()
+    for b in range(len(lams)):
+        assert torch.equal(got[b], ref[b])
