@@ -670,6 +670,19 @@ int oc_adam_step_dev(oc_context* ctx, size_t dim, double* x, const double* g,
                      double* m, double* v, const double* lo, const double* hi,
                      uint64_t t, const oc_adam_config* cfg);
 
+/* analysis::mtf (src/analysis.rs:133-150) for a batch of PSFs: the fy = 0
+   row of the 2-D DFT of each intensity, |F(k, 0)| / max(|F(0, 0)|,
+   DBL_MIN) capped at 1, for k = 0..nx/2, at frequencies k / (nx dx).  That
+   row is the 1-D DFT of the column sums, which is what the device computes
+   (the sums in f64 over y, then a direct DFT with an exact twiddle index).
+   psfs: [count][ny][nx] f64; values: [count][nx/2 + 1]; freqs (may be
+   NULL): [nx/2 + 1], host.  Errors: OC_EMPTY_GRID, OC_INVALID_GRID_PITCH
+   (Psf::new, field.rs), OC_NON_FINITE_SAMPLE for a non-finite sample. */
+int oc_mtf(oc_context* ctx, const double* psfs, size_t count, size_t nx,
+           size_t ny, double dx, double* values, double* freqs);
+int oc_mtf_dev(oc_context* ctx, const double* psfs, size_t count, size_t nx,
+               size_t ny, double dx, double* values, double* freqs);
+
 #ifdef __cplusplus
 }
 #endif

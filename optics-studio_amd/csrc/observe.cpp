@@ -241,6 +241,43 @@ int fd_impl(oc_context* ctx, bool dbl, const oc_band_geom& g, const oc_design_st
 
 }  // namespace
 
+namespace {
+
+// analysis::mtf (analysis.rs:133-150) over `count` device-resident PSFs.
+int mtf_impl(oc_context* ctx, const double* psfs, size_t count, size_t nx, size_t ny, double dx, double* values,
+             double* freqs) {
+    if (nx == 0 || ny == 0) return fail(ctx, OC_EMPTY_GRID, "empty PSF grid");
+    if (!std::isfinite(dx) || dx <= 0.0) return fail(ctx, OC_INVALID_GRID_PITCH, "PSF pitch must be positive");
+    const size_t half = nx / 2;
+    if (freqs)
+        for (size_t k = 0; k <= half; ++k) freqs[k] = (double)k / ((double)nx * dx);
+    if (count == 0) return OC_OK;
+    // e^{-2 pi i j / nx}, j = 0..nx-1, from long double
+    std::vector<double> tw(2 * nx);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (size_t j = 0; j < nx; ++j) {
+        const long double a = two_pi * (long double)j / (long double)nx;
+        tw[2 * j] = (double)cosl(a);
+        tw[2 * j + 1] = (double)-sinl(a);
+    }
+    void *dtw, *sums, *fl;
+    int st;
+    if ((st = ensure_buf(ctx, "mtf_tw", 16 * nx, &dtw)) || (st = ensure_buf(ctx, "mtf_sums", 8 * nx * count, &sums)) ||
+        (st = ensure_buf(ctx, "mtf_flag", sizeof(int), &fl)))
+        return st;
+    HIP_TRY(ctx, hipMemcpyAsync(dtw, tw.data(), 16 * nx, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, ocl::launch_fill_int(static_cast<int*>(fl), 0, 1, ctx->stream));
+    HIP_TRY(ctx, ocl::launch_mtf(psfs, nx, ny, count, static_cast<double*>(sums), dtw, values, static_cast<int*>(fl),
+                                 ctx->stream));
+    int hf = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(&hf, fl, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (hf) return fail(ctx, OC_NON_FINITE_SAMPLE, "non-finite PSF sample");
+    return OC_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int oc_fd_gradient_dev_c128(oc_context* ctx, const oc_band_geom* g, const oc_design_step* designs, size_t nsteps,
@@ -365,6 +402,31 @@ int oc_render(oc_context* ctx, const oc_band_geom* g, const double* bands, size_
     st = render_impl(ctx, true, *g, bands, nb, ns, *rc, ds.data(), nsteps, *cfg, din, static_cast<double*>(dobs));
     if (st) return st;
     HIP_TRY(ctx, hipMemcpy(obs, dobs, nch * nb * ns * n * 8, hipMemcpyDeviceToHost));
+    return OC_OK;
+}
+
+int oc_mtf_dev(oc_context* ctx, const double* psfs, size_t count, size_t nx, size_t ny, double dx, double* values,
+               double* freqs) {
+    OC_CONTEXT_LOCK(ctx);
+    if (!ctx || (count && (!psfs || !values))) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    return mtf_impl(ctx, psfs, count, nx, ny, dx, values, freqs);
+}
+
+int oc_mtf(oc_context* ctx, const double* psfs, size_t count, size_t nx, size_t ny, double dx, double* values,
+           double* freqs) {
+    OC_CONTEXT_LOCK(ctx);
+    if (!ctx || (count && (!psfs || !values))) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (nx == 0 || ny == 0 || count == 0) return mtf_impl(ctx, psfs, count, nx, ny, dx, values, freqs);
+    Scoped mem(ctx);
+    void *dp, *dv;
+    int st;
+    if ((st = mem.alloc(8 * nx * ny * count, &dp)) || (st = mem.alloc(8 * (nx / 2 + 1) * count, &dv))) return st;
+    HIP_TRY(ctx, hipMemcpy(dp, psfs, 8 * nx * ny * count, hipMemcpyHostToDevice));
+    st = mtf_impl(ctx, static_cast<const double*>(dp), count, nx, ny, dx, static_cast<double*>(dv), freqs);
+    if (st) return st;
+    HIP_TRY(ctx, hipMemcpy(values, dv, 8 * (nx / 2 + 1) * count, hipMemcpyDeviceToHost));
     return OC_OK;
 }
 

@@ -448,6 +448,49 @@ __global__ void k_adam(double* __restrict__ x, const double* __restrict__ g, dou
     }
 }
 
+// analysis::mtf (analysis.rs:133-150).  k_colsum: the column sums of each
+// PSF (f64, in order over y), flagging non-finite samples; k_mtf_dft: the
+// 1-D DFT of those sums at bins 0..nx/2, e^{-2 pi i k x / nx} from a table
+// indexed by (k x) mod nx, then |X_k| / max(|X_0|, DBL_MIN) capped at 1.
+__global__ void k_colsum(const double* __restrict__ psf, size_t nx, size_t ny, size_t count,
+                         double* __restrict__ sums, int* flag) {
+    const size_t total = nx * count;
+    bool bad = false;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = i / nx, x = i - c * nx;
+        const double* p = psf + c * nx * ny + x;
+        double s = 0.0;
+        for (size_t y = 0; y < ny; ++y) {
+            const double v = p[y * nx];
+            bad |= !isfinite(v);
+            s += v;
+        }
+        sums[i] = s;
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
+__global__ void k_mtf_dft(const double* __restrict__ sums, const double2* __restrict__ tw, size_t nx, size_t count,
+                          double* __restrict__ values) {
+    const size_t nb = nx / 2 + 1, total = nb * count;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t c = i / nb, k = i - c * nb;
+        const double* p = sums + c * nx;
+        double re = 0.0, im = 0.0, d0 = 0.0;
+        size_t e = 0;  // (k x) mod nx
+        for (size_t x = 0; x < nx; ++x) {
+            const double2 w = tw[e];
+            re = fma(p[x], w.x, re);
+            im = fma(p[x], w.y, im);
+            d0 += p[x];
+            e += k;
+            if (e >= nx) e -= nx;
+        }
+        const double dc = fmax(fabs(d0), 2.2250738585072014e-308);
+        values[i] = fmin(hypot(re, im) / dc, 1.0);
+    }
+}
+
 template <typename R>
 __global__ void k_seed_mse(const cpx<R>* __restrict__ u, const double* __restrict__ target,
                            cpx<R>* __restrict__ out, size_t n, double nf, double* sums) {
@@ -667,6 +710,17 @@ hipError_t launch_adam(double* x, const double* g, double* m, double* v, const d
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(k_adam, dim3(grid), dim3(256), 0, st, x, g, m, v, lo, hi, n, scale, lr, b1, b2, eps, bc1,
                        bc2);
+    return hipGetLastError();
+}
+
+hipError_t launch_mtf(const double* psf, size_t nx, size_t ny, size_t count, double* sums, const void* tw,
+                      double* values, int* flag, hipStream_t st) {
+    int grid = (int)std::min<size_t>((nx * count + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_colsum, dim3(grid), dim3(256), 0, st, psf, nx, ny, count, sums, flag);
+    const size_t nb = (nx / 2 + 1) * count;
+    grid = (int)std::min<size_t>((nb + 63) / 64, 16384);
+    hipLaunchKernelGGL(k_mtf_dft, dim3(grid), dim3(64), 0, st, sums, static_cast<const double2*>(tw), nx, count,
+                       values);
     return hipGetLastError();
 }
 

@@ -68,6 +68,8 @@ hipError_t launch_render_obs(bool dbl, const void* u, double* obs, size_t n, int
                              int cl, double hs, double hl, int* flag, hipStream_t st);
 hipError_t launch_mse(bool dbl, const void* u, const double* target, size_t n, int nr, double* part, double* merit,
                       hipStream_t st);
+hipError_t launch_mtf(const double* psf, size_t nx, size_t ny, size_t count, double* sums, const void* tw,
+                      double* values, int* flag, hipStream_t st);
 hipError_t launch_adam(double* x, const double* g, double* m, double* v, const double* lo, const double* hi, size_t n,
                        double* part, double* scale, int has_clip, double cn, double lr, double b1, double b2,
                        double eps, double bc1, double bc2, hipStream_t st);

@@ -103,6 +103,8 @@ EXPORTS = {
     "oc_propagate_batch_submit_dev_c64": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_batch_submit_dev_c128": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_context_wait": ([vp], C.c_int),
+    "oc_mtf": ([vp, vp, c_sz, c_sz, c_sz, c_f64, vp, vp], C.c_int),
+    "oc_mtf_dev": ([vp, vp, c_sz, c_sz, c_sz, c_f64, vp, vp], C.c_int),
     "oc_render": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz, P(T.AsmConfig), vp,
                    vp], C.c_int),
     "oc_render_dev_c128": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz,
@@ -715,3 +717,34 @@ def adam_step_dev(x_dev, g_dev, m_dev, v_dev, lo_dev, hi_dev, t, cfg, ctx=None):
         raise ValueError("x, g, m, v, lo, hi must be contiguous float64 of one length")
     torch.cuda.synchronize()
     return lib.oc_adam_step_dev(ctx.h, n, *[t_.data_ptr() for t_ in ts], int(t), C.byref(cfg))
+
+
+def mtf(psfs, dx, ctx=None):
+    """analysis::mtf (analysis.rs:133-150) for one PSF (ny, nx) or a stack
+    (count, ny, nx) of host intensities.  Returns (status, freqs, values)."""
+    ctx = ctx or context()
+    p = np.ascontiguousarray(psfs, np.float64)
+    one = p.ndim == 2
+    if one:
+        p = p[None]
+    count, ny, nx = p.shape
+    v = np.zeros((count, nx // 2 + 1))
+    f = np.zeros(nx // 2 + 1)
+    st = lib.oc_mtf(ctx.h, p.ctypes.data, count, nx, ny, dx, v.ctypes.data, f.ctypes.data)
+    return st, f, (v[0] if one else v)
+
+
+def mtf_dev(psfs_dev, dx, values_dev, ctx=None):
+    """Device form: psfs_dev float64 (count, ny, nx), values_dev float64
+    (count, nx/2 + 1).  Returns (status, freqs)."""
+    torch = _torch()
+    ctx = ctx or context()
+    count, ny, nx = psfs_dev.shape
+    if psfs_dev.dtype != torch.float64 or not psfs_dev.is_contiguous():
+        raise ValueError("psfs_dev must be contiguous float64")
+    if tuple(values_dev.shape) != (count, nx // 2 + 1) or values_dev.dtype != torch.float64:
+        raise ValueError("values_dev must be float64 (count, nx/2 + 1)")
+    f = np.zeros(nx // 2 + 1)
+    torch.cuda.synchronize()
+    st = lib.oc_mtf_dev(ctx.h, psfs_dev.data_ptr(), count, nx, ny, dx, values_dev.data_ptr(), f.ctypes.data)
+    return st, f

@@ -42,6 +42,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -1334,6 +1335,19 @@ int or_adam_step(uint64_t dim, double* x, const double* g, double* m, double* v,
         x[i] = xi;
     }
     return OC_OK;
+}
+
+// analysis.rs:133-150
+void or_mtf(const double* psf, uint64_t nx, uint64_t ny, double dx, double* freqs, double* values) {
+    std::vector<double> buf(2 * nx * ny, 0.0);
+    for (uint64_t k = 0; k < nx * ny; ++k) buf[2 * k] = psf[k];
+    or_fft2(buf.data(), nx, ny);
+    const double dc = std::max(std::hypot(buf[0], buf[1]), std::numeric_limits<double>::min());
+    const uint64_t half = nx / 2;
+    for (uint64_t k = 0; k <= half; ++k) {
+        freqs[k] = (double)k / ((double)nx * dx);
+        values[k] = std::min(std::hypot(buf[2 * k], buf[2 * k + 1]) / dc, 1.0);
+    }
 }
 
 // observe.rs:213-314

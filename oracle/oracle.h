@@ -178,6 +178,10 @@ int or_fd_gradient(const oc_band_geom* g, const or_design_step* designs,
 int or_adam_step(uint64_t dim, double* x, const double* g, double* m,
                  double* v, const double* lo, const double* hi, uint64_t t,
                  const oc_adam_config* cfg);
+/* analysis::mtf (analysis.rs:133-150): fft2 of the intensity, row fy = 0,
+   |F(k,0)| / max(|F(0,0)|, DBL_MIN) capped at 1 for k = 0..nx/2. */
+void or_mtf(const double* psf, uint64_t nx, uint64_t ny, double dx,
+            double* freqs, double* values);
 int or_render(const oc_band_geom* g, const double* bands, uint64_t nb,
               uint64_t ns, const oc_render_config* rc,
               const or_design_step* designs, uint64_t nsteps,

@@ -83,6 +83,8 @@ def _declare(L):
     L.or_trace.argtypes = [P(DesignStep), c_u64, G, vp, A, vp]
     L.or_fd_gradient.argtypes = [G, P(DesignStep), c_u64, c_u64, vp, vp, vp, vp, A, vp, vp, vp, vp, vp]
     L.or_adam_step.argtypes = [c_u64, vp, vp, vp, vp, vp, vp, c_u64, P(T.AdamConfig)]
+    L.or_mtf.argtypes = [vp, c_u64, c_u64, c_f64, vp, vp]
+    L.or_mtf.restype = None
     L.or_render.argtypes = [G, vp, c_u64, c_u64, P(T.RenderConfig), P(DesignStep), c_u64, A, vp, vp]
 
 
@@ -390,6 +392,16 @@ def adam_step(x, g, m, v, lo, hi, t, cfg):
         assert a.dtype == np.float64 and a.flags.c_contiguous
     return lib().or_adam_step(len(x), _ptr(x), _ptr(arrs[0]), _ptr(m), _ptr(v), _ptr(arrs[1]), _ptr(arrs[2]), t,
                               C.byref(cfg))
+
+
+def mtf(psf, dx):
+    """analysis::mtf (analysis.rs:133-150): (freqs, values) along +fx."""
+    p = np.ascontiguousarray(psf, np.float64)
+    ny, nx = p.shape
+    f = np.zeros(nx // 2 + 1)
+    v = np.zeros(nx // 2 + 1)
+    lib().or_mtf(_ptr(p), nx, ny, dx, _ptr(f), _ptr(v))
+    return f, v
 
 
 def render(designs, inputs, dx, dy, bands, asm_cfg, rc):
