@@ -17,10 +17,9 @@ constexpr double kInvTwoPi = 0.159154943091895335768883763372514362;
 // sin/cos of a phase argument.  Argument reduction modulo pi/2 is done in
 // FP64 by Cody-Waite with a three-part pi/2 (the product n*C1 and n*C2 are
 // exact for |n| < 2^31, so phases up to ~3e9 rad reduce to ~1e-16 absolute);
-// the reduced |x| <= pi/4 goes through Taylor polynomials: degree 17/18 in
-// FP64 (truncation < 5e-17) for the complex128 path, degree 11/10 in FP32
-// (truncation < 1.2e-10, so FP32 rounding dominates) for the FP32-bulk path.
-// The quadrant n mod 4 then rotates (sin, cos).
+// the reduced |x| <= pi/4 goes through Taylor polynomials of degree 17/18
+// in FP64 (truncation < 5e-17) for the complex128 path.  The quadrant
+// n mod 4 then rotates (sin, cos).
 constexpr double kPio2_1 = 1.57079632673412561416626;  // pi/2 rounded to 33 bits
 constexpr double kPio2_2 = 6.077100506303965976595549e-11;  // next 33 bits
 constexpr double kPio2_3 = 2.022266248795950631541144e-21;  // remainder (53 bits)
@@ -77,30 +76,17 @@ __device__ __forceinline__ cpx<double> unit_phase<double>(double th) {
     return quadrant<double>(s, c, q);
 }
 
-// FP32-bulk path: FP64 argument and reduction, FP32 polynomials.  The
-// reduction keeps two of the three parts of pi/2: the third contributes
-// n * 2.0e-21 rad, below 1e-14 rad for |th| < 1e7.
+// FP32-bulk path: the phase in turns, reduced in FP64 to f = th/(2 pi) - n,
+// n = rint(th/(2 pi)), with one rounding (fma), so |f| <= 1/2 carries an
+// error of |th| 2^-53 / (2 pi) turns (below 1e-9 rad for |th| < 1e7); f is
+// rounded to FP32 (at most 1e-7 rad) and goes to the hardware sine and
+// cosine, which take turns (v_sin_f32 / v_cos_f32; measured on gfx950 over
+// 2^22 points of [-1/2, 1/2): at most 1.26e-7 absolute error, rms 3.5e-8).
 template <>
 __device__ __forceinline__ cpx<float> unit_phase<float>(double th) {
-    const double n = rint(th * kInvPio2);
-    const int q = (int)n & 3;
-    const float x = (float)fma(-n, kPio2_2, fma(-n, kPio2_1, th));
-    const float x2 = x * x;
-    // Taylor terms through x^11 (sin) and x^10 (cos): truncation below
-    // 1.2e-10 on |x| <= pi/4, so only FP32 rounding remains.
-    float ps = -2.50521084e-08f;            // -1/11!
-    ps = fmaf(ps, x2, 2.75573192e-06f);     // 1/9!
-    ps = fmaf(ps, x2, -1.98412698e-04f);    // -1/7!
-    ps = fmaf(ps, x2, 8.33333333e-03f);     // 1/5!
-    ps = fmaf(ps, x2, -1.66666667e-01f);    // -1/3!
-    const float s = fmaf(x * x2, ps, x);
-    float pc = -2.75573192e-07f;            // -1/10!
-    pc = fmaf(pc, x2, 2.48015873e-05f);     // 1/8!
-    pc = fmaf(pc, x2, -1.38888889e-03f);    // -1/6!
-    pc = fmaf(pc, x2, 4.16666667e-02f);     // 1/4!
-    pc = fmaf(pc, x2, -0.5f);
-    const float c = fmaf(x2, pc, 1.0f);
-    return quadrant<float>(s, c, q);
+    const double n = rint(th * kInvTwoPi);
+    const float f = (float)fma(th, kInvTwoPi, -n);
+    return {__builtin_amdgcn_cosf(f), __builtin_amdgcn_sinf(f)};
 }
 
 template <typename R>
