@@ -414,7 +414,9 @@ def main():
 
     # pick the fastest fast-kernel layout for the padded length (untimed)
     variant, variant_ms = ctx.autotune(2 * n, double=False)
-    log(f"autotune P={2 * n} c64: variant {variant}, ms per variant {variant_ms}")
+    sep_layout = ctx.fresnel_sep_layout(2 * n)
+    log(f"autotune P={2 * n} c64: variant {variant}, ms per variant {variant_ms}; "
+        f"two-pass Fresnel layouts (columns, rows) {sep_layout}")
     log(f"workload: {nb} bands x 2048^2, c64; warmup {args.warmup}")
     for _ in range(args.warmup):
         step(d_asm, d_fr)
@@ -480,7 +482,8 @@ def main():
     copy_gbps = 2 * (1 << 30) / (ev0.elapsed_time(ev1) / 10 * 1e-3) / 1e9
     del ca, cb
     step_bytes = sum(alg[k] * timing[k][1] for k in alg) / nbreak + nb * n * n * 8  # + f64 mask lattice per band
-    extra = {"passes": passes, "fft_variant": variant, "autotune_ms": variant_ms, "step_alg_GB": step_bytes / 1e9,
+    extra = {"passes": passes, "fft_variant": variant, "autotune_ms": variant_ms,
+             "fresnel_sep_layout": list(sep_layout), "step_alg_GB": step_bytes / 1e9,
              "device_copy_GBps": copy_gbps,
              "step_alg_GBps": step_bytes / (elapsed / args.steps) / 1e9}
 

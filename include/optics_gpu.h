@@ -261,6 +261,19 @@ int oc_autotune_fft(oc_context* ctx, uint64_t n, int double_precision,
 int oc_context_set_fft_variant(oc_context* ctx, uint64_t n,
                                int double_precision, int variant);
 int oc_context_fft_variant(oc_context* ctx, uint64_t n, int double_precision);
+/* The complex64 Fresnel transfer function factors per axis, so its batches
+   run two passes (columns, then rows) whose layouts are chosen separately:
+   any two variants that block the intermediate with the same width.
+   oc_autotune_fft (complex64) times every such pair after the variants and
+   keeps the fastest; oc_context_set_fft_variant makes both passes follow
+   the forced variant; untuned, 4096 uses (0, 3), the pair measured fastest
+   on MI355X, and other lengths the propagation layout for both.  Setting a
+   pair whose widths differ returns OC_ERR_INVALID_ARGUMENT.  No reference
+   counterpart. */
+int oc_context_set_fresnel_sep_layout(oc_context* ctx, uint64_t n, int v_col,
+                                      int v_row);
+int oc_context_fresnel_sep_layout(oc_context* ctx, uint64_t n, int* v_col,
+                                  int* v_row);
 int oc_context_force_generic(oc_context* ctx, int enable);
 
 /* ------------------------------------------------------------------------ */

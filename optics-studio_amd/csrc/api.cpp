@@ -515,6 +515,7 @@ int oc_context_set_fft_variant(oc_context* ctx, uint64_t n, int dbl, int variant
     OC_CONTEXT_LOCK(ctx);
     if (!ctx || variant < 0 || variant >= std::max(1, ocl::fast_variants((int)n))) return OC_ERR_INVALID_ARGUMENT;
     ctx->fft_variant[std::make_pair((int)n, dbl ? 1 : 0)] = variant;
+    if (!dbl) ctx->sep_variant.erase((int)n);  // both Fresnel passes follow the forced layout
     return OC_OK;
 }
 
@@ -522,6 +523,33 @@ int oc_context_fft_variant(oc_context* ctx, uint64_t n, int dbl) {
     OC_CONTEXT_LOCK(ctx);
     if (!ctx || ocl::fast_variants((int)n) == 0) return -1;
     return ctx->variant_for((int)n, dbl != 0);
+}
+
+int oc_context_set_fresnel_sep_layout(oc_context* ctx, uint64_t n, int v_col, int v_row) {
+    OC_CONTEXT_LOCK(ctx);
+    if (!ctx) return OC_ERR_INVALID_ARGUMENT;
+    int cG, cB, cC, rG, rB, rC;
+    const int nv = ocl::fast_variants((int)n);
+    if (nv == 0 || v_col < 0 || v_row < 0 || v_col >= nv || v_row >= nv ||
+        !ocl::fast_supported(false, (int)n, v_col, &cG, &cB, &cC) ||
+        !ocl::fast_supported(false, (int)n, v_row, &rG, &rB, &rC))
+        return fail(ctx, OC_ERR_INVALID_ARGUMENT, "no such layout at this length");
+    if (cB != rB) return fail(ctx, OC_ERR_INVALID_ARGUMENT, "the two layouts block the intermediate differently");
+    ctx->sep_variant[(int)n] = std::make_pair(v_col, v_row);
+    return OC_OK;
+}
+
+int oc_context_fresnel_sep_layout(oc_context* ctx, uint64_t n, int* v_col, int* v_row) {
+    OC_CONTEXT_LOCK(ctx);
+    if (!ctx || !v_col || !v_row) return OC_ERR_INVALID_ARGUMENT;
+    if (ocl::fast_variants((int)n) == 0) {
+        *v_col = *v_row = -1;
+        return OC_OK;
+    }
+    const std::pair<int, int> sv = ctx->sep_variants_for((int)n);
+    *v_col = sv.first;
+    *v_row = sv.second;
+    return OC_OK;
 }
 
 int oc_context_force_generic(oc_context* ctx, int enable) {
@@ -533,7 +561,8 @@ int oc_context_force_generic(oc_context* ctx, int enable) {
 
 // Times every layout variant of the specialised kernels at padded length n on
 // a synthetic round of masked ASM then Fresnel propagations (up to 16 fields
-// of n/2 x n/2, padding 2) and keeps the fastest.
+// of n/2 x n/2, padding 2) and keeps the fastest; for complex64 it then times
+// the Fresnel round alone over the layout pairs of its two separable passes.
 int oc_autotune_fft(oc_context* ctx, uint64_t n, int dbl, int* variant_out, double* ms_out) {
     OC_CONTEXT_LOCK(ctx);
     if (!ctx) return OC_ERR_INVALID_ARGUMENT;
@@ -572,39 +601,72 @@ int oc_autotune_fft(oc_context* ctx, uint64_t n, int dbl, int* variant_out, doub
     const auto key = std::make_pair((int)n, dbl ? 1 : 0);
     const bool had = ctx->fft_variant.count(key) != 0;
     const int prev = had ? ctx->fft_variant[key] : 0;
+    const bool had_sep = !dbl && ctx->sep_variant.count((int)n) != 0;
+    const std::pair<int, int> prev_sep = had_sep ? ctx->sep_variant[(int)n] : std::make_pair(0, 0);
+    if (!dbl) ctx->sep_variant.erase((int)n);
     const bool timing = ctx->timing;
     ctx->timing = false;
-    int best = 0;
-    double best_ms = 1e300;
-    for (int v = 0; v < nv; ++v) {
-        ctx->fft_variant[key] = v;
-        double ms = 1e300;
+    // best of three timed rounds (after one untimed) of the ASM batch and/or
+    // the Fresnel batch
+    auto round_ms = [&](bool with_asm, double* ms) {
+        *ms = 1e300;
         for (int rep = 0; rep < 4; ++rep) {
             std::vector<Prob> pv = probs, pf = probs_f;
             hipEvent_t a = pool_event(ctx), e = pool_event(ctx);
             (void)hipEventRecord(a, ctx->stream);
-            st = run_batch(ctx, dbl, pv);
-            if (!st) st = run_batch(ctx, dbl, pf);
+            int s = with_asm ? run_batch(ctx, dbl, pv) : OC_OK;
+            if (!s) s = run_batch(ctx, dbl, pf);
             (void)hipEventRecord(e, ctx->stream);
             (void)hipEventSynchronize(e);
             float f = 0.f;
             (void)hipEventElapsedTime(&f, a, e);
             ctx->event_pool.push_back(a);
             ctx->event_pool.push_back(e);
-            if (st) break;
-            if (rep > 0) ms = std::min(ms, (double)f);
+            if (s) return s;
+            if (rep > 0) *ms = std::min(*ms, (double)f);
         }
-        if (st) break;
+        return (int)OC_OK;
+    };
+    int best = 0;
+    double best_ms = 1e300;
+    for (int v = 0; v < nv; ++v) {
+        ctx->fft_variant[key] = v;
+        double ms;
+        if ((st = round_ms(true, &ms))) break;
         if (ms_out) ms_out[v] = ms;
         if (ms < best_ms) {
             best_ms = ms;
             best = v;
         }
     }
+    // complex64: then the two separable Fresnel passes' layouts, over every
+    // pair that shares the blocked width
+    if (!st && !dbl) {
+        ctx->fft_variant[key] = best;
+        std::pair<int, int> best_sep(best, best);
+        double best_sep_ms = 1e300;
+        for (int vc = 0; vc < nv && !st; ++vc)
+            for (int vr = 0; vr < nv && !st; ++vr) {
+                int cG, cB, cC, rG, rB, rC;
+                if (!ocl::fast_supported(false, (int)n, vc, &cG, &cB, &cC) ||
+                    !ocl::fast_supported(false, (int)n, vr, &rG, &rB, &rC) || cB != rB)
+                    continue;
+                ctx->sep_variant[(int)n] = std::make_pair(vc, vr);
+                double ms;
+                if ((st = round_ms(false, &ms))) break;
+                if (ms < best_sep_ms) {
+                    best_sep_ms = ms;
+                    best_sep = std::make_pair(vc, vr);
+                }
+            }
+        if (!st) ctx->sep_variant[(int)n] = best_sep;
+    }
     ctx->timing = timing;
     if (st) {
         if (had) ctx->fft_variant[key] = prev;
         else ctx->fft_variant.erase(key);
+        if (had_sep) ctx->sep_variant[(int)n] = prev_sep;
+        else if (!dbl) ctx->sep_variant.erase((int)n);
         return st;
     }
     ctx->fft_variant[key] = best;

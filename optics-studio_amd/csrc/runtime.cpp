@@ -557,17 +557,28 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             b.rs = rs0;
         }
     }
-    // complex64 Fresnel batches without post-ops factor per axis: two passes
-    // (the x embedding offset must fall on a block boundary of the layout)
+    // complex64 Fresnel batches without post-ops factor per axis: two passes,
+    // each with its own layout (the pair shares the blocked width B, and the
+    // x embedding offset must fall on a block boundary)
     if (b.fast && !dbl && !b.rs) {
-        bool sep = true;
+        const std::pair<int, int> sv = ctx->sep_variants_for(b.fast_n);
+        int cG, cB, cC, rG, rB, rC;
+        bool sep = ocl::fast_supported(false, b.fast_n, sv.first, &cG, &cB, &cC) &&
+                   ocl::fast_supported(false, b.fast_n, sv.second, &rG, &rB, &rC) && cB == rB;
         for (int i : cand) {
             const Prob& p = probs[i];
             const size_t ox = (p.nxp - p.g.nx) / 2;
             sep = sep && p.engine == ENG_FRESNEL && (p.ops.pre == PRE_NONE || p.ops.pre == PRE_MASK) &&
-                  p.ops.post == POST_NONE && ox % (size_t)fB == 0;
+                  p.ops.post == POST_NONE && ox % (size_t)cB == 0;
         }
         b.sep = sep;
+        if (sep) {  // rows per workgroup from the row pass, columns from the column pass
+            b.fast_v = sv.first;
+            b.fast_v_row = sv.second;
+            fG = rG;
+            fB = cB;
+            fC = cC;
+        }
     }
     void *work = nullptr, *kbuf = nullptr;
     if (!cand.empty()) {
@@ -752,7 +763,8 @@ int launch_batch(oc_context* ctx, Batch& b, unsigned mask) {
         const int tpass = b.sep ? (pass == 1 ? PASS_T_SEP_COL : PASS_T_SEP_ROW) : (fused ? 4 : pass);
         hipError_t e = timed(ctx, tpass, [&] {
             if (b.fast) {
-                const ocd::FastSel sel{pass,       b.fast_n,   b.fast_v,     b.rs ? 1 : 0, b.post_kind,
+                const int v = (b.sep && pass == 2) ? b.fast_v_row : b.fast_v;
+                const ocd::FastSel sel{pass,       b.fast_n,   v,            b.rs ? 1 : 0, b.post_kind,
                                        b.col_kind, b.pre_kind, b.sep ? 1 : 0};
                 return b.dbl ? ocl::fast_launch_c128(sel, dprobs, nprob, c.total, grid, ctx->stream)
                              : ocl::fast_launch_c64(sel, dprobs, nprob, c.total, grid, ctx->stream);

@@ -121,6 +121,7 @@ struct oc_context {
     void* queued = nullptr;      // batches submitted without waiting (QueuedSet, created on first use)
     std::recursive_mutex mu;     // held by every entry point for the duration of its call
     std::map<std::pair<int, int>, int> fft_variant;  // (n, double?) -> fast-kernel layout variant
+    std::map<int, std::pair<int, int>> sep_variant;  // n -> (column, row) layouts of the two-pass Fresnel
     // The autotuned layout if there is one, else the layout that timed
     // fastest for a masked-ASM + Fresnel round on MI355X (oc_autotune_fft
     // over 16 problems, both precisions; 1024/2048 complex128 are within 1%
@@ -133,6 +134,18 @@ struct oc_context {
             case 4096: case 8192: return dbl ? 3 : 2;
             default: return 0;
         }
+    }
+    // Layouts of the two passes of the separable complex64 Fresnel, which
+    // need only agree on the blocked width B of their intermediate: the
+    // tuned or forced pair if there is one, else (for 4096) the pair that
+    // timed fastest on MI355X (columns with V0's 4-column workgroups, rows
+    // with V3's 2-row ones), else the propagation layout for both.
+    std::pair<int, int> sep_variants_for(int n) const {
+        auto it = sep_variant.find(n);
+        if (it != sep_variant.end()) return it->second;
+        if (n == 4096 && fft_variant.find(std::make_pair(n, 0)) == fft_variant.end()) return {0, 3};
+        const int v = variant_for(n, false);
+        return {v, v};
     }
 };
 
@@ -227,6 +240,7 @@ struct Batch {
     bool fast = false;  // power-of-two kernels (all problems share P_x = P_y = fast_n)
     int fast_n = 0;
     int fast_v = 0;
+    int fast_v_row = 0;      // layout of the separable Fresnel's row pass (pass 2)
     int post_kind = -2;      // post-op shared by every problem (-1: mixed)
     int col_kind = -2;       // engine shared by every problem (-1: mixed)
     int pre_kind = -2;       // pre-op shared by every problem (-1: mixed)

@@ -49,6 +49,8 @@ EXPORTS = {
     "oc_autotune_fft": ([vp, c_u64, C.c_int, P(C.c_int), P(c_f64)], C.c_int),
     "oc_context_set_fft_variant": ([vp, c_u64, C.c_int, C.c_int], C.c_int),
     "oc_context_fft_variant": ([vp, c_u64, C.c_int], C.c_int),
+    "oc_context_set_fresnel_sep_layout": ([vp, c_u64, C.c_int, C.c_int], C.c_int),
+    "oc_context_fresnel_sep_layout": ([vp, c_u64, P(C.c_int), P(C.c_int)], C.c_int),
     "oc_context_force_generic": ([vp, C.c_int], C.c_int),
     "oc_asm_sampling_report": ([P(T.BandGeom), c_f64, P(T.AsmConfig), P(T.AsmReport)], C.c_int),
     "oc_asm_propagate": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), vp, P(T.AsmReport)], C.c_int),
@@ -192,6 +194,17 @@ class Context:
 
     def fft_variant(self, n, double=False):
         return lib.oc_context_fft_variant(self.h, n, 1 if double else 0)
+
+    def set_fresnel_sep_layout(self, n, v_col, v_row):
+        """Layouts of the two-pass complex64 Fresnel's column and row passes."""
+        return lib.oc_context_set_fresnel_sep_layout(self.h, n, v_col, v_row)
+
+    def fresnel_sep_layout(self, n):
+        vc, vr = C.c_int(), C.c_int()
+        st = lib.oc_context_fresnel_sep_layout(self.h, n, C.byref(vc), C.byref(vr))
+        if st:
+            raise OcError(f"fresnel_sep_layout: {T.STATUS.get(st, st)}")
+        return vc.value, vr.value
 
     def force_generic(self, on=True):
         return lib.oc_context_force_generic(self.h, 1 if on else 0)

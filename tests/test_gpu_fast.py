@@ -171,6 +171,61 @@ def test_autotune_picks_a_variant(G):
     assert ctx.set_fft_variant(512, 1) != 0  # one layout only below 1024
 
 
+@pytest.mark.parametrize("P", [1024, 4096])
+def test_fresnel_sep_layout_pairs_are_bit_identical(G, P):
+    """The two passes of the complex64 separable Fresnel take their layouts
+    separately (any pair with one blocked width).  A layout only regroups
+    whole transforms per workgroup, so every pair must give the same bits,
+    masked or not; a pair with different widths is refused."""
+    import torch
+    n = P // 2
+    d, lam = 2e-6, 550e-9
+    u = field(n, P + 5)
+    rng = np.random.default_rng(P + 6)
+    phi = rng.uniform(0, 2 * np.pi, (n, n))
+    mask, keep = G.dev_mask(phi, d, d)
+    t = torch.as_tensor(u.astype(np.complex64)).cuda()
+    z = 1.5 * n * d * d / lam
+    cfg = T.FresnelConfig(2.0, 0, 0, 2.0)
+    ctx = G.Context(0)
+    for v in range(4):
+        assert ctx.set_fft_variant(P, v) == 0
+        assert ctx.fresnel_sep_layout(P) == (v, v)  # a forced variant drives both passes
+    # blocked widths at P >= 1024: variant 1 (G = B) blocks at half the width
+    # of variants 0, 2 and 3 (16 / 8 at 1024, 4 / 2 at 4096)
+    widths = {0: 2, 1: 1, 2: 2, 3: 2}
+    ref = None
+    pairs = [(a, b) for a in range(4) for b in range(4) if widths[a] == widths[b]]
+    assert len(pairs) > 4
+    for a, b in pairs:
+        assert ctx.set_fresnel_sep_layout(P, a, b) == 0
+        assert ctx.fresnel_sep_layout(P) == (a, b)
+        ctx.enable_timing(True)
+        st, out, _ = G.fresnel_propagate_dev(t, d, d, lam, z, cfg, ctx=ctx)
+        assert st == 0
+        st, out_m, _ = G.fresnel_propagate_dev(t, d, d, lam, z, cfg, mask=mask, ctx=ctx)
+        assert st == 0
+        assert ctx.pass_timing()["sep_col"][1] == 2  # both calls ran the two-pass form
+        ctx.enable_timing(False)
+        got = (out.cpu().numpy(), out_m.cpu().numpy())
+        say("sep pair", P, a, b)
+        if ref is None:
+            ref = got
+        else:
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (a, b)
+    bad = [(a, b) for a in range(4) for b in range(4) if widths[a] != widths[b]]
+    assert bad
+    for a, b in bad:
+        assert ctx.set_fresnel_sep_layout(P, a, b) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
+    assert ctx.set_fresnel_sep_layout(P, 4, 0) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
+    assert ctx.set_fresnel_sep_layout(96, 0, 0) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
+    v, _ = ctx.autotune(P)
+    a, b = ctx.fresnel_sep_layout(P)
+    say("tuned", P, v, (a, b))
+    assert (a, b) in pairs and ctx.fft_variant(P) == v
+    ctx.close()
+
+
 SEP_CASES = [(1024, 2.0), (1000, 2.048), (600, 3.4133), (777, 2.6357)]
 
 
