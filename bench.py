@@ -7,8 +7,12 @@ pixel blocks, then ASM z = 20 mm (padding 2 -> P = 4096, band-limited,
 strict) and Fresnel transfer-function z = 200 mm (padding 2, strict off).
 One step = one pass over the whole batch in the FP32-bulk form (complex64
 field, FP64 phase arguments and twiddles): 16 masked ASM propagations and 16
-Fresnel propagations, every problem of a stage in one persistent launch per
-pass.  value = propagations/s (32 per step), inputs resident in HBM.
+Fresnel propagations of their results, as one chained call
+(oc_propagate_chain_dev_c64: every problem of a pass in one persistent
+launch; the ASM's inverse row pass and the Fresnel's row pass fused, the ASM
+field kept on chip).  value = propagations/s (32 per step), inputs resident
+in HBM.  The same chain with the ASM field written out, and the two stages
+as two batch calls, are reported beside it.
 
 Also reported: per-pass kernel times from HIP events on the library stream
 (roofline of the dominant kernel), the complex128 form, the f64 host drop-in
@@ -394,12 +398,21 @@ def main():
                 x.mask = T.C.pointer(mask)
             else:
                 x.z, x.padding, x.strict, x.max_padding = c["z_fresnel"], 2.0, 0, 2.0
-            x.in_ = src[b].data_ptr()
-            x.out = dst[b].data_ptr()
+            x.in_ = src[b].data_ptr() if src is not None else 0
+            x.out = dst[b].data_ptr() if dst is not None else 0
         return arr
 
     d_asm = descs(T.ENGINE_ASM, u_in, u_asm)
     d_fr = descs(T.ENGINE_FRESNEL, u_asm, u_fr)
+    # the step: one chained call, mask + ASM then Fresnel per band; the ASM
+    # field is kept on chip (first stage's out = NULL)
+    ch_first = descs(T.ENGINE_ASM, u_in, None)
+    ch_second = descs(T.ENGINE_FRESNEL, None, u_fr)
+
+    def chain_step(first, second):
+        st = G.propagate_chain_dev(first, second, ctx=ctx)
+        if st:
+            raise RuntimeError(f"chain failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
 
     def step(da, df, dbl=False):
         # both batches queued, one wait: the Fresnel batch reads the ASM
@@ -419,7 +432,7 @@ def main():
         f"two-pass Fresnel layouts (columns, rows) {sep_layout}")
     log(f"workload: {nb} bands x 2048^2, c64; warmup {args.warmup}")
     for _ in range(args.warmup):
-        step(d_asm, d_fr)
+        chain_step(ch_first, ch_second)
     torch.cuda.synchronize()
     # the timed region brackets the column pass (the dominant kernel) with
     # HIP events on the library's stream; the other passes run unbracketed
@@ -429,7 +442,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(d_asm, d_fr)
+        chain_step(ch_first, ch_second)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ctx.enable_timing(False)
@@ -439,7 +452,7 @@ def main():
     ctx.enable_timing(True)
     nbreak = max(5, args.steps // 5)
     for _ in range(nbreak):
-        step(d_asm, d_fr)
+        chain_step(ch_first, ch_second)
     torch.cuda.synchronize()
     ctx.enable_timing(False)
     timing = ctx.pass_timing(reset=True)
@@ -449,15 +462,16 @@ def main():
     ms_step = 1e3 * elapsed / args.steps
     log(f"c64: {ms_step:.3f} ms/step, {value:.1f} propagations/s; passes {timing}")
 
-    # Algorithmic bytes per launch (minimal-sweep model, complex64 = 8 B):
-    # three-pass form (the masked ASM batch): row_fwd reads N^2 and writes
-    # N P; col reads and writes N P; row_inv reads N P and writes N^2.
-    # Two-pass separable form (the complex64 Fresnel batch): each pass reads
-    # and writes N^2.  Per band; 16 bands per launch.
+    # Algorithmic bytes per launch (minimal-sweep model, complex64 = 8 B),
+    # per band, 16 bands per launch.  The chained step: row_fwd reads N^2
+    # and writes N P; col reads and writes N P; the fused row pass (timing
+    # slot sep_row: the ASM's inverse rows and the Fresnel's rows) reads N P
+    # and writes N^2; the Fresnel's column pass (slot sep_col) reads and
+    # writes N^2.
     B = 8
     P = 2 * n
     alg = {"row_fwd": nb * (n * n + n * P) * B, "col": nb * (2 * n * P) * B, "row_inv": nb * (n * P + n * n) * B,
-           "sep_col": nb * (2 * n * n) * B, "sep_row": nb * (2 * n * n) * B}
+           "sep_col": nb * (2 * n * n) * B, "sep_row": nb * (n * P + n * n) * B}
     alg = {k: v for k, v in alg.items() if timing[k][1]}
     dom = max(alg, key=lambda k: timing[k][0])
     src = timing_col if dom == "col" else timing  # the timed region's events for the column pass
@@ -486,6 +500,23 @@ def main():
              "fresnel_sep_layout": list(sep_layout), "step_alg_GB": step_bytes / 1e9,
              "device_copy_GBps": copy_gbps,
              "step_alg_GBps": step_bytes / (elapsed / args.steps) / 1e9}
+
+    # the same chain with the ASM field written out, and the two stages as
+    # two queued batch calls (untimed for `value`)
+    k = max(5, args.steps // 5)
+    for name, fn in (("chain_with_intermediate", lambda: chain_step(d_asm, ch_second)),
+                     ("two_batch_calls", lambda: step(d_asm, d_fr))):
+        fn()
+        torch.cuda.synchronize()
+        s0 = time.perf_counter()
+        for _ in range(k):
+            fn()
+        torch.cuda.synchronize()
+        extra[name + "_propagations_per_s"] = 2 * nb * k / (time.perf_counter() - s0)
+    log(f"chain with the ASM field written: {extra['chain_with_intermediate_propagations_per_s']:.0f}/s; "
+        f"two batch calls: {extra['two_batch_calls_propagations_per_s']:.0f}/s")
+    chain_step(d_asm, ch_second)  # leaves both stages' fields for the parity check below
+    torch.cuda.synchronize()
 
     # parity of this run's last band output against the f64 oracle (and the
     # oracle timed on the host as the CPU baseline)
@@ -583,7 +614,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"polychromatic BandField {nb} bands 450-650 nm, 2048^2 at 2 um, 8-level DOE mask "
                                "+ ASM 20 mm (P=4096) + Fresnel TF 200 mm, FP32 bulk / FP64 phase",
-                   "bands": nb, "grid": n, "padded": P, "propagations_per_step": 2 * nb},
+                   "bands": nb, "grid": n, "padded": P, "propagations_per_step": 2 * nb,
+                   "call": "chained (ASM field not written)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": alg[dom], "avg_launch_ms": avg_ms},

@@ -435,6 +435,24 @@ int oc_propagate_batch_dev_c64(oc_context* ctx, oc_prop_desc* descs,
 int oc_propagate_batch_dev_c128(oc_context* ctx, oc_prop_desc* descs,
                                 size_t count);
 
+/* Two propagations in a row, problem by problem: second[i] propagates the
+   output of first[i] (two gaps of Beamline::trace, src/beamline.rs:109-130,
+   which calls asm::propagate then fresnel::propagate for a
+   mask-ASM-Fresnel line).  second[i].geom must equal first[i].geom and
+   second[i].in is ignored; first[i].out receives the intermediate field, or
+   may be NULL when the caller does not need it.  Statuses as two batch calls
+   would give them, except that a problem whose first stage fails is not
+   propagated further: second[i].status repeats first[i].status and
+   second[i].out is unspecified.  complex64 chains of masked or plain ASM
+   into Fresnel at one power-of-two padded length run fused (the ASM's
+   inverse row pass and the Fresnel's row pass in one kernel, the Fresnel's
+   column pass on the blocked intermediate); all others run as two batches.
+   No single reference counterpart: it replaces the pair of calls. */
+int oc_propagate_chain_dev_c64(oc_context* ctx, oc_prop_desc* first,
+                               oc_prop_desc* second, size_t count);
+int oc_propagate_chain_dev_c128(oc_context* ctx, oc_prop_desc* first,
+                                oc_prop_desc* second, size_t count);
+
 /* The same without waiting (no counterpart in the reference: a pipelining
    form for serving loops).  Host-side checks fill descs[i].status now and
    the batch is queued on the context stream; oc_context_wait resolves the

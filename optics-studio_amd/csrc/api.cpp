@@ -392,6 +392,144 @@ static int submit(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
     return first;
 }
 
+// A propagation chained into a second one of its output (two gaps of
+// Beamline::trace, src/beamline.rs:109-130, for many fields at once).  When
+// every problem is a complex64 ASM at a power-of-two padded length followed
+// by a Fresnel on the same padded grid, the ASM's inverse row pass and the
+// Fresnel's row pass run as one kernel and the Fresnel's column pass reads
+// the blocked intermediate (f_row_chain, f_sep_col_fin); otherwise the two
+// stages run as two batches.
+static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, size_t count) {
+    if (!ctx || ((!a || !f) && count)) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    if (count == 0) return OC_OK;
+    for (size_t i = 0; i < count; ++i) {
+        const oc_band_geom &ga = a[i].geom, &gf = f[i].geom;
+        if (ga.nx != gf.nx || ga.ny != gf.ny || ga.dx != gf.dx || ga.dy != gf.dy || ga.lambda != gf.lambda)
+            return fail(ctx, OC_ERR_INVALID_ARGUMENT, "the second stage's grid must be the first stage's");
+    }
+    std::vector<Prob> pa, pf;
+    int st = batch_probs(a, count, pa);
+    if (!st) st = batch_probs(f, count, pf);
+    if (st) return st;
+    const int np = (int)count;
+    evict_tables(ctx);
+    // fusable? (prepared copies; the unfused path prepares its own)
+    std::vector<Prob> qa = pa, qf = pf;
+    bool fuse = !dbl && !ctx->force_generic;
+    for (int i = 0; i < np && fuse; ++i) {
+        Prob &x = qa[i], &y = qf[i];
+        prepare(x);
+        prepare(y);
+        if (x.status == OC_OK && x.mask) x.status = mask_footprint(x.g, *x.mask);
+        fuse = x.status == OC_OK && y.status == OC_OK && x.launch && y.launch && !x.check_only && !y.check_only &&
+               x.engine == ENG_ASM && y.engine == ENG_FRESNEL && !y.mask && x.nxp == x.nyp && y.nxp == x.nxp &&
+               y.nyp == x.nyp && x.nxp == qa[0].nxp && ocl::fast_variants((int)x.nxp) > 0;
+    }
+    void* fl;
+    st = ensure_buf(ctx, "chain_flags", sizeof(int) * 2 * count, &fl);
+    if (st) return st;
+    int* dflags = static_cast<int*>(fl);
+    if (fuse) {
+        for (int i = 0; i < np; ++i) qf[i].in = qa[i].out;  // unused by the fused passes
+        HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, 2 * np, ctx->stream));
+        Batch ba, bf;
+        // the Fresnel batch takes the separable form's layouts (rows for the
+        // fused row pass, columns for its column pass); both must block the
+        // intermediates with the ASM batch's width
+        st = build_batch(ctx, false, qa, dflags, ba, false, nullptr, "chA");
+        if (!st) st = build_batch(ctx, false, qf, dflags + np, bf, false, nullptr, "chF");
+        if (st) return st;
+        fuse = ba.fast && !ba.sep && !ba.rs && ba.col_kind == ENG_ASM && ba.post_kind == POST_NONE &&
+               (int)ba.pd.size() == np && bf.fast && bf.sep && bf.fast_n == ba.fast_n && (int)bf.pd.size() == np &&
+               ba.pd[0].B == bf.pd[0].B;
+        if (fuse) {
+            st = launch_batch(ctx, ba, (1u << 0) | (1u << 1));
+            if (st) return st;
+            const ProbDev* da = static_cast<const ProbDev*>(ba.dprobs);
+            const ProbDev* df = static_cast<const ProbDev*>(bf.dprobs);
+            const int rows = bf.pc[2].total, cols = bf.pc[1].total;
+            hipError_t e = timed(ctx, PASS_T_SEP_ROW, [&] {
+                return ocl::fast_launch_chain_c64(0, bf.fast_n, bf.fast_v_row, da, df, np, rows,
+                                                  std::min(rows, ctx->num_cu * 8), ctx->stream);
+            });
+            if (e == hipSuccess)
+                e = timed(ctx, PASS_T_SEP_COL, [&] {
+                    return ocl::fast_launch_chain_c64(1, bf.fast_n, bf.fast_v, nullptr, df, np, cols,
+                                                      std::min(cols, ctx->num_cu * 8), ctx->stream);
+                });
+            if (e != hipSuccess) return fail(ctx, OC_ERR_DEVICE, std::string("chain launch: ") + hipGetErrorString(e));
+            std::vector<Prob> all = qa;
+            all.insert(all.end(), qf.begin(), qf.end());
+            finish_problems(ctx, all, dflags);
+            int first = OC_OK;
+            for (int i = 0; i < np; ++i) {
+                const int sa = all[i].status;
+                const int sf = sa != OC_OK ? sa : all[np + i].status;  // a failed first stage fails the chain
+                a[i].status = sa;
+                f[i].status = sf;
+                if (first == OC_OK && sa != OC_OK) first = sa;
+                if (first == OC_OK && sf != OC_OK) first = sf;
+            }
+            return first;
+        }
+        // the batches were only built (and any shared transmission computed): fall through
+    }
+    // two batches: the first stage into the caller's buffer or a scratch one
+    size_t need = 0;
+    const size_t esz = dbl ? 16 : 8;
+    for (int i = 0; i < np; ++i)
+        if (!pa[i].out) need += pa[i].g.nx * pa[i].g.ny * esz;
+    void* scratch = nullptr;
+    if (need) {
+        st = ensure_buf(ctx, "chain_mid", need, &scratch);
+        if (st) return st;
+    }
+    size_t off = 0;
+    for (int i = 0; i < np; ++i) {
+        if (!pa[i].out) {
+            pa[i].out = static_cast<unsigned char*>(scratch) + off;
+            off += pa[i].g.nx * pa[i].g.ny * esz;
+        }
+        pf[i].in = pa[i].out;
+    }
+    st = launch_problems(ctx, dbl, pa, dflags);
+    if (st) return st;
+    finish_problems(ctx, pa, dflags);
+    std::vector<Prob> rest;
+    std::vector<int> where;
+    for (int i = 0; i < np; ++i)
+        if (pa[i].status == OC_OK) {
+            rest.push_back(pf[i]);
+            where.push_back(i);
+        } else {
+            pf[i].status = pa[i].status;
+        }
+    if (!rest.empty()) {
+        st = launch_problems(ctx, dbl, rest, dflags + np);
+        if (st) return st;
+        finish_problems(ctx, rest, dflags + np);
+        for (size_t k = 0; k < rest.size(); ++k) pf[where[k]].status = rest[k].status;
+    }
+    int first = OC_OK;
+    for (int i = 0; i < np; ++i) {
+        a[i].status = pa[i].status;
+        f[i].status = pf[i].status;
+        if (first == OC_OK && pa[i].status != OC_OK) first = pa[i].status;
+        if (first == OC_OK && pf[i].status != OC_OK) first = pf[i].status;
+    }
+    return first;
+}
+
+int oc_propagate_chain_dev_c64(oc_context* ctx, oc_prop_desc* first, oc_prop_desc* second, size_t count) {
+    OC_CONTEXT_LOCK(ctx);
+    return chain(ctx, false, first, second, count);
+}
+int oc_propagate_chain_dev_c128(oc_context* ctx, oc_prop_desc* first, oc_prop_desc* second, size_t count) {
+    OC_CONTEXT_LOCK(ctx);
+    return chain(ctx, true, first, second, count);
+}
+
 int oc_propagate_batch_dev_c64(oc_context* ctx, oc_prop_desc* d, size_t count) {
     OC_CONTEXT_LOCK(ctx);
     return batch(ctx, false, d, count);

@@ -626,6 +626,140 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* A propagation chained into a complex64 separable Fresnel.            */
+/* ------------------------------------------------------------------ */
+// The first propagation's inverse row pass and the Fresnel's row pass touch
+// the same rows, and the Fresnel embeds the first result at the same offset
+// (same grid, same padded length), so one row kernel does both: inverse row
+// transform, crop and scale (the first propagation's output, written only
+// when the caller asks for it), forward row transform, x factor, inverse,
+// crop, blocked store for the Fresnel's column pass.  That pass then runs
+// on contiguous slabs and writes the natural-layout output.  A and F are
+// the two stages' descriptors, problem by problem; row tiles follow F (the
+// Fresnel batch's row layout), and A's intermediate must be blocked with
+// this layout's width B.
+template <typename R, int N, int V>
+__global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_chain(const ProbDev* __restrict__ A,
+                                                                           const ProbDev* __restrict__ F,
+                                                                           int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, G = LY::G, Q = LY::B;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
+    const int t = tb + Q * tq;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const int pi = find_problem<PASS_ROW>(F, nprob, tile);  // F's row tiles (this kernel's G)
+        const ProbDev& P = A[pi];
+        const ProbDev& Fp = F[pi];
+        const int iy = (tile - Fp.base_row) * G + g;
+        const bool rowok = iy < P.ny;
+        const int ny = P.ny, nx = P.nx, ox = P.ox;
+        const cpx<R>* work = static_cast<const cpx<R>*>(P.work);
+        const unsigned b0 = ((unsigned)tq * ny + iy) * Q + tb;
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            v[m] = {R(0), R(0)};
+            if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
+        }
+        const cpx<R>* tw = static_cast<const cpx<R>*>(P.prow.tw);
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw);
+        // the first propagation's output, re-embedded where the Fresnel puts it
+        const R s = (R)P.scale;
+        cpx<R>* mid = static_cast<cpx<R>*>(P.out);
+        bool bad = false;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int x = t + m * T - ox;
+            const bool in = rowok && x >= 0 && x < nx;
+            const cpx<R> o = {v[m].re * s, v[m].im * s};
+            v[m] = in ? o : cpx<R>{R(0), R(0)};
+            bad |= in && !finite_c(o);
+            if (in && mid) at32(mid, (unsigned)iy * nx + x) = o;
+        }
+        if (bad) {
+            atomicOr(P.flag, 2);
+            atomicOr(Fp.flag, 1);
+        }
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, tw);
+        const cpx<R>* hx = static_cast<const cpx<R>*>(Fp.eng.hx);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            cpx<R> h;
+            h = at32(hx, t + m * T);
+            v[m] = cmul(v[m], h);
+        }
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw);
+        // column c = x - ox goes to block c / Q, lane c % Q = tb (ox % Q == 0)
+        if (rowok) {
+            cpx<R>* fw = static_cast<cpx<R>*>(Fp.work);
+            const unsigned w0 = ((unsigned)tq * ny + iy) * Q + tb - (unsigned)(ox / Q) * ny * Q;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int x = t + m * T;
+                if (x >= ox && x < ox + nx) at32(fw, w0 + (unsigned)(m * (T / Q)) * ny * Q) = v[m];
+            }
+        }
+    }
+}
+
+// The chained Fresnel's column pass: CB columns per tile from the blocked
+// intermediate, y factor, crop, 1/(P_x P_y), natural-layout output.
+template <typename R, int N, int V>
+__global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_col_fin(
+    const ProbDev* __restrict__ probs, int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, B = LY::B, CB = LY::CB;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int g = tid % CB, t = tid / CB;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
+        const int kx = (tile - P.base_col) * CB + g;
+        const int nx = P.nx, ny = P.ny, oy = P.oy;
+        const bool colok = kx < nx;
+        const cpx<R>* work = static_cast<const cpx<R>*>(P.work);
+        // element y of column kx: b0 + y * B
+        const unsigned b0 = ((unsigned)(kx / B) * ny - oy) * B + kx % B;
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int y = t + m * T;
+            v[m] = {R(0), R(0)};
+            if (colok && y >= oy && y < oy + ny) v[m] = at32(work, b0 + (unsigned)y * B);
+        }
+        const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
+        fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
+        const cpx<R>* hy = static_cast<const cpx<R>*>(P.eng.hy);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            cpx<R> h;
+            h = at32(hy, t + m * T);
+            v[m] = cmul(v[m], h);
+        }
+        fft_regs<R, N, CB, +1>(v, lds, g, t, tw);
+        const R s = (R)P.scale;
+        bool bad = false;
+        if (colok) {
+            cpx<R>* out = static_cast<cpx<R>*>(P.out);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int y = t + m * T;
+                if (y >= oy && y < oy + ny) {
+                    const cpx<R> o = {v[m].re * s, v[m].im * s};
+                    bad |= !finite_c(o);
+                    at32(out, (unsigned)(y - oy) * nx + kx) = o;
+                }
+            }
+        }
+        if (bad) atomicOr(P.flag, 2);
+    }
+}
+
 }  // namespace ocd
 
 namespace ocl {
@@ -714,6 +848,30 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
     return hipGetLastError();
 }
 
+// which = 0: f_row_chain over (A, F); 1: f_sep_col_fin over F.  complex64.
+template <int N, int V>
+static hipError_t launch_chain_nv(int which, const ProbDev* a, const ProbDev* f, int np, int total, int grid,
+                                  hipStream_t st) {
+    using R = Real;
+    if constexpr (sizeof(R) == 4) {
+        using LY = FastLayout<R, N, V>;
+        constexpr int T = N / 16;
+        const dim3 g(grid);
+        if (which == 0) {
+            const size_t lds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
+            hipLaunchKernelGGL((f_row_chain<R, N, V>), g, dim3(LY::G * T), lds, st, a, f, np, total);
+            return hipGetLastError();
+        }
+        if (which == 1) {
+            const size_t lds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
+            hipLaunchKernelGGL((f_sep_col_fin<R, N, V>), g, dim3(LY::CB * T), lds, st, f, np, total);
+            return hipGetLastError();
+        }
+    }
+    (void)a; (void)f; (void)np; (void)total; (void)grid; (void)st; (void)which;
+    return hipErrorInvalidValue;
+}
+
 static hipError_t allow_lds(const void* kernel) {
     return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
@@ -746,7 +904,8 @@ static hipError_t attr_nv() {
     if constexpr (sizeof(R) == 4) {
         const void* ks[] = {(const void*)f_sep_col<R, N, V, PRE_NONE>, (const void*)f_sep_col<R, N, V, PRE_MASK>,
                             (const void*)f_sep_col<R, N, V, PRE_TMUL>, (const void*)f_sep_col<R, N, V, PRE_MASKS>,
-                            (const void*)f_sep_row<R, N, V>};
+                            (const void*)f_sep_row<R, N, V>,         (const void*)f_row_chain<R, N, V>,
+                            (const void*)f_sep_col_fin<R, N, V>};
         for (const void* f : ks) {
             const hipError_t e = allow_lds(f);
             if (e != hipSuccess) return e;
@@ -778,6 +937,29 @@ hipError_t OC_FAST_FN(fast_launch)(const FastSel& s, const ProbDev* d, int np, i
         }
         OC_FAST_SIZES_2(OC_C2)
 #undef OC_C2
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// The chained propagation's fused row pass (which = 0, descriptors a and f)
+// and final Fresnel column pass (which = 1, descriptors f); complex64 only.
+hipError_t OC_FAST_FN(fast_launch_chain)(int which, int n, int v, const ProbDev* a, const ProbDev* f, int np,
+                                         int total, int grid, hipStream_t st) {
+    switch (n) {
+#define OC_H1(NN) \
+    case NN: return launch_chain_nv<NN, 0>(which, a, f, np, total, grid, st);
+        OC_FAST_SIZES_1(OC_H1)
+#undef OC_H1
+#define OC_H2(NN)                                                                  \
+    case NN:                                                                       \
+        switch (v) {                                                               \
+            case 1: return launch_chain_nv<NN, 1>(which, a, f, np, total, grid, st); \
+            case 2: return launch_chain_nv<NN, 2>(which, a, f, np, total, grid, st); \
+            case 3: return launch_chain_nv<NN, 3>(which, a, f, np, total, grid, st); \
+            default: return launch_chain_nv<NN, 0>(which, a, f, np, total, grid, st); \
+        }
+        OC_FAST_SIZES_2(OC_H2)
+#undef OC_H2
         default: return hipErrorInvalidValue;
     }
 }

@@ -517,7 +517,7 @@ int batch_alloc(oc_context* ctx, Batch& b, bool persistent, const std::string& n
 // `subset` (optional) restricts the batch to those problem indices; `tag`
 // names the transient buffers so that several batches can be in flight.
 int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags, Batch& b, bool persistent,
-                const std::vector<int>* subset, const std::string& tag) {
+                const std::vector<int>* subset, const std::string& tag, int force_v) {
     const size_t esz = dbl ? 16 : 8;
     b.dbl = dbl;
     size_t work_elems = 0, kbuf_elems = 0;
@@ -549,7 +549,7 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         const bool rs0 = conv_engine(probs[cand[0]].engine);
         for (int i : cand)
             ok = ok && probs[i].nxp == n0 && probs[i].nyp == n0 && conv_engine(probs[i].engine) == rs0;
-        const int v = ctx->variant_for((int)n0, dbl);
+        const int v = force_v >= 0 ? force_v : ctx->variant_for((int)n0, dbl);
         if (ok && ocl::fast_supported(dbl, (int)n0, v, &fG, &fB, &fC)) {
             b.fast = true;
             b.fast_n = (int)n0;
@@ -561,7 +561,8 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     // each with its own layout (the pair shares the blocked width B, and the
     // x embedding offset must fall on a block boundary)
     if (b.fast && !dbl && !b.rs) {
-        const std::pair<int, int> sv = ctx->sep_variants_for(b.fast_n);
+        const std::pair<int, int> sv =
+            force_v >= 0 ? std::make_pair(force_v, force_v) : ctx->sep_variants_for(b.fast_n);
         int cG, cB, cC, rG, rB, rC;
         bool sep = ocl::fast_supported(false, b.fast_n, sv.first, &cG, &cB, &cC) &&
                    ocl::fast_supported(false, b.fast_n, sv.second, &rG, &rB, &rC) && cB == rB;

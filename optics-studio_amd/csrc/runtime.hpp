@@ -37,6 +37,10 @@ hipError_t launch_copy(bool dbl, const void* in, void* out, size_t n, hipStream_
 hipError_t launch_fill_int(int* p, int v, int n, hipStream_t st);
 hipError_t fast_launch_c64(const ocd::FastSel& s, const ocd::ProbDev* d, int np, int total, int grid, hipStream_t st);
 hipError_t fast_launch_c128(const ocd::FastSel& s, const ocd::ProbDev* d, int np, int total, int grid, hipStream_t st);
+hipError_t fast_launch_chain_c64(int which, int n, int v, const ocd::ProbDev* a, const ocd::ProbDev* f, int np,
+                                 int total, int grid, hipStream_t st);
+hipError_t fast_launch_chain_c128(int which, int n, int v, const ocd::ProbDev* a, const ocd::ProbDev* f, int np,
+                                  int total, int grid, hipStream_t st);
 void fast_layout_c64(int n, int v, int* G, int* B, int* CB);
 void fast_layout_c128(int n, int v, int* G, int* B, int* CB);
 hipError_t fast_set_lds_limits_c64();
@@ -268,8 +272,10 @@ struct QueuedSet {
 QueuedSet& queued_set(oc_context* ctx);
 void release_queued(oc_context* ctx);
 int batch_alloc(oc_context* ctx, Batch& b, bool persistent, const std::string& name, size_t bytes, void** out);
+// force_v >= 0: every pass of the batch uses layout variant force_v (the
+// separable Fresnel's two passes included) instead of the context's choice.
 int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags, Batch& b, bool persistent,
-                const std::vector<int>* subset = nullptr, const std::string& tag = "");
+                const std::vector<int>* subset = nullptr, const std::string& tag = "", int force_v = -1);
 constexpr unsigned kAllPasses = 0xF;
 // pass 4 (power-of-two batches with a GS phase post-op): one iteration's
 // inverse row pass fused with the next one's forward row pass

@@ -102,6 +102,8 @@ EXPORTS = {
     "oc_fd_gradient_dev_c64": ([vp, P(T.BandGeom), P(T.DesignStep), c_sz, c_sz, vp, vp, vp, vp, P(T.AsmConfig), vp,
                                 vp, vp, vp, P(c_sz)], C.c_int),
     "oc_adam_step_dev": ([vp, c_sz, vp, vp, vp, vp, vp, vp, C.c_uint64, P(T.AdamConfig)], C.c_int),
+    "oc_propagate_chain_dev_c64": ([vp, P(T.PropDesc), P(T.PropDesc), c_sz], C.c_int),
+    "oc_propagate_chain_dev_c128": ([vp, P(T.PropDesc), P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_batch_submit_dev_c64": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_batch_submit_dev_c128": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_context_wait": ([vp], C.c_int),
@@ -462,6 +464,17 @@ def propagate_batch_dev(descs, double=False, ctx=None):
     _torch().cuda.synchronize()
     fn = lib.oc_propagate_batch_dev_c128 if double else lib.oc_propagate_batch_dev_c64
     return fn(ctx.h, descs, len(descs))
+
+
+def propagate_chain_dev(first, second, double=False, ctx=None):
+    """first, second: ctypes arrays of PropDesc of one length; second[i]
+    propagates first[i]'s output (first[i].out may be 0).  Returns status."""
+    ctx = ctx or context()
+    _torch().cuda.synchronize()
+    if len(first) != len(second):
+        raise ValueError("first and second must have one length")
+    fn = lib.oc_propagate_chain_dev_c128 if double else lib.oc_propagate_chain_dev_c64
+    return fn(ctx.h, first, second, len(first))
 
 
 def propagate_batch_submit_dev(descs, double=False, ctx=None):
