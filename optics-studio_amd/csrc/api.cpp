@@ -439,6 +439,10 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
         // intermediates with the ASM batch's width
         st = build_batch(ctx, false, qa, dflags, ba, false, nullptr, "chA");
         if (!st) st = build_batch(ctx, false, qf, dflags + np, bf, false, nullptr, "chF");
+        if (!st && ba.fast && bf.fast && !ba.pd.empty() && !bf.pd.empty() && ba.pd[0].B != bf.pd[0].B) {
+            bf = Batch();  // the tuned pair blocks differently: both Fresnel passes take the ASM's layout
+            st = build_batch(ctx, false, qf, dflags + np, bf, false, nullptr, "chF", ba.fast_v);
+        }
         if (st) return st;
         fuse = ba.fast && !ba.sep && !ba.rs && ba.col_kind == ENG_ASM && ba.post_kind == POST_NONE &&
                (int)ba.pd.size() == np && bf.fast && bf.sep && bf.fast_n == ba.fast_n && (int)bf.pd.size() == np &&

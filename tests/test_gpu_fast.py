@@ -171,6 +171,18 @@ def test_autotune_picks_a_variant(G):
     assert ctx.set_fft_variant(512, 1) != 0  # one layout only below 1024
 
 
+def block_width(P, v):
+    """Width B of the blocked intermediate of complex64 layout variant v at
+    padded length P >= 1024: G = min(row-thread cap / (P/16), rows that fit
+    140 KB of LDS at the row stride, 16) with caps 1024 for variants 0 and 2
+    and 512 otherwise; B = min(G * (1, 1, 1, 2)[v], P/16, 16)."""
+    t = P // 16
+    slots = P + P // 16
+    rstride = slots + (40 - slots % 32) % 32
+    g = min((512 if v in (1, 3) else 1024) // t, 140 * 1024 // (rstride * 8), 16)
+    return min(g * (1, 1, 1, 2)[v], t, 16)
+
+
 @pytest.mark.parametrize("P", [1024, 4096])
 def test_fresnel_sep_layout_pairs_are_bit_identical(G, P):
     """The two passes of the complex64 separable Fresnel take their layouts
@@ -188,14 +200,14 @@ def test_fresnel_sep_layout_pairs_are_bit_identical(G, P):
     z = 1.5 * n * d * d / lam
     cfg = T.FresnelConfig(2.0, 0, 0, 2.0)
     ctx = G.Context(0)
-    for v in range(4):
+    nv = G.FFT_VARIANTS
+    for v in range(nv):
         assert ctx.set_fft_variant(P, v) == 0
         assert ctx.fresnel_sep_layout(P) == (v, v)  # a forced variant drives both passes
-    # blocked widths at P >= 1024: variant 1 (G = B) blocks at half the width
-    # of variants 0, 2 and 3 (16 / 8 at 1024, 4 / 2 at 4096)
-    widths = {0: 2, 1: 1, 2: 2, 3: 2}
+    widths = {v: block_width(P, v) for v in range(nv)}
+    say("block widths", P, widths)
     ref = None
-    pairs = [(a, b) for a in range(4) for b in range(4) if widths[a] == widths[b]]
+    pairs = [(a, b) for a in range(nv) for b in range(nv) if widths[a] == widths[b]]
     assert len(pairs) > 4
     for a, b in pairs:
         assert ctx.set_fresnel_sep_layout(P, a, b) == 0
@@ -213,11 +225,11 @@ def test_fresnel_sep_layout_pairs_are_bit_identical(G, P):
             ref = got
         else:
             assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), (a, b)
-    bad = [(a, b) for a in range(4) for b in range(4) if widths[a] != widths[b]]
+    bad = [(a, b) for a in range(nv) for b in range(nv) if widths[a] != widths[b]]
     assert bad
     for a, b in bad:
         assert ctx.set_fresnel_sep_layout(P, a, b) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
-    assert ctx.set_fresnel_sep_layout(P, 4, 0) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
+    assert ctx.set_fresnel_sep_layout(P, nv, 0) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
     assert ctx.set_fresnel_sep_layout(96, 0, 0) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
     v, _ = ctx.autotune(P)
     a, b = ctx.fresnel_sep_layout(P)
