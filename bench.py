@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the f64 and host drop-in legs")
     ap.add_argument("--only", choices=["gs", "adjoint", "rs", "render", "fd"],
                     help="run only that secondary workload and print its record (for profiling)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every workgroup layout at P = 4096 first and run the fastest (default: the layouts "
+                         "measured fastest on MI355X, so that every run, profiled or not, runs the same kernels)")
+    ap.add_argument("--layout", type=str, default=None,
+                    help="V or V,C,R: run layout variant V (and the separable Fresnel's column/row layouts C,R)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB (8 launches) for PMC counter calibration")
     return ap.parse_args()
@@ -425,10 +430,18 @@ def main():
         if st:
             raise RuntimeError(f"step failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
 
-    # pick the fastest fast-kernel layout for the padded length (untimed)
-    variant, variant_ms = ctx.autotune(2 * n, double=False)
+    # workgroup layouts for the padded length (untimed)
+    if args.layout:
+        lv = [int(x) for x in args.layout.split(",")]
+        if ctx.set_fft_variant(2 * n, lv[0]) != 0 or (len(lv) == 3 and ctx.set_fresnel_sep_layout(2 * n, lv[1], lv[2])):
+            raise SystemExit(f"--layout {args.layout}: no such layout at P = {2 * n}")
+        variant, variant_ms = lv[0], None
+    elif args.autotune:
+        variant, variant_ms = ctx.autotune(2 * n, double=False)
+    else:
+        variant, variant_ms = ctx.fft_variant(2 * n), None
     sep_layout = ctx.fresnel_sep_layout(2 * n)
-    log(f"autotune P={2 * n} c64: variant {variant}, ms per variant {variant_ms}; "
+    log(f"layout P={2 * n} c64: variant {variant}, autotune ms per variant {variant_ms}; "
         f"two-pass Fresnel layouts (columns, rows) {sep_layout}")
     log(f"workload: {nb} bands x 2048^2, c64; warmup {args.warmup}")
     for _ in range(args.warmup):
@@ -501,27 +514,28 @@ def main():
              "device_copy_GBps": copy_gbps,
              "step_alg_GBps": step_bytes / (elapsed / args.steps) / 1e9}
 
-    # the same chain with the ASM field written out, and the two stages as
-    # two queued batch calls (untimed for `value`)
-    k = max(5, args.steps // 5)
-    for name, fn in (("chain_with_intermediate", lambda: chain_step(d_asm, ch_second)),
-                     ("two_batch_calls", lambda: step(d_asm, d_fr))):
-        fn()
-        torch.cuda.synchronize()
-        s0 = time.perf_counter()
-        for _ in range(k):
+    if not args.no_extra:
+        # the same chain with the ASM field written out, and the two stages
+        # as two queued batch calls (untimed for `value`)
+        k = max(5, args.steps // 5)
+        for name, fn in (("chain_with_intermediate", lambda: chain_step(d_asm, ch_second)),
+                         ("two_batch_calls", lambda: step(d_asm, d_fr))):
             fn()
-        torch.cuda.synchronize()
-        extra[name + "_propagations_per_s"] = 2 * nb * k / (time.perf_counter() - s0)
-    log(f"chain with the ASM field written: {extra['chain_with_intermediate_propagations_per_s']:.0f}/s; "
-        f"two batch calls: {extra['two_batch_calls_propagations_per_s']:.0f}/s")
-    chain_step(d_asm, ch_second)  # leaves both stages' fields for the parity check below
-    torch.cuda.synchronize()
+            torch.cuda.synchronize()
+            s0 = time.perf_counter()
+            for _ in range(k):
+                fn()
+            torch.cuda.synchronize()
+            extra[name + "_propagations_per_s"] = 2 * nb * k / (time.perf_counter() - s0)
+        log(f"chain with the ASM field written: {extra['chain_with_intermediate_propagations_per_s']:.0f}/s; "
+            f"two batch calls: {extra['two_batch_calls_propagations_per_s']:.0f}/s")
 
     # parity of this run's last band output against the f64 oracle (and the
     # oracle timed on the host as the CPU baseline)
     cpu = None
     if not args.no_cpu:
+        chain_step(d_asm, ch_second)  # both stages' fields of the chained call
+        torch.cuda.synchronize()
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import oracle
         lam = lams[-1]
