@@ -465,7 +465,8 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
             if (e != hipSuccess) return fail(ctx, OC_ERR_DEVICE, std::string("chain launch: ") + hipGetErrorString(e));
             std::vector<Prob> all = qa;
             all.insert(all.end(), qf.begin(), qf.end());
-            finish_problems(ctx, all, dflags);
+            const int fs = finish_problems(ctx, all, dflags);
+            if (fs == OC_ERR_DEVICE || fs == OC_ERR_OUT_OF_MEMORY) return fs;  // flags never read back
             int first = OC_OK;
             for (int i = 0; i < np; ++i) {
                 const int sa = all[i].status;
@@ -499,7 +500,8 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
     }
     st = launch_problems(ctx, dbl, pa, dflags);
     if (st) return st;
-    finish_problems(ctx, pa, dflags);
+    st = finish_problems(ctx, pa, dflags);
+    if (st == OC_ERR_DEVICE || st == OC_ERR_OUT_OF_MEMORY) return st;
     std::vector<Prob> rest;
     std::vector<int> where;
     for (int i = 0; i < np; ++i)
@@ -512,7 +514,8 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
     if (!rest.empty()) {
         st = launch_problems(ctx, dbl, rest, dflags + np);
         if (st) return st;
-        finish_problems(ctx, rest, dflags + np);
+        st = finish_problems(ctx, rest, dflags + np);
+        if (st == OC_ERR_DEVICE || st == OC_ERR_OUT_OF_MEMORY) return st;
         for (size_t k = 0; k < rest.size(); ++k) pf[where[k]].status = rest[k].status;
     }
     int first = OC_OK;

@@ -54,6 +54,54 @@ struct FastLayout {
     static_assert(G >= 1 && B >= 1 && CB >= 1 && (T % B) == 0 && (B % CB) == 0, "layout");
 };
 
+// The transmission or phase-mask pre-op of a row pass on the row's 16
+// samples in registers (x = t + m T in the padded row, iy the field row):
+// PRE_TMUL multiplies by the shared transmission, PRE_MASK applies the
+// problem's mask, PRE_MASKS either per problem; others leave v alone.
+template <typename R, int N, int PK>
+__device__ __forceinline__ void row_pre(const ProbDev& P, cpx<R> (&v)[16], int iy, bool rowok, int t) {
+    constexpr int T = N / 16;
+    const int ox = P.ox, nx = P.nx;
+    const unsigned r0 = (unsigned)iy * nx - ox;
+    if (PK == PRE_TMUL || (PK == PRE_MASKS && P.ops.pre == PRE_TMUL)) {
+        const cpx<R>* tr = static_cast<const cpx<R>*>(P.ops.trans);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int x = t + m * T;
+            if (rowok && x >= ox && x < ox + nx) {
+                cpx<R> w;
+                w = at32(tr, r0 + x);
+                v[m] = cmul(v[m], w);
+            }
+        }
+    }
+    if (PK == PRE_MASK || (PK == PRE_MASKS && P.ops.pre == PRE_MASK)) {
+        const MaskDev& mk = P.mask;
+        if (mk.mode == MASK_SEPARABLE) {
+            MaskAxis Y;
+            Y = at32(mk.sy, rowok ? iy : 0);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int x = t + m * T;
+                if (rowok && x >= ox && x < ox + nx) {
+                    MaskAxis X;
+                    X = at32(mk.sx, x - ox);
+                    const double lv = bilinear(mk.values, mk.mnx, X.i0, X.i1, Y.i0, Y.i1, X.t, Y.t);
+                    const double phi = (X.in & Y.in) ? lv : 0.0;
+                    const cpx<R> o = cmul(v[m], unit_phase<R>(phi));
+                    v[m] = ap_contains(mk, X.r, Y.r) ? o : cpx<R>{R(0), R(0)};
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int x = t + m * T;
+                if (rowok && x >= ox && x < ox + nx) v[m] = apply_mask(mk, x - ox, iy, v[m]);
+            }
+        }
+    }
+}
+
 // PK: pre-op (PRE_NONE, PRE_MASK, PRE_TMUL, PRE_POLAR, PRE_MASKS for a mix of the
 // first three, or -1 for the descriptor's).
 template <typename R, int N, int V, int PK>
@@ -95,43 +143,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
             }
 #pragma unroll
             for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
-            if (PK == PRE_TMUL || (PK == PRE_MASKS && P.ops.pre == PRE_TMUL)) {
-                const cpx<R>* tr = static_cast<const cpx<R>*>(P.ops.trans);
-#pragma unroll
-                for (int m = 0; m < 16; ++m) {
-                    const int x = t + m * T;
-                    if (rowok && x >= ox && x < ox + nx) {
-                        cpx<R> w;
-                        w = at32(tr, r0 + x);
-                        v[m] = cmul(v[m], w);
-                    }
-                }
-            }
-            if (PK == PRE_MASK || (PK == PRE_MASKS && P.ops.pre == PRE_MASK)) {
-                const MaskDev& mk = P.mask;
-                if (mk.mode == MASK_SEPARABLE) {
-                    MaskAxis Y;
-                    Y = at32(mk.sy, rowok ? iy : 0);
-#pragma unroll
-                    for (int m = 0; m < 16; ++m) {
-                        const int x = t + m * T;
-                        if (rowok && x >= ox && x < ox + nx) {
-                            MaskAxis X;
-                            X = at32(mk.sx, x - ox);
-                            const double lv = bilinear(mk.values, mk.mnx, X.i0, X.i1, Y.i0, Y.i1, X.t, Y.t);
-                            const double phi = (X.in & Y.in) ? lv : 0.0;
-                            const cpx<R> o = cmul(v[m], unit_phase<R>(phi));
-                            v[m] = ap_contains(mk, X.r, Y.r) ? o : cpx<R>{R(0), R(0)};
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int m = 0; m < 16; ++m) {
-                        const int x = t + m * T;
-                        if (rowok && x >= ox && x < ox + nx) v[m] = apply_mask(mk, x - ox, iy, v[m]);
-                    }
-                }
-            }
+            row_pre<R, N, PK>(P, v, iy, rowok, t);
         } else if constexpr (PK == PRE_POLAR) {
             // GS source plane: amplitude and phase of the 16 samples as one
             // batch of loads, then from_polar (masked by the stop)
@@ -627,6 +639,70 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
 }
 
 /* ------------------------------------------------------------------ */
+/* Two consecutive propagations of a trace, linked through their rows.  */
+/* ------------------------------------------------------------------ */
+// Propagation k's inverse row pass and propagation k+1's forward row pass
+// on the same grid and padded length: per row, inverse row transform, crop
+// and scale (propagation k's output, written only where A.out is set), the
+// next step's pre-op PK (mask or shared transmission, from B), re-embed at
+// the same offset, forward row transform, blocked store into B's
+// intermediate for B's column pass.  Row tiles follow B (both batches
+// share the layout).
+template <typename R, int N, int V, int PK>
+__global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_link(const ProbDev* __restrict__ A,
+                                                                          const ProbDev* __restrict__ Bp,
+                                                                          int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, G = LY::G, Q = LY::B;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
+    const int t = tb + Q * tq;
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const int pi = find_problem<PASS_ROW>(Bp, nprob, tile);
+        const ProbDev& P = A[pi];
+        const ProbDev& Nx = Bp[pi];
+        const int iy = (tile - Nx.base_row) * G + g;
+        const bool rowok = iy < P.ny;
+        const int ny = P.ny, nx = P.nx, ox = P.ox;
+        const unsigned b0 = ((unsigned)tq * ny + iy) * Q + tb;
+        cpx<R> v[16];
+        const cpx<R>* work = static_cast<const cpx<R>*>(P.work);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            v[m] = {R(0), R(0)};
+            if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
+        }
+        const cpx<R>* tw = static_cast<const cpx<R>*>(P.prow.tw);
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw);
+        const R s = (R)P.scale;
+        cpx<R>* mid = static_cast<cpx<R>*>(P.out);
+        bool bad = false;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int x = t + m * T - ox;
+            const bool in = rowok && x >= 0 && x < nx;
+            const cpx<R> o = {v[m].re * s, v[m].im * s};
+            v[m] = in ? o : cpx<R>{R(0), R(0)};
+            bad |= in && !finite_c(o);
+            if (in && mid) at32(mid, (unsigned)iy * nx + x) = o;
+        }
+        if (bad) {
+            atomicOr(P.flag, 2);
+            atomicOr(Nx.flag, 1);
+        }
+        row_pre<R, N, PK>(Nx, v, iy, rowok, t);
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, tw);
+        if (rowok) {
+            cpx<R>* nw = static_cast<cpx<R>*>(Nx.work);
+#pragma unroll
+            for (int m = 0; m < 16; ++m) at32(nw, b0 + (unsigned)(m * (T / Q)) * ny * Q) = v[m];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* A propagation chained into a complex64 separable Fresnel.            */
 /* ------------------------------------------------------------------ */
 // The first propagation's inverse row pass and the Fresnel's row pass touch
@@ -872,6 +948,26 @@ static hipError_t launch_chain_nv(int which, const ProbDev* a, const ProbDev* f,
     return hipErrorInvalidValue;
 }
 
+// f_row_link over (a, b) with b's pre-op pk (PRE_NONE, PRE_MASK, PRE_TMUL,
+// PRE_MASKS); both precisions.
+template <int N, int V>
+static hipError_t launch_link_nv(int pk, const ProbDev* a, const ProbDev* b, int np, int total, int grid,
+                                 hipStream_t st) {
+    using R = Real;
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16;
+    const size_t lds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
+    const dim3 g(grid), blk(LY::G * T);
+    switch (pk) {
+        case PRE_NONE: hipLaunchKernelGGL((f_row_link<R, N, V, PRE_NONE>), g, blk, lds, st, a, b, np, total); break;
+        case PRE_MASK: hipLaunchKernelGGL((f_row_link<R, N, V, PRE_MASK>), g, blk, lds, st, a, b, np, total); break;
+        case PRE_TMUL: hipLaunchKernelGGL((f_row_link<R, N, V, PRE_TMUL>), g, blk, lds, st, a, b, np, total); break;
+        case PRE_MASKS: hipLaunchKernelGGL((f_row_link<R, N, V, PRE_MASKS>), g, blk, lds, st, a, b, np, total); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 static hipError_t allow_lds(const void* kernel) {
     return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 }
@@ -896,7 +992,11 @@ static hipError_t attr_nv() {
                        (const void*)f_row_inv<R, N, V, -1>,
                        (const void*)f_rs_kgen<R, N, V>,
                        (const void*)f_row_gs<R, N, V, true>,
-                       (const void*)f_row_gs<R, N, V, false>};
+                       (const void*)f_row_gs<R, N, V, false>,
+                       (const void*)f_row_link<R, N, V, PRE_NONE>,
+                       (const void*)f_row_link<R, N, V, PRE_MASK>,
+                       (const void*)f_row_link<R, N, V, PRE_TMUL>,
+                       (const void*)f_row_link<R, N, V, PRE_MASKS>};
     for (const void* f : k) {
         const hipError_t e = allow_lds(f);
         if (e != hipSuccess) return e;
@@ -937,6 +1037,27 @@ hipError_t OC_FAST_FN(fast_launch)(const FastSel& s, const ProbDev* d, int np, i
         }
         OC_FAST_SIZES_2(OC_C2)
 #undef OC_C2
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t OC_FAST_FN(fast_launch_link)(int pk, int n, int v, const ProbDev* a, const ProbDev* b, int np, int total,
+                                        int grid, hipStream_t st) {
+    switch (n) {
+#define OC_K1(NN) \
+    case NN: return launch_link_nv<NN, 0>(pk, a, b, np, total, grid, st);
+        OC_FAST_SIZES_1(OC_K1)
+#undef OC_K1
+#define OC_K2(NN)                                                              \
+    case NN:                                                                   \
+        switch (v) {                                                           \
+            case 1: return launch_link_nv<NN, 1>(pk, a, b, np, total, grid, st); \
+            case 2: return launch_link_nv<NN, 2>(pk, a, b, np, total, grid, st); \
+            case 3: return launch_link_nv<NN, 3>(pk, a, b, np, total, grid, st); \
+            default: return launch_link_nv<NN, 0>(pk, a, b, np, total, grid, st); \
+        }
+        OC_FAST_SIZES_2(OC_K2)
+#undef OC_K2
         default: return hipErrorInvalidValue;
     }
 }

@@ -41,6 +41,10 @@ hipError_t fast_launch_chain_c64(int which, int n, int v, const ocd::ProbDev* a,
                                  int total, int grid, hipStream_t st);
 hipError_t fast_launch_chain_c128(int which, int n, int v, const ocd::ProbDev* a, const ocd::ProbDev* f, int np,
                                   int total, int grid, hipStream_t st);
+hipError_t fast_launch_link_c64(int pk, int n, int v, const ocd::ProbDev* a, const ocd::ProbDev* b, int np,
+                                int total, int grid, hipStream_t st);
+hipError_t fast_launch_link_c128(int pk, int n, int v, const ocd::ProbDev* a, const ocd::ProbDev* b, int np,
+                                 int total, int grid, hipStream_t st);
 void fast_layout_c64(int n, int v, int* G, int* B, int* CB);
 void fast_layout_c128(int n, int v, int* G, int* B, int* CB);
 hipError_t fast_set_lds_limits_c64();

@@ -161,3 +161,54 @@ def test_render_errors(G):
     st, _ = G.render(designs, rcases.inputs_for(src, bands, [(0, 0)], rcl), d, d, bands,
                      T.AsmConfig.default(strict=0), rcl)
     assert st == T.CODE["OC_ERR_UNSUPPORTED"]
+
+
+@pytest.mark.parametrize("dtype", ["c64", "c128"])
+def test_linked_trace_equals_separate_batches(G, dtype):
+    """A render whose design has two gaps with a mask before each (mask,
+    ASM, mask, ASM) runs its two propagations linked through their rows
+    (the first one's inverse row pass, the second mask and the second one's
+    forward row pass in one kernel).  That is the same arithmetic as two
+    batch calls, so the intensity must match them bit for bit; the linked
+    form launches one forward row pass instead of two."""
+    import torch
+    cdt = torch.complex64 if dtype == "c64" else torch.complex128
+    n, d = 128, 4e-6
+    bands = [500e-9, 600e-9]
+    rng = np.random.default_rng(17)
+    p1, p2 = rng.uniform(0, 2 * np.pi, (n, n)), rng.uniform(0, 2 * np.pi, (n, n))
+    m1, k1 = G.dev_mask(p1, d, d)
+    m2, k2 = G.dev_mask(p2, d, d)
+    z1, z2 = 2e-3, 3e-3
+    designs = [[[(m1, z1, 1.0), (m2, z2, 1.0)]] for _ in bands]
+    src = W.gaussian_beam(n, d, 550e-9, 120e-6)
+    inputs = torch.as_tensor(np.broadcast_to(src, (len(bands), 1, 1, n, n)).copy()).to(cdt).cuda()
+    rc = T.RenderConfig.make()
+    cfg = T.AsmConfig.default(strict=0)
+    obs = torch.zeros((1, len(bands), 1, n, n), dtype=torch.float64, device="cuda")
+    ctx = G.Context(0)
+    ctx.enable_timing(True)
+    ctx.pass_timing(reset=True)
+    assert G.render_dev(designs, inputs, d, d, bands, cfg, rc, obs, ctx=ctx) == 0
+    tm = ctx.pass_timing()
+    say("linked render", dtype, {k: v[1] for k, v in tm.items() if v[1]})
+    assert tm["row_fwd"][1] == 1 and tm["col"][1] == 2
+    for b, lam in enumerate(bands):
+        u = inputs[b, 0, 0].clone()
+        mid, out = torch.empty_like(u), torch.empty_like(u)
+        for m, z, a_in, a_out in ((m1, z1, u, mid), (m2, z2, mid, out)):
+            x = (T.PropDesc * 1)()
+            x[0].geom = T.geom(n, n, d, d, lam)
+            x[0].engine = T.ENGINE_ASM
+            x[0].z, x[0].padding, x[0].band_limited, x[0].strict, x[0].max_padding, x[0].medium = (
+                z, cfg.padding, cfg.band_limited, 0, cfg.max_padding, 1.0)
+            x[0].mask = T.C.pointer(m)
+            x[0].in_, x[0].out = a_in.data_ptr(), a_out.data_ptr()
+            assert G.propagate_batch_dev(x, double=dtype == "c128", ctx=ctx) == 0
+        re, im = out.real.double(), out.imag.double()
+        inten = re * re + im * im  # as the observation kernel forms it
+        got = obs[0, b, 0]
+        e = float((got - inten).abs().max() / inten.abs().max())
+        say("   band", b, "linked vs two batch calls", e)
+        assert torch.equal(got, inten)
+    ctx.close()
