@@ -33,7 +33,8 @@ namespace ocd {
 //   CB columns per column-pass workgroup (CB*T threads, CB | B).
 // Thread caps per variant (rows / cols) and the run width:
 //   V0: 1024 / 1024, B = G    -- fewest, widest workgroups
-//   V1:  512 /  512, B = G    -- more registers per thread
+//   V1:  512 /  512, B = G    -- more registers per thread (complex128:
+//                                 half the LDS, two workgroups per CU)
 //   V2: 1024 /  512, B = G    -- wide row passes, two column workgroups/CU
 //   V3:  512 /  512, B = 2G   -- narrow row passes with V0's run width
 // each within the 140 KB LDS budget and at most 16 rows/columns.  The
@@ -43,7 +44,10 @@ struct FastLayout {
     static constexpr int T = N / 16;
     static constexpr int ESZ = (int)sizeof(cpx<R>);
     static constexpr int BYTES = Pow2Plan<N>::RSTRIDE * ESZ;  // per transform (the row layout's stride)
-    static constexpr int LG = 140 * 1024 / BYTES;
+    // LDS budget per workgroup: 140 KB; complex128 variant 1 takes 70 KB so
+    // that two workgroups share a CU
+    static constexpr int LDSB = (V == 1 && ESZ == 16) ? 70 * 1024 : 140 * 1024;
+    static constexpr int LG = LDSB / BYTES > 1 ? LDSB / BYTES : 1;
     static constexpr int ROWCAP = (V == 1 || V == 3) ? 512 : 1024;
     static constexpr int COLCAP = V == 0 ? 1024 : 512;
     static constexpr int BM = V == 3 ? 2 : 1;

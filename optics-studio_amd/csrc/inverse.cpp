@@ -425,50 +425,66 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
     void *cur, *nxt, *ckpt = nullptr, *sums, *flags;
     int st;
     if ((st = ensure_buf(ctx, "adj_cur", n * 16, &cur)) || (st = ensure_buf(ctx, "adj_nxt", n * 16, &nxt)) ||
-        (st = ensure_buf(ctx, "adj_sums", 64, &sums)) ||
+        (st = ensure_buf(ctx, "adj_sums", sizeof(double) * (8 + ocd::kSeedParts), &sums)) ||
         (st = ensure_buf(ctx, "adj_flags", sizeof(int) * (2 * nsteps + 1), &flags)))
         return st;
     if (ne && (st = ensure_buf(ctx, "adj_ckpt", ne * n * 16, &ckpt))) return st;
     // Every step is queued without waiting; the finiteness flags of all
-    // propagations and the merit are read back once at the end.
+    // propagations (zeroed once here) and the merit are read back once at
+    // the end.
     std::vector<std::vector<Prob>> queued;
     queued.reserve(2 * nsteps);
     int* dflags = static_cast<int*>(flags);
+    HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, (int)(2 * nsteps + 1), ctx->stream));
     auto propagate = [&](Prob p) -> int {
         queued.push_back({p});
-        int s2 = launch_problems(ctx, true, queued.back(), dflags + (queued.size() - 1));
+        int s2 = launch_problems(ctx, true, queued.back(), dflags + (queued.size() - 1), false);
         if (s2 == OC_OK) s2 = queued.back()[0].status;
         return s2;
     };
-    HIP_TRY(ctx, hipMemcpyAsync(cur, d_input, n * 16, hipMemcpyDeviceToDevice, ctx->stream));
-    // forward: store each element's incoming field, fuse the mask into the propagation
+    // forward, the mask fused into the propagation.  The field arriving at
+    // element e is its checkpoint: the input itself for a first-step
+    // element, else the previous step writes it straight into slot e.
+    // Other intermediates alternate between two scratch buffers.
+    std::vector<const void*> ck_at(ne, nullptr);
+    void* scratch[2] = {cur, nxt};
+    const void* src = d_input;
     size_t e = 0;
     for (size_t s = 0; s < nsteps; ++s) {
         const oc_design_step& stp = steps[s];
-        if (stp.has_element) {
-            HIP_TRY(ctx, hipMemcpyAsync(static_cast<unsigned char*>(ckpt) + e * n * 16, cur, n * 16,
-                                        hipMemcpyDeviceToDevice, ctx->stream));
-            ++e;
-        }
+        if (stp.has_element) ck_at[e++] = src;
+        const bool next_elem = s + 1 < nsteps && steps[s + 1].has_element;
+        void* dst = next_elem ? static_cast<unsigned char*>(ckpt) + e * n * 16
+                              : (src == scratch[0] ? scratch[1] : scratch[0]);
         if (stp.z_after != 0.0) {
             if (propagate(make_prob(&g, stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
-                                    cfg.max_padding, stp.medium, stp.has_element ? &stp.mask : nullptr, cur,
-                                    nxt)) != OC_OK)
+                                    cfg.max_padding, stp.medium, stp.has_element ? &stp.mask : nullptr, src,
+                                    dst)) != OC_OK)
                 return fail(ctx, OC_INV_ADJOINT, "forward propagation failed");
-            std::swap(cur, nxt);
+            src = dst;
         } else if (stp.has_element) {
             const int fs = mask_footprint(g, stp.mask);
             if (fs) return OC_INV_ADJOINT;
             MaskDev md;
             if ((st = mask_tables(ctx, g, stp.mask, stp.mask.values, &md))) return st;
-            HIP_TRY(ctx, ocl::launch_mask_apply_c128(cur, nxt, md, (int)g.nx, (int)g.ny, nullptr, ctx->stream));
-            std::swap(cur, nxt);
+            HIP_TRY(ctx, ocl::launch_mask_apply_c128(src, dst, md, (int)g.nx, (int)g.ny, nullptr, ctx->stream));
+            src = dst;
+        } else if (next_elem) {  // an empty step before an element: its field is that element's checkpoint
+            HIP_TRY(ctx, hipMemcpyAsync(dst, src, n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+            src = dst;
         }
     }
-    // seed at the detector
-    HIP_TRY(ctx, hipMemsetAsync(sums, 0, 64, ctx->stream));
-    if (dmdi) HIP_TRY(ctx, ocl::launch_seed(true, cur, dmdi, nxt, n, ctx->stream));
-    else HIP_TRY(ctx, ocl::launch_seed_mse(true, cur, target, nxt, n, static_cast<double*>(sums), ctx->stream));
+    // the detector field into a scratch buffer of its own for the sweep
+    cur = const_cast<void*>(src);
+    nxt = (src == scratch[0]) ? scratch[1] : scratch[0];
+    // seed at the detector (into a scratch buffer: the detector field may be
+    // the caller's input or a checkpoint)
+    if (dmdi) {
+        HIP_TRY(ctx, hipMemsetAsync(sums, 0, 64, ctx->stream));
+        HIP_TRY(ctx, ocl::launch_seed(true, cur, dmdi, nxt, n, ctx->stream));
+    } else {
+        HIP_TRY(ctx, ocl::launch_seed_mse(true, cur, target, nxt, n, static_cast<double*>(sums), ctx->stream));
+    }
     std::swap(cur, nxt);
     auto finish = [&]() -> int {
         double hs[4];
@@ -491,7 +507,7 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
     for (size_t s = nsteps; s-- > 0;) {
         const oc_design_step& stp = steps[s];
         if (stp.has_element) --e;
-        void* ck = stp.has_element ? static_cast<unsigned char*>(ckpt) + e * n * 16 : nullptr;
+        const void* ck = stp.has_element ? ck_at[e] : nullptr;
         double* gr = stp.has_element ? grads + e * n : nullptr;
         if (stp.z_after != 0.0) {
             Prob pb = make_prob(&g, -stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding, cfg.max_padding,

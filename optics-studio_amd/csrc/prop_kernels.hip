@@ -492,8 +492,9 @@ __global__ void k_mtf_dft(const double* __restrict__ sums, const double2* __rest
 }
 
 template <typename R>
-__global__ void k_seed_mse(const cpx<R>* __restrict__ u, const double* __restrict__ target,
-                           cpx<R>* __restrict__ out, size_t n, double nf, double* sums) {
+__global__ __launch_bounds__(256) void k_seed_mse(const cpx<R>* __restrict__ u, const double* __restrict__ target,
+                                                  cpx<R>* __restrict__ out, size_t n, double nf,
+                                                  double* __restrict__ part) {
     double acc = 0.0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const double re = u[i].re, im = u[i].im;
@@ -502,7 +503,30 @@ __global__ void k_seed_mse(const cpx<R>* __restrict__ u, const double* __restric
         const double g = 2.0 * d / nf;
         out[i] = {(R)(g * re), (R)(g * im)};
     }
-    accumulate4(acc, 0.0, 0.0, 0.0, sums);
+    __shared__ double sm[256];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// sums[0] = the np workgroup partials added in a fixed order (thread t adds
+// t, t + 256, ... in order, then an LDS tree).
+__global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int np,
+                                                   double* __restrict__ sums) {
+    double s = 0.0;
+    for (int c = threadIdx.x; c < np; c += 256) s += part[c];
+    __shared__ double sm[256];
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[0] = sm[0];
 }
 
 __global__ void k_fill_int(int* __restrict__ p, int v, int n) {
@@ -660,16 +684,17 @@ hipError_t launch_block_reorder(bool dbl, const void* src, void* dst, int nx, in
 
 hipError_t launch_seed_mse(bool dbl, const void* u, const double* target, void* out, size_t n, double* sums,
                            hipStream_t st) {
-    // few enough workgroups that the per-wave atomics into one accumulator
-    // do not serialise (2048 waves)
-    int grid = (int)std::min<size_t>((n + 255) / 256, 512);
+    // a fixed grid (for a given n) of workgroup partials at sums + 8, added
+    // in order into sums[0]: the merit does not depend on scheduling
+    int grid = (int)std::min<size_t>((n + 255) / 256, kSeedParts);
     if (grid < 1) grid = 1;
     if (dbl)
         hipLaunchKernelGGL(k_seed_mse<double>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<double>*>(u),
-                           target, static_cast<cpx<double>*>(out), n, (double)n, sums);
+                           target, static_cast<cpx<double>*>(out), n, (double)n, sums + 8);
     else
         hipLaunchKernelGGL(k_seed_mse<float>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<float>*>(u),
-                           target, static_cast<cpx<float>*>(out), n, (double)n, sums);
+                           target, static_cast<cpx<float>*>(out), n, (double)n, sums + 8);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, sums + 8, grid, sums);
     return hipGetLastError();
 }
 

@@ -29,6 +29,9 @@ enum PreOp : int { PRE_NONE = 0, PRE_MASK = 1, PRE_POLAR = 2, PRE_SEED_MSE = 3, 
 // Kernel selector (never a descriptor's op): a launch whose problems mix
 // PRE_NONE, PRE_MASK and PRE_TMUL; each tile takes its problem's branch.
 constexpr int PRE_MASKS = 6;
+// Workgroup partials of the MseToTarget seed's merit sum (launch_seed_mse
+// keeps them after the first 8 doubles of its sums buffer).
+constexpr int kSeedParts = 1024;
 // Element-wise op fused into the last pass's (cropped) stores.
 enum PostOp : int { POST_NONE = 0, POST_GS_TARGET = 1, POST_GS_PHASE = 2, POST_ADJ_GRAD = 3 };
 enum MaskMode : int { MASK_NONE = 0, MASK_SEPARABLE = 1, MASK_GENERAL = 2 };

@@ -789,7 +789,7 @@ void evict_tables(oc_context* ctx) {
 // Queues all problems on the context stream (host-side validation, the
 // standalone finiteness checks, then one batch per padded-size group) with
 // their finiteness flags in dflags[0..n).  No synchronisation.
-int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags) {
+int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags, bool zero_flags) {
     evict_tables(ctx);
     for (auto& p : probs) {
         prepare(p);
@@ -799,7 +799,7 @@ int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* df
         }
     }
     const int np = (int)probs.size();
-    HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, np, ctx->stream));
+    if (zero_flags) HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, np, ctx->stream));
     for (int i = 0; i < np; ++i) {
         Prob& p = probs[i];
         if (p.status != OC_OK || !p.check_only) continue;

@@ -70,6 +70,7 @@ hipError_t launch_seed(bool dbl, const void* u, const double* dmdi, void* out, s
 hipError_t launch_adj_elem(bool dbl, void* co, const void* ckpt, double* grad, const ocd::MaskDev& m, int nx, int ny,
                            hipStream_t st);
 hipError_t launch_block_reorder(bool dbl, const void* src, void* dst, int nx, int ny, int B, hipStream_t st);
+// sums: 8 + ocd::kSeedParts doubles; sums[0] receives the merit sum.
 hipError_t launch_seed_mse(bool dbl, const void* u, const double* target, void* out, size_t n, double* sums,
                            hipStream_t st);
 hipError_t launch_render_obs(bool dbl, const void* u, double* obs, size_t n, int frames, int nr, int cx, int cy,
@@ -291,7 +292,8 @@ constexpr int PASS_GS_AMP = 5;
 constexpr int PASS_T_SEP_COL = 6, PASS_T_SEP_ROW = 7;
 int launch_batch(oc_context* ctx, Batch& b, unsigned mask = kAllPasses);
 void evict_tables(oc_context* ctx);
-int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags);
+// zero_flags = false: the caller has zeroed dflags[0..probs.size()) on the stream
+int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags, bool zero_flags = true);
 int finish_problems(oc_context* ctx, std::vector<Prob>& probs, const int* dflags);
 int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs);
 Prob make_prob(const oc_band_geom* g, double z, int engine, int band_limited, int strict, double padding,
