@@ -508,7 +508,9 @@ def main():
     torch.cuda.synchronize()
     copy_gbps = 2 * (1 << 30) / (ev0.elapsed_time(ev1) / 10 * 1e-3) / 1e9
     del ca, cb
-    step_bytes = sum(alg[k] * timing[k][1] for k in alg) / nbreak + nb * n * n * 8  # + f64 mask lattice per band
+    # + the shared DOE: k_mask_trans reads the f64 lattice and writes the
+    # complex64 transmission once, and row_fwd reads it once (L2 across bands)
+    step_bytes = sum(alg[k] * timing[k][1] for k in alg) / nbreak + 3 * n * n * 8
     extra = {"passes": passes, "fft_variant": variant, "autotune_ms": variant_ms,
              "fresnel_sep_layout": list(sep_layout), "step_alg_GB": step_bytes / 1e9,
              "device_copy_GBps": copy_gbps,
