@@ -462,6 +462,19 @@ void prepare(Prob& p) {
         p.nxp = r.nx_pad;
         p.nyp = r.ny_pad;
     }
+    if (conv_engine(p.engine) && p.nxp + 1 >= 2 * g.nx && p.nyp + 1 >= 2 * g.ny) {
+        // Sampled-kernel convolution with P >= 2N - 1 on both axes: every
+        // cropped output sample is sum_m u[m] h((n - m) d) over offsets
+        // |n - m| < N, which the DFT-order sampling places at the same
+        // coordinates for any such P, and no term wraps around.  The result
+        // is the same linear convolution for every padded length >= 2N - 1,
+        // so a non-power-of-two P runs at the next power of two (the
+        // register-resident kernels) up to 8192, square.  The report keeps
+        // the reference's padded size.
+        size_t q = 64;
+        while (q < std::max(p.nxp, p.nyp)) q <<= 1;
+        if (q <= 8192 && (q != p.nxp || q != p.nyp)) p.nxp = p.nyp = q;
+    }
     p.launch = true;
 }
 

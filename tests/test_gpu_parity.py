@@ -259,3 +259,34 @@ def test_fresnel_convolution_parity(G, c64, c128, orc, shape):
         e = rel_l2(out, ref)
         say("  ", name, e)
         assert e <= tol, (name, e)
+
+
+@pytest.mark.parametrize("pad", [2.0, 1.5, 2.5])
+def test_convolution_padding_is_linear_window(G, c64, c128, orc, pad):
+    """RS and the Fresnel convolution with P >= 2N - 1 on both axes compute a
+    linear convolution whose cropped window does not depend on P; such
+    non-power-of-two P run at the next power of two.  Against the oracle at
+    the reference's P: padding 2 and 2.5 on a 384 x 300 grid (P = 768 x 600
+    and 960 x 750, run at 1024), and padding 1.5 (P < 2N - 1, wrap-around
+    present: run at P itself).  The report keeps the reference's P."""
+    shape = (300, 384)
+    u = rand_field(shape, 77)
+    d, lam = 3e-6, 600e-9
+    for kind in ("rs", "fconv"):
+        if kind == "rs":
+            cfg = T.RsConfig(pad, 0, 0)
+            st, ref, rref = orc.rs_propagate(u, d, d, lam, 5e-3, cfg)
+        else:
+            cfg = T.FresnelConfig(pad, 0, 0, pad)
+            st, ref, rref = orc.fresnel_propagate_convolution(u, d, d, lam, 8e-3, cfg)
+        assert st == 0
+        for name, eng, tol in (("f64", G, F64_TOL), ("c128", c128, F64_TOL), ("c64", c64, C64_TOL)):
+            if kind == "rs":
+                st, out, rep = eng.rs_propagate(u, d, d, lam, 5e-3, cfg)
+            else:
+                st, out, rep = eng.fresnel_propagate_convolution(u, d, d, lam, 8e-3, cfg)
+            assert st == 0, (kind, name, T.STATUS[st])
+            assert T.report_dict(rep) == T.report_dict(rref)
+            e = rel_l2(out, ref)
+            say("  ", kind, pad, name, e)
+            assert e <= tol, (kind, pad, name, e)
