@@ -222,11 +222,16 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
         }
         fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
         if constexpr (RS) {
+            // kernel row y is row min(y, N - y) of the stored half
+            constexpr int KR = N / 2 + 1;
             const cpx<R>* kb = static_cast<const cpx<R>*>(P.kbuf);
-            const unsigned k0 = ((unsigned)q * N + t) * B + bq;
+            const unsigned k0 = (unsigned)q * KR * B + bq;
             cpx<R> w[16];
 #pragma unroll
-            for (int m = 0; m < 16; ++m) w[m] = at32(kb, k0 + (unsigned)(m * T * B));
+            for (int m = 0; m < 16; ++m) {
+                const int y = t + m * T;
+                w[m] = at32(kb, k0 + (unsigned)min(y, N - y) * B);
+            }
             fft_regs<R, N, CB, -1>(w, lds, g, t, tw);
 #pragma unroll
             for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], w[m]);
@@ -476,6 +481,12 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(
     }
 }
 
+// Row spectra of the sampled convolution kernel (RS or Fresnel
+// convolution).  The kernel depends on y only through y^2 and the DFT-order
+// coordinates of rows y and N - y are negatives of each other exactly, so
+// those rows are equal: only rows 0..N/2 are sampled and transformed, and
+// stored blocked [N/B][N/2 + 1][B]; the column pass reads row y from row
+// min(y, N - y).
 template <typename R, int N, int V>
 __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen(const ProbDev* __restrict__ probs,
                                                                          int nprob, int total) {
@@ -486,10 +497,11 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
+    constexpr int KR = N / 2 + 1;  // distinct kernel rows
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_KGEN>(probs, nprob, tile)];
         const int y = (tile - P.base_kgen) * G + g;
-        const bool rowok = y < N;
+        const bool rowok = y < KR;
         cpx<R> v[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m)
@@ -497,9 +509,9 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
         fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
         if (rowok) {
             cpx<R>* kb = static_cast<cpx<R>*>(P.kbuf);
-            const unsigned k0 = ((unsigned)tq * N + y) * Q + tb;
+            const unsigned k0 = ((unsigned)tq * KR + y) * Q + tb;
 #pragma unroll
-            for (int m = 0; m < 16; ++m) at32(kb, k0 + (unsigned)(m * (T / Q)) * N * Q) = v[m];
+            for (int m = 0; m < 16; ++m) at32(kb, k0 + (unsigned)(m * (T / Q)) * KR * Q) = v[m];
         }
     }
 }

@@ -653,7 +653,8 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         d.rows_cnt = L.cnt;
         d.tiles_row = (d.ny + L.cnt - 1) / L.cnt;
         d.tiles_col = b.sep ? (d.nx + L.ccnt - 1) / L.ccnt : d.px / L.ccnt;
-        d.tiles_kgen = rs ? (d.py + L.cnt - 1) / L.cnt : 0;
+        // the power-of-two kernels transform only the distinct kernel rows 0..P/2
+        d.tiles_kgen = rs ? ((b.fast ? d.py / 2 + 1 : d.py) + L.cnt - 1) / L.cnt : 0;
         d.base_row = base_row;
         d.base_col = base_col;
         d.base_kgen = base_kgen;
