@@ -397,8 +397,9 @@ static int submit(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
 // every problem is a complex64 ASM at a power-of-two padded length followed
 // by a Fresnel on the same padded grid, the ASM's inverse row pass and the
 // Fresnel's row pass run as one kernel and the Fresnel's column pass reads
-// the blocked intermediate (f_row_chain, f_sep_col_fin); otherwise the two
-// stages run as two batches.
+// the blocked intermediate (f_row_chain, f_sep_col_fin).  Three-pass chains
+// (complex128, or complex64 ASM into ASM) at one power-of-two length run
+// linked through their rows (f_row_link); all others as two batches.
 static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, size_t count) {
     if (!ctx || ((!a || !f) && count)) return OC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
@@ -430,6 +431,22 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
     st = ensure_buf(ctx, "chain_flags", sizeof(int) * 2 * count, &fl);
     if (st) return st;
     int* dflags = static_cast<int*>(fl);
+    // per-problem statuses of a one-launch chain (all = first stages, then
+    // second stages, flags at dflags): a failed first stage fails the chain
+    auto resolve = [&](std::vector<Prob>& all) -> int {
+        const int fs = finish_problems(ctx, all, dflags);
+        if (fs == OC_ERR_DEVICE || fs == OC_ERR_OUT_OF_MEMORY) return fs;  // flags never read back
+        int first = OC_OK;
+        for (int i = 0; i < np; ++i) {
+            const int sa = all[i].status;
+            const int sf = sa != OC_OK ? sa : all[np + i].status;
+            a[i].status = sa;
+            f[i].status = sf;
+            if (first == OC_OK && sa != OC_OK) first = sa;
+            if (first == OC_OK && sf != OC_OK) first = sf;
+        }
+        return first;
+    };
     if (fuse) {
         for (int i = 0; i < np; ++i) qf[i].in = qa[i].out;  // unused by the fused passes
         HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, 2 * np, ctx->stream));
@@ -465,20 +482,22 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
             if (e != hipSuccess) return fail(ctx, OC_ERR_DEVICE, std::string("chain launch: ") + hipGetErrorString(e));
             std::vector<Prob> all = qa;
             all.insert(all.end(), qf.begin(), qf.end());
-            const int fs = finish_problems(ctx, all, dflags);
-            if (fs == OC_ERR_DEVICE || fs == OC_ERR_OUT_OF_MEMORY) return fs;  // flags never read back
-            int first = OC_OK;
-            for (int i = 0; i < np; ++i) {
-                const int sa = all[i].status;
-                const int sf = sa != OC_OK ? sa : all[np + i].status;  // a failed first stage fails the chain
-                a[i].status = sa;
-                f[i].status = sf;
-                if (first == OC_OK && sa != OC_OK) first = sa;
-                if (first == OC_OK && sf != OC_OK) first = sf;
-            }
-            return first;
+            return resolve(all);
         }
         // the batches were only built (and any shared transmission computed): fall through
+    }
+    // three-pass transfer-function engines at one power-of-two length: the
+    // two steps linked through their rows (f_row_link), the intermediate
+    // written where the caller asked for it
+    if (!ctx->force_generic) {
+        std::vector<std::vector<Prob>> run{pa, pf}, done;
+        const int sl = launch_linked(ctx, dbl, run, dflags, std::vector<int>{0, np}, done, true);
+        if (sl < 0) return -sl;
+        if (sl == 0) {
+            std::vector<Prob> all = done[0];
+            all.insert(all.end(), done[1].begin(), done[1].end());
+            return resolve(all);
+        }
     }
     // two batches: the first stage into the caller's buffer or a scratch one
     size_t need = 0;

@@ -845,6 +845,69 @@ int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* df
     return OC_OK;
 }
 
+// Consecutive propagations (every problem of step k reads step k-1's
+// output, the next pre-op at most a mask or shared transmission; transfer
+// function engines, three-pass form) on one power-of-two padded length:
+// step 0's forward row and column passes,
+// then per later step f_row_link (step k-1's inverse rows, the pre-op, step
+// k's forward rows) and step k's column pass, then the last step's inverse
+// rows.  The fields between the steps are written only with keep_mid, to
+// each problem's own output pointer where it is set.  Returns 0 when
+// launched (the steps' problems appended to `queued`, flags at dflags +
+// off[k]), 1 when the run does not qualify (nothing launched), or minus an
+// error status.
+int launch_linked(oc_context* ctx, bool dbl, std::vector<std::vector<Prob>>& run, int* dflags,
+                  const std::vector<int>& off, std::vector<std::vector<Prob>>& queued, bool keep_mid) {
+    const size_t K = run.size();
+    std::vector<std::vector<Prob>> q = run;  // prepared copies
+    size_t P = 0;
+    for (size_t k = 0; k < K; ++k)
+        for (Prob& p : q[k]) {
+            prepare(p);
+            if (p.status == OC_OK && p.mask) p.status = mask_footprint(p.g, *p.mask);
+            if (p.status != OC_OK || !p.launch || p.check_only || (p.engine != ENG_ASM && p.engine != ENG_FRESNEL) ||
+                p.nxp != p.nyp || ocl::fast_variants((int)p.nxp) == 0 || (P && p.nxp != P))
+                return 1;
+            P = p.nxp;
+            if (k + 1 < K && !keep_mid) p.out = nullptr;  // stays on chip
+        }
+    if (ctx->force_generic) return 1;
+    std::vector<Batch> b(K);
+    for (size_t k = 0; k < K; ++k) {
+        int st = build_batch(ctx, dbl, q[k], dflags + off[k], b[k], false, nullptr, "ln" + std::to_string(k));
+        if (st) return -st;
+        const Batch& x = b[k];
+        const bool pre_ok = k == 0 || x.pre_kind == PRE_NONE || x.pre_kind == PRE_MASK || x.pre_kind == PRE_TMUL ||
+                            x.pre_kind == PRE_MASKS;
+        if (!x.fast || x.sep || x.rs || x.post_kind != POST_NONE || (x.col_kind != ENG_ASM && x.col_kind != ENG_FRESNEL) ||
+            !pre_ok ||
+            x.pd.size() != q[k].size() || x.fast_v != b[0].fast_v || x.pd[0].B != b[0].pd[0].B)
+            return 1;
+    }
+    const hipError_t ef = ocl::launch_fill_int(dflags + off[0], 0, (int)(K * q[0].size()), ctx->stream);
+    if (ef != hipSuccess) return -fail(ctx, OC_ERR_DEVICE, std::string("flags: ") + hipGetErrorString(ef));
+    int st = launch_batch(ctx, b[0], (1u << 0) | (1u << 1));
+    for (size_t k = 1; k < K && !st; ++k) {
+        const int np = (int)b[k].pd.size(), rows = b[k].pc[0].total;
+        const ProbDev* da = static_cast<const ProbDev*>(b[k - 1].dprobs);
+        const ProbDev* db = static_cast<const ProbDev*>(b[k].dprobs);
+        hipError_t e = timed(ctx, 2, [&] {
+            const int grid = std::min(rows, ctx->num_cu * 8);
+            return dbl ? ocl::fast_launch_link_c128(b[k].pre_kind, b[k].fast_n, b[k].fast_v, da, db, np, rows, grid,
+                                                    ctx->stream)
+                       : ocl::fast_launch_link_c64(b[k].pre_kind, b[k].fast_n, b[k].fast_v, da, db, np, rows, grid,
+                                                   ctx->stream);
+        });
+        if (e != hipSuccess) return -fail(ctx, OC_ERR_DEVICE, std::string("link launch: ") + hipGetErrorString(e));
+        st = launch_batch(ctx, b[k], 1u << 1);
+    }
+    if (!st) st = launch_batch(ctx, b[K - 1], 1u << 2);
+    if (st) return -st;
+    for (size_t k = 0; k < K; ++k) queued.push_back(std::move(q[k]));
+    run.clear();
+    return 0;
+}
+
 // Waits for the queued work and turns the flags of launch_problems into
 // per-problem statuses; returns the first non-OK one.
 int finish_problems(oc_context* ctx, std::vector<Prob>& probs, const int* dflags) {

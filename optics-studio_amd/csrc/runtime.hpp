@@ -295,6 +295,10 @@ void evict_tables(oc_context* ctx);
 // zero_flags = false: the caller has zeroed dflags[0..probs.size()) on the stream
 int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags, bool zero_flags = true);
 int finish_problems(oc_context* ctx, std::vector<Prob>& probs, const int* dflags);
+// Consecutive propagation steps linked through their rows (f_row_link);
+// 0 = launched, 1 = the run does not qualify (nothing launched), < 0 = -error.
+int launch_linked(oc_context* ctx, bool dbl, std::vector<std::vector<Prob>>& run, int* dflags,
+                  const std::vector<int>& off, std::vector<std::vector<Prob>>& queued, bool keep_mid);
 int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs);
 Prob make_prob(const oc_band_geom* g, double z, int engine, int band_limited, int strict, double padding,
                double max_padding, double medium, const oc_phase_mask* mask, const void* in, void* out);

@@ -3,8 +3,9 @@ output, as two gaps of Beamline::trace (src/beamline.rs:109-130).  The fused
 complex64 form (masked ASM into Fresnel at a power-of-two padded length)
 must give the ASM intermediate bit for bit as the ASM batch does, and the
 final field within the complex64 bar of the two batch calls and of the f64
-oracle; every other chain runs as the two batch calls and equals them bit
-for bit.  Statuses: a failing first stage repeats its status on the second."""
+oracle; every other chain equals the two batch calls bit for bit (complex128
+runs its two three-pass steps linked through their rows, the rest as two
+batches).  Statuses: a failing first stage repeats its status on the second."""
 import sys
 
 import numpy as np
@@ -150,7 +151,11 @@ def test_unfused_chains_equal_two_batches(G, case):
     first = descs(T.ENGINE_ASM, lams, n, d, u, mid, z1, mask)
     second = descs(T.ENGINE_FRESNEL, lams, n, d, None, out, z2, m2)
     assert G.propagate_chain_dev(first, second, double=dbl, ctx=ctx) == 0
-    say("unfused chain", case, {k: v[1] for k, v in ctx.pass_timing().items()})
+    tm = ctx.pass_timing()
+    say("unfused chain", case, {k: v[1] for k, v in tm.items()})
+    # complex128 (three-pass Fresnel): the steps run linked through their
+    # rows, one forward row pass; the others as two batch calls
+    assert tm["row_fwd"][1] == (1 if case == "c128" else 2 if case != "masked_second" else 1)
     for b in range(len(lams)):
         assert torch.equal(mid[b], mid_ref[b]) and torch.equal(out[b], out_ref[b]), (case, b)
     # without the intermediate: a scratch buffer carries it
