@@ -450,6 +450,23 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
     void* scratch[2] = {cur, nxt};
     const void* src = d_input;
     size_t e = 0;
+    // consecutive forward propagations are launched together, linked
+    // through their rows when they qualify (each checkpoint still written)
+    std::vector<std::vector<Prob>> run;
+    auto flush = [&]() -> int {
+        if (run.empty()) return OC_OK;
+        std::vector<int> off;
+        for (size_t k = 0; k < run.size(); ++k) off.push_back((int)(queued.size() + k));
+        int sl = run.size() > 1 ? launch_linked(ctx, true, run, dflags, off, queued, true) : 1;
+        if (sl < 0) return -sl;
+        if (sl > 0)
+            for (auto& r : run)
+                if (propagate(r[0]) != OC_OK) return fail(ctx, OC_INV_ADJOINT, "forward propagation failed");
+        for (size_t k = queued.size() - run.size(); k < queued.size(); ++k)
+            if (queued[k][0].status != OC_OK) return fail(ctx, OC_INV_ADJOINT, "forward propagation failed");
+        run.clear();
+        return OC_OK;
+    };
     for (size_t s = 0; s < nsteps; ++s) {
         const oc_design_step& stp = steps[s];
         if (stp.has_element) ck_at[e++] = src;
@@ -457,12 +474,13 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
         void* dst = next_elem ? static_cast<unsigned char*>(ckpt) + e * n * 16
                               : (src == scratch[0] ? scratch[1] : scratch[0]);
         if (stp.z_after != 0.0) {
-            if (propagate(make_prob(&g, stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
-                                    cfg.max_padding, stp.medium, stp.has_element ? &stp.mask : nullptr, src,
-                                    dst)) != OC_OK)
-                return fail(ctx, OC_INV_ADJOINT, "forward propagation failed");
+            run.push_back({make_prob(&g, stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
+                                     cfg.max_padding, stp.medium, stp.has_element ? &stp.mask : nullptr, src, dst)});
             src = dst;
-        } else if (stp.has_element) {
+            continue;
+        }
+        if ((st = flush())) return st;
+        if (stp.has_element) {
             const int fs = mask_footprint(g, stp.mask);
             if (fs) return OC_INV_ADJOINT;
             MaskDev md;
@@ -474,6 +492,7 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
             src = dst;
         }
     }
+    if ((st = flush())) return st;
     // the detector field into a scratch buffer of its own for the sweep
     cur = const_cast<void*>(src);
     nxt = (src == scratch[0]) ? scratch[1] : scratch[0];
