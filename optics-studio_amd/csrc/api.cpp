@@ -70,8 +70,11 @@ void oc_context_destroy(oc_context* ctx) {
     for (auto& kv : ctx->tables) (void)hipFree(kv.second.dev);
     collect_timing(ctx);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
-    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
-    if (ctx->pinned_done) (void)hipEventDestroy(ctx->pinned_done);
+    for (int k = 0; k < oc_context::kPinSlots; ++k) {
+        if (ctx->pinned_done[k]) (void)hipEventSynchronize(ctx->pinned_done[k]);
+        if (ctx->pinned[k]) (void)hipHostFree(ctx->pinned[k]);
+        if (ctx->pinned_done[k]) (void)hipEventDestroy(ctx->pinned_done[k]);
+    }
     release_queued(ctx);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;

@@ -59,17 +59,26 @@ int ensure_buf(oc_context* ctx, const std::string& name, size_t bytes, void** ou
     return OC_OK;
 }
 
-int ensure_pinned(oc_context* ctx, size_t bytes) {
-    if (ctx->pinned_cap >= bytes) return OC_OK;
-    if (ctx->pinned) {
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        HIP_TRY(ctx, hipHostFree(ctx->pinned));
-        ctx->pinned = nullptr;
-        ctx->pinned_cap = 0;
+int ensure_pinned(oc_context* ctx, int slot, size_t bytes) {
+    if (ctx->pinned_cap[slot] >= bytes) return OC_OK;
+    if (!ctx->pinned[slot] && bytes <= (size_t)(1 << 18)) {
+        // first use: every empty slot at once (pinned allocation is slow, and
+        // the ring would otherwise allocate on the first kPinSlots uploads)
+        for (int k = 0; k < oc_context::kPinSlots; ++k)
+            if (!ctx->pinned[k]) {
+                HIP_TRY(ctx, hipHostMalloc(&ctx->pinned[k], (size_t)1 << 18, hipHostMallocDefault));
+                ctx->pinned_cap[k] = (size_t)1 << 18;
+            }
+        return OC_OK;
     }
-    size_t cap = std::max<size_t>(bytes, 1 << 16);
-    HIP_TRY(ctx, hipHostMalloc(&ctx->pinned, cap, hipHostMallocDefault));
-    ctx->pinned_cap = cap;
+    if (ctx->pinned[slot]) {
+        HIP_TRY(ctx, hipHostFree(ctx->pinned[slot]));
+        ctx->pinned[slot] = nullptr;
+        ctx->pinned_cap[slot] = 0;
+    }
+    size_t cap = std::max<size_t>(bytes, (size_t)1 << 18);
+    HIP_TRY(ctx, hipHostMalloc(&ctx->pinned[slot], cap, hipHostMallocDefault));
+    ctx->pinned_cap[slot] = cap;
     return OC_OK;
 }
 
@@ -746,15 +755,17 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     if (persistent) {
         HIP_TRY(ctx, hipMemcpy(b.dprobs, b.pd.data(), bytes, hipMemcpyHostToDevice));
     } else {
-        st = ensure_pinned(ctx, bytes);
+        // the slot is free once its previous copy ran (not the kernels
+        // queued after that copy)
+        const int slot = ctx->pin_next;
+        ctx->pin_next = (slot + 1) % oc_context::kPinSlots;
+        if (ctx->pinned_done[slot]) HIP_TRY(ctx, hipEventSynchronize(ctx->pinned_done[slot]));
+        else HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->pinned_done[slot], hipEventDisableTiming));
+        st = ensure_pinned(ctx, slot, bytes);
         if (st) return st;
-        // the staging buffer is free once the previous copy out of it ran
-        // (not the kernels queued after that copy)
-        if (ctx->pinned_done) HIP_TRY(ctx, hipEventSynchronize(ctx->pinned_done));
-        else HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->pinned_done, hipEventDisableTiming));
-        std::memcpy(ctx->pinned, b.pd.data(), bytes);
-        HIP_TRY(ctx, hipMemcpyAsync(b.dprobs, ctx->pinned, bytes, hipMemcpyHostToDevice, ctx->stream));
-        HIP_TRY(ctx, hipEventRecord(ctx->pinned_done, ctx->stream));
+        std::memcpy(ctx->pinned[slot], b.pd.data(), bytes);
+        HIP_TRY(ctx, hipMemcpyAsync(b.dprobs, ctx->pinned[slot], bytes, hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->pinned_done[slot], ctx->stream));
     }
     return OC_OK;
 }

@@ -112,9 +112,14 @@ struct oc_context {
     std::map<std::pair<int, int>, void*> twiddles;  // (n, double?) -> table
     std::map<std::string, ocr::TableEntry> tables;       // per-parameter axis tables
     size_t tables_bytes = 0;
-    void* pinned = nullptr;  // host staging for descriptors / flags
-    size_t pinned_cap = 0;
-    hipEvent_t pinned_done = nullptr;  // recorded after the last copy out of `pinned`
+    // host staging for descriptor uploads: a ring of slots, each with the
+    // event recorded after the last copy out of it, so that building a
+    // batch waits only on the copy kPinSlots uploads back
+    static constexpr int kPinSlots = 8;
+    void* pinned[kPinSlots] = {};
+    size_t pinned_cap[kPinSlots] = {};
+    hipEvent_t pinned_done[kPinSlots] = {};
+    int pin_next = 0;
     // per-pass timing
     bool timing = false;
     uint32_t timing_mask = 0xFFFFFFFFu;  // pass kinds that get timing brackets
@@ -192,7 +197,9 @@ void collect_timing(oc_context* ctx);
     } while (0)
 
 int ensure_buf(oc_context* ctx, const std::string& name, size_t bytes, void** out);
-int ensure_pinned(oc_context* ctx, size_t bytes);
+// Staging slot `slot` with room for `bytes` (the caller has waited for the
+// slot's previous copy).
+int ensure_pinned(oc_context* ctx, int slot, size_t bytes);
 int get_twiddles(oc_context* ctx, int n, bool dbl, const void** out);
 bool make_radices(int n, std::vector<int>& rad, bool& generic);
 int make_plan(oc_context* ctx, int n, bool dbl, FftPlanDev* pl);
