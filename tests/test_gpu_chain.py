@@ -210,3 +210,15 @@ def test_chain_statuses(G, fused):
     bad = descs(T.ENGINE_FRESNEL, lams, n, d * 2, None, out, 0.03)
     assert G.propagate_chain_dev(first, bad, double=not fused, ctx=ctx) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
     ctx.close()
+
+
+def test_chain_argument_checks(G):
+    """count 0 is a no-op; missing arrays with a count are refused."""
+    ctx = G.Context(0)
+    empty = (T.PropDesc * 1)()
+    assert G.lib.oc_propagate_chain_dev_c64(ctx.h, empty, empty, 0) == 0
+    assert G.lib.oc_propagate_chain_dev_c128(ctx.h, None, None, 0) == 0
+    assert G.lib.oc_propagate_chain_dev_c64(ctx.h, None, empty, 1) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
+    assert G.lib.oc_propagate_chain_dev_c64(ctx.h, empty, None, 1) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
+    assert G.lib.oc_propagate_chain_dev_c64(None, empty, empty, 1) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
+    ctx.close()
