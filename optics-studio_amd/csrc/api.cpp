@@ -473,13 +473,14 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
             const ProbDev* da = static_cast<const ProbDev*>(ba.dprobs);
             const ProbDev* df = static_cast<const ProbDev*>(bf.dprobs);
             const int rows = bf.pc[2].total, cols = bf.pc[1].total;
+            const int hw = (ba.half && bf.half) ? 2 : 0;  // the kernels for N = P/2
             hipError_t e = timed(ctx, PASS_T_SEP_ROW, [&] {
-                return ocl::fast_launch_chain_c64(0, bf.fast_n, bf.fast_v_row, da, df, np, rows,
+                return ocl::fast_launch_chain_c64(hw + 0, bf.fast_n, bf.fast_v_row, da, df, np, rows,
                                                   std::min(rows, ctx->num_cu * 8), ctx->stream);
             });
             if (e == hipSuccess)
                 e = timed(ctx, PASS_T_SEP_COL, [&] {
-                    return ocl::fast_launch_chain_c64(1, bf.fast_n, bf.fast_v, nullptr, df, np, cols,
+                    return ocl::fast_launch_chain_c64(hw + 1, bf.fast_n, bf.fast_v, nullptr, df, np, cols,
                                                       std::min(cols, ctx->num_cu * 8), ctx->stream);
                 });
             if (e != hipSuccess) return fail(ctx, OC_ERR_DEVICE, std::string("chain launch: ") + hipGetErrorString(e));

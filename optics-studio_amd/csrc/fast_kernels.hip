@@ -195,7 +195,11 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
 
 // EK: engine (ENG_ASM, ENG_FRESNEL, ENG_GS_PROJECT, or -1 for the descriptor's); RS: the
 // Rayleigh-Sommerfeld product with the kernel spectrum instead.
-template <typename R, int N, int V, bool RS, int EK>
+// HW: every problem is N = P/2 on both axes, so the rows held are exactly
+// m = 4..11 of each thread's sixteen (compile-time zeros elsewhere: the
+// compiler drops the forward transform's work on them and the inverse's on
+// outputs never stored).
+template <typename R, int N, int V, bool RS, int EK, bool HW = false>
 __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(const ProbDev* __restrict__ probs,
                                                                       int nprob, int total) {
     using LY = FastLayout<R, N, V>;
@@ -218,7 +222,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
             v[m] = {R(0), R(0)};
-            if (y >= oy && y < oy + ny) v[m] = at32(work, b0 + (unsigned)y * B);
+            if (HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) v[m] = at32(work, b0 + (unsigned)y * B);
         }
         fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
         if constexpr (RS) {
@@ -294,7 +298,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
-            if (y >= oy && y < oy + ny) at32(work, b0 + (unsigned)y * B) = v[m];
+            if (HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) at32(work, b0 + (unsigned)y * B) = v[m];
         }
     }
 }
@@ -731,7 +735,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_lin
 // the two stages' descriptors, problem by problem; row tiles follow F (the
 // Fresnel batch's row layout), and A's intermediate must be blocked with
 // this layout's width B.
-template <typename R, int N, int V>
+template <typename R, int N, int V, bool HW = false>
 __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_chain(const ProbDev* __restrict__ A,
                                                                            const ProbDev* __restrict__ F,
                                                                            int nprob, int total) {
@@ -766,7 +770,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_cha
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int x = t + m * T - ox;
-            const bool in = rowok && x >= 0 && x < nx;
+            const bool in = HW ? ((m >= 4 && m < 12) && rowok) : (rowok && x >= 0 && x < nx);
             const cpx<R> o = {v[m].re * s, v[m].im * s};
             v[m] = in ? o : cpx<R>{R(0), R(0)};
             bad |= in && !finite_c(o);
@@ -792,7 +796,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_cha
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
                 const int x = t + m * T;
-                if (x >= ox && x < ox + nx) at32(fw, w0 + (unsigned)(m * (T / Q)) * ny * Q) = v[m];
+                if (HW ? (m >= 4 && m < 12) : (x >= ox && x < ox + nx))
+                    at32(fw, w0 + (unsigned)(m * (T / Q)) * ny * Q) = v[m];
             }
         }
     }
@@ -800,7 +805,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_cha
 
 // The chained Fresnel's column pass: CB columns per tile from the blocked
 // intermediate, y factor, crop, 1/(P_x P_y), natural-layout output.
-template <typename R, int N, int V>
+template <typename R, int N, int V, bool HW = false>
 __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_col_fin(
     const ProbDev* __restrict__ probs, int nprob, int total) {
     using LY = FastLayout<R, N, V>;
@@ -822,7 +827,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
             v[m] = {R(0), R(0)};
-            if (colok && y >= oy && y < oy + ny) v[m] = at32(work, b0 + (unsigned)y * B);
+            if ((HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) && colok) v[m] = at32(work, b0 + (unsigned)y * B);
         }
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
         fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
@@ -841,7 +846,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
                 const int y = t + m * T;
-                if (y >= oy && y < oy + ny) {
+                if (HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) {
                     const cpx<R> o = {v[m].re * s, v[m].im * s};
                     bad |= !finite_c(o);
                     at32(out, (unsigned)(y - oy) * nx + kx) = o;
@@ -913,6 +918,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
         case 1:
             if (s.rs)
                 hipLaunchKernelGGL((f_col<R, N, V, true, -1>), g, bcol, col_lds, st, d, np, total);
+            else if (s.col_kind == ENG_ASM && s.half)
+                hipLaunchKernelGGL((f_col<R, N, V, false, ENG_ASM, true>), g, bcol, col_lds, st, d, np, total);
             else if (s.col_kind == ENG_ASM)
                 hipLaunchKernelGGL((f_col<R, N, V, false, ENG_ASM>), g, bcol, col_lds, st, d, np, total);
             else if (s.col_kind == ENG_FRESNEL)
@@ -940,7 +947,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
     return hipGetLastError();
 }
 
-// which = 0: f_row_chain over (A, F); 1: f_sep_col_fin over F.  complex64.
+// which = 0: f_row_chain over (A, F); 1: f_sep_col_fin over F; 2, 3: the same
+// for batches whose problems are all N = P/2.  complex64.
 template <int N, int V>
 static hipError_t launch_chain_nv(int which, const ProbDev* a, const ProbDev* f, int np, int total, int grid,
                                   hipStream_t st) {
@@ -949,16 +957,20 @@ static hipError_t launch_chain_nv(int which, const ProbDev* a, const ProbDev* f,
         using LY = FastLayout<R, N, V>;
         constexpr int T = N / 16;
         const dim3 g(grid);
-        if (which == 0) {
-            const size_t lds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
-            hipLaunchKernelGGL((f_row_chain<R, N, V>), g, dim3(LY::G * T), lds, st, a, f, np, total);
-            return hipGetLastError();
+        const size_t rlds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
+        const size_t clds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
+        switch (which) {  // + 2: every problem is N = P/2
+            case 0: hipLaunchKernelGGL((f_row_chain<R, N, V>), g, dim3(LY::G * T), rlds, st, a, f, np, total); break;
+            case 1: hipLaunchKernelGGL((f_sep_col_fin<R, N, V>), g, dim3(LY::CB * T), clds, st, f, np, total); break;
+            case 2:
+                hipLaunchKernelGGL((f_row_chain<R, N, V, true>), g, dim3(LY::G * T), rlds, st, a, f, np, total);
+                break;
+            case 3:
+                hipLaunchKernelGGL((f_sep_col_fin<R, N, V, true>), g, dim3(LY::CB * T), clds, st, f, np, total);
+                break;
+            default: return hipErrorInvalidValue;
         }
-        if (which == 1) {
-            const size_t lds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
-            hipLaunchKernelGGL((f_sep_col_fin<R, N, V>), g, dim3(LY::CB * T), lds, st, f, np, total);
-            return hipGetLastError();
-        }
+        return hipGetLastError();
     }
     (void)a; (void)f; (void)np; (void)total; (void)grid; (void)st; (void)which;
     return hipErrorInvalidValue;
@@ -999,6 +1011,7 @@ static hipError_t attr_nv() {
                        (const void*)f_row_fwd<R, N, V, -1>,
                        (const void*)f_col<R, N, V, true, -1>,
                        (const void*)f_col<R, N, V, false, ENG_ASM>,
+                       (const void*)f_col<R, N, V, false, ENG_ASM, true>,
                        (const void*)f_col<R, N, V, false, ENG_FRESNEL>,
                        (const void*)f_col<R, N, V, false, ENG_GS_PROJECT>,
                        (const void*)f_col<R, N, V, false, -1>,
@@ -1021,7 +1034,8 @@ static hipError_t attr_nv() {
         const void* ks[] = {(const void*)f_sep_col<R, N, V, PRE_NONE>, (const void*)f_sep_col<R, N, V, PRE_MASK>,
                             (const void*)f_sep_col<R, N, V, PRE_TMUL>, (const void*)f_sep_col<R, N, V, PRE_MASKS>,
                             (const void*)f_sep_row<R, N, V>,         (const void*)f_row_chain<R, N, V>,
-                            (const void*)f_sep_col_fin<R, N, V>};
+                            (const void*)f_sep_col_fin<R, N, V>,     (const void*)f_row_chain<R, N, V, true>,
+                            (const void*)f_sep_col_fin<R, N, V, true>};
         for (const void* f : ks) {
             const hipError_t e = allow_lds(f);
             if (e != hipSuccess) return e;

@@ -577,6 +577,9 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
             b.fast_n = (int)n0;
             b.fast_v = v;
             b.rs = rs0;
+            bool half = true;
+            for (int i : cand) half = half && probs[i].g.nx * 2 == n0 && probs[i].g.ny * 2 == n0;
+            b.half = half;
         }
     }
     // complex64 Fresnel batches without post-ops factor per axis: two passes,
@@ -790,8 +793,8 @@ int launch_batch(oc_context* ctx, Batch& b, unsigned mask) {
         hipError_t e = timed(ctx, tpass, [&] {
             if (b.fast) {
                 const int v = (b.sep && pass == 2) ? b.fast_v_row : b.fast_v;
-                const ocd::FastSel sel{pass,       b.fast_n,   v,            b.rs ? 1 : 0, b.post_kind,
-                                       b.col_kind, b.pre_kind, b.sep ? 1 : 0};
+                const ocd::FastSel sel{pass,       b.fast_n,   v,             b.rs ? 1 : 0, b.post_kind,
+                                       b.col_kind, b.pre_kind, b.sep ? 1 : 0, b.half ? 1 : 0};
                 return b.dbl ? ocl::fast_launch_c128(sel, dprobs, nprob, c.total, grid, ctx->stream)
                              : ocl::fast_launch_c64(sel, dprobs, nprob, c.total, grid, ctx->stream);
             }
