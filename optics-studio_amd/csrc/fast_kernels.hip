@@ -195,11 +195,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
 
 // EK: engine (ENG_ASM, ENG_FRESNEL, ENG_GS_PROJECT, or -1 for the descriptor's); RS: the
 // Rayleigh-Sommerfeld product with the kernel spectrum instead.
-// HW: every problem is N = P/2 on both axes, so the rows held are exactly
-// m = 4..11 of each thread's sixteen (compile-time zeros elsewhere: the
-// compiler drops the forward transform's work on them and the inverse's on
-// outputs never stored).
-template <typename R, int N, int V, bool RS, int EK, bool HW = false>
+template <typename R, int N, int V, bool RS, int EK>
 __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(const ProbDev* __restrict__ probs,
                                                                       int nprob, int total) {
     using LY = FastLayout<R, N, V>;
@@ -222,7 +218,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
             v[m] = {R(0), R(0)};
-            if (HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) v[m] = at32(work, b0 + (unsigned)y * B);
+            if (y >= oy && y < oy + ny) v[m] = at32(work, b0 + (unsigned)y * B);
         }
         fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
         if constexpr (RS) {
@@ -298,7 +294,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
-            if (HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) at32(work, b0 + (unsigned)y * B) = v[m];
+            if (y >= oy && y < oy + ny) at32(work, b0 + (unsigned)y * B) = v[m];
         }
     }
 }
@@ -735,6 +731,13 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_lin
 // the two stages' descriptors, problem by problem; row tiles follow F (the
 // Fresnel batch's row layout), and A's intermediate must be blocked with
 // this layout's width B.
+// HW: every problem is N = P/2 on both axes, so the window is exactly rows
+// (here: samples) m = 4..11 of each thread's sixteen; the zeros outside it
+// are compile-time constants, and the compiler drops the forward
+// transform's first-stage work on them and the inverse's last-stage work on
+// outputs never stored (VALU 2458 -> 2260 per tile at N = 4096).  The
+// column pass has the same property but ran 3% slower specialised, so it
+// keeps the runtime window.
 template <typename R, int N, int V, bool HW = false>
 __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_chain(const ProbDev* __restrict__ A,
                                                                            const ProbDev* __restrict__ F,
@@ -918,8 +921,6 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
         case 1:
             if (s.rs)
                 hipLaunchKernelGGL((f_col<R, N, V, true, -1>), g, bcol, col_lds, st, d, np, total);
-            else if (s.col_kind == ENG_ASM && s.half)
-                hipLaunchKernelGGL((f_col<R, N, V, false, ENG_ASM, true>), g, bcol, col_lds, st, d, np, total);
             else if (s.col_kind == ENG_ASM)
                 hipLaunchKernelGGL((f_col<R, N, V, false, ENG_ASM>), g, bcol, col_lds, st, d, np, total);
             else if (s.col_kind == ENG_FRESNEL)
@@ -1011,7 +1012,6 @@ static hipError_t attr_nv() {
                        (const void*)f_row_fwd<R, N, V, -1>,
                        (const void*)f_col<R, N, V, true, -1>,
                        (const void*)f_col<R, N, V, false, ENG_ASM>,
-                       (const void*)f_col<R, N, V, false, ENG_ASM, true>,
                        (const void*)f_col<R, N, V, false, ENG_FRESNEL>,
                        (const void*)f_col<R, N, V, false, ENG_GS_PROJECT>,
                        (const void*)f_col<R, N, V, false, -1>,

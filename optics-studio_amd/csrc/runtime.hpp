@@ -261,7 +261,7 @@ struct Batch {
     int col_kind = -2;       // engine shared by every problem (-1: mixed)
     int pre_kind = -2;       // pre-op shared by every problem (-1: mixed)
     bool sep = false;        // separable complex64 Fresnel: two passes
-    bool half = false;       // power-of-two batch whose problems are all N = P/2 (square)
+    bool half = false;       // power-of-two batch whose problems are all N = P/2 (the chain's specialised kernels)
     bool rs = false;
     std::vector<ProbDev> pd;
     std::vector<int> idx;  // index into the problem vector of each descriptor
