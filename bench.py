@@ -82,7 +82,9 @@ def bench_gs(G, W, torch, np):
     t = torch.as_tensor(tgt.astype(np.float32)).cuda()
     p0 = torch.as_tensor(ph.astype(np.float32)).cuda()
     chunk = 20
-    variant, _ = G.context().autotune(c["n"], double=False)
+    # the default layout at 1024 (variant 1) is the fastest for GS: 103k
+    # restart-iterations/s against 98k, 95k and 86k for variants 3, 2, 0
+    variant = G.context().fft_variant(c["n"])
     st, _ = G.gs_fft_run_dev(a, t, p0.clone(), chunk, chunk)  # warm-up (plans, graph)
     if st:
         raise RuntimeError(f"gs warm-up failed: {st}")
