@@ -446,7 +446,10 @@ int oc_propagate_batch_dev_c128(oc_context* ctx, oc_prop_desc* descs,
    second[i].out is unspecified.  complex64 chains of masked or plain ASM
    into Fresnel at one power-of-two padded length run fused (the ASM's
    inverse row pass and the Fresnel's row pass in one kernel, the Fresnel's
-   column pass on the blocked intermediate); all others run as two batches.
+   column pass on the blocked intermediate); three-pass chains (complex128,
+   or complex64 ASM into ASM) at one power-of-two length run linked through
+   their rows (one kernel for the first inverse and the second forward row
+   pass); all others run as two batches.
    No single reference counterpart: it replaces the pair of calls. */
 int oc_propagate_chain_dev_c64(oc_context* ctx, oc_prop_desc* first,
                                oc_prop_desc* second, size_t count);
