@@ -193,6 +193,88 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     }
 }
 
+// The forward row pass of a batch whose problems share one transmission
+// (PRE_TMUL) and are all N = P/2: each thread's window is samples
+// m = 4..11, and those eight inputs are loaded one tile ahead, so that the
+// next tile's loads are in flight during this tile's transform.  Per tile:
+// the transmission loads first, then the next tile's inputs, then the
+// multiply (waiting on the transmission only), transform and store.
+template <typename R, int N, int V>
+__global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd_pf(
+    const ProbDev* __restrict__ probs, int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, G = LY::G, Q = LY::B;
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
+    const int t = tb + Q * tq;
+    const bool ilv = probs[0].ilv != 0;
+    auto locate = [&](int tile, int& pi, int& iy) {
+        int lt;
+        if (ilv) {
+            pi = tile % nprob;
+            lt = tile / nprob;
+        } else {
+            pi = find_problem<PASS_ROW>(probs, nprob, tile);
+            lt = tile - probs[pi].base_row;
+        }
+        iy = lt * G + g;
+    };
+    // window sample m + 4 of row iy of problem pi (x = t + (m + 4) T - ox)
+    auto load8 = [&](int pi, int iy, cpx<R> (&w)[8]) {
+        const ProbDev& P = probs[pi];
+        const bool ok = iy < P.ny;
+        const cpx<R>* in = static_cast<const cpx<R>*>(P.in);
+        const unsigned r0 = (unsigned)iy * P.nx - P.ox;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            w[m] = {R(0), R(0)};
+            if (ok) w[m] = at32(in, r0 + (unsigned)(t + (m + 4) * T));
+        }
+    };
+    cpx<R> nxt[8];
+    int tile = xcd_block(), pi = 0, iy = 0;
+    if (tile < total) {
+        locate(tile, pi, iy);
+        load8(pi, iy, nxt);
+    }
+    for (; tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[pi];
+        const int cy = iy;
+        const bool rowok = cy < P.ny;
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = (m >= 4 && m < 12) ? nxt[m - 4] : cpx<R>{R(0), R(0)};
+        cpx<R> tr[8];
+        const cpx<R>* trs = static_cast<const cpx<R>*>(P.ops.trans);
+        const unsigned r0 = (unsigned)cy * P.nx - P.ox;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            tr[m] = {R(0), R(0)};
+            if (rowok) tr[m] = at32(trs, r0 + (unsigned)(t + (m + 4) * T));
+        }
+        const ProbDev* Pc = &P;
+        if (tile + (int)gridDim.x < total) {
+            locate(tile + gridDim.x, pi, iy);
+            load8(pi, iy, nxt);
+        }
+        bool bad = false;
+#pragma unroll
+        for (int m = 4; m < 12; ++m) bad |= !finite_c(v[m]);
+        if (bad) atomicOr(Pc->flag, 1);
+#pragma unroll
+        for (int m = 4; m < 12; ++m) v[m] = cmul(v[m], tr[m - 4]);
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(Pc->prow.tw));
+        if (rowok) {
+            cpx<R>* work = static_cast<cpx<R>*>(Pc->work);
+            const unsigned ny = Pc->ny, b0 = ((unsigned)tq * ny + cy) * Q + tb;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q) = v[m];
+        }
+    }
+}
+
 // EK: engine (ENG_ASM, ENG_FRESNEL, ENG_GS_PROJECT, or -1 for the descriptor's); RS: the
 // Rayleigh-Sommerfeld product with the kernel spectrum instead.
 template <typename R, int N, int V, bool RS, int EK>
@@ -909,6 +991,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_NONE>), g, brow, row_lds, st, d, np, total);
             else if (s.pre_kind == PRE_MASK)
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_MASK>), g, brow, row_lds, st, d, np, total);
+            else if (s.pre_kind == PRE_TMUL && s.half)
+                hipLaunchKernelGGL((f_row_fwd_pf<R, N, V>), g, brow, row_lds, st, d, np, total);
             else if (s.pre_kind == PRE_TMUL)
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, PRE_TMUL>), g, brow, row_lds, st, d, np, total);
             else if (s.pre_kind == PRE_POLAR)
@@ -1007,6 +1091,7 @@ static hipError_t attr_nv() {
     const void* k[] = {(const void*)f_row_fwd<R, N, V, PRE_NONE>,
                        (const void*)f_row_fwd<R, N, V, PRE_MASK>,
                        (const void*)f_row_fwd<R, N, V, PRE_TMUL>,
+                       (const void*)f_row_fwd_pf<R, N, V>,
                        (const void*)f_row_fwd<R, N, V, PRE_POLAR>,
                        (const void*)f_row_fwd<R, N, V, PRE_MASKS>,
                        (const void*)f_row_fwd<R, N, V, -1>,

@@ -98,6 +98,7 @@ struct FastSel {
     int col_kind;    // ENG_ASM / ENG_FRESNEL when uniform, else -1
     int pre_kind;    // PRE_NONE / PRE_MASK / PRE_TMUL / PRE_POLAR when uniform, else -1
     int sep;         // separable complex64 Fresnel (two passes)
+    int half;        // every problem is N = P/2 on both axes (window = samples m = 4..11 of each thread)
 };
 
 // Parameters of the fused element-wise ops (gs.rs:484-564, adjoint.rs:200-236,

@@ -794,7 +794,7 @@ int launch_batch(oc_context* ctx, Batch& b, unsigned mask) {
             if (b.fast) {
                 const int v = (b.sep && pass == 2) ? b.fast_v_row : b.fast_v;
                 const ocd::FastSel sel{pass,       b.fast_n,   v,             b.rs ? 1 : 0, b.post_kind,
-                                       b.col_kind, b.pre_kind, b.sep ? 1 : 0};
+                                       b.col_kind, b.pre_kind, b.sep ? 1 : 0, b.half ? 1 : 0};
                 return b.dbl ? ocl::fast_launch_c128(sel, dprobs, nprob, c.total, grid, ctx->stream)
                              : ocl::fast_launch_c64(sel, dprobs, nprob, c.total, grid, ctx->stream);
             }
