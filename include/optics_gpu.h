@@ -21,8 +21,11 @@
  *    and the message is available from oc_last_error().
  *
  * Threading: a context owns one HIP stream, a scratch arena and plan caches.
- * Calls on one context are serialised by the caller; separate contexts may be
- * used from separate threads.
+ * Every entry point holds its context's lock for the length of the call, so
+ * calls on one context from several threads are serialised by the library
+ * (in an unspecified order: a caller that needs an order between them, or
+ * between queued batches, orders its own calls); separate contexts run
+ * concurrently from separate threads.
  */
 #ifndef OPTICS_GPU_H
 #define OPTICS_GPU_H

@@ -75,7 +75,7 @@ __device__ __forceinline__ void row_pre(const ProbDev& P, cpx<R> (&v)[16], int i
             if (rowok && x >= ox && x < ox + nx) {
                 cpx<R> w;
                 w = at32(tr, r0 + x);
-                v[m] = cmul(v[m], w);
+                v[m] = tmul(v[m], w);
             }
         }
     }
@@ -145,9 +145,11 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
                 v[m] = {R(0), R(0)};
                 if (rowok && x >= ox && x < ox + nx) v[m] = at32(in, r0 + x);
             }
+            // the engine validates the masked field (samples outside the
+            // stop are zero after the apply)
+            row_pre<R, N, PK>(P, v, iy, rowok, t);
 #pragma unroll
             for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
-            row_pre<R, N, PK>(P, v, iy, rowok, t);
         } else if constexpr (PK == PRE_POLAR) {
             // GS source plane: amplitude and phase of the 16 samples as one
             // batch of loads, then from_polar (masked by the stop)
@@ -259,12 +261,12 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
             locate(tile + gridDim.x, pi, iy);
             load8(pi, iy, nxt);
         }
+#pragma unroll
+        for (int m = 4; m < 12; ++m) v[m] = tmul(v[m], tr[m - 4]);
         bool bad = false;
 #pragma unroll
         for (int m = 4; m < 12; ++m) bad |= !finite_c(v[m]);
         if (bad) atomicOr(Pc->flag, 1);
-#pragma unroll
-        for (int m = 4; m < 12; ++m) v[m] = cmul(v[m], tr[m - 4]);
         fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(Pc->prow.tw));
         if (rowok) {
             cpx<R>* work = static_cast<cpx<R>*>(Pc->work);
@@ -635,10 +637,6 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
             v[m] = {R(0), R(0)};
             if (colok && y >= oy && y < oy + ny) v[m] = at32(in, (unsigned)(y - oy) * nx + kx);
         }
-        bool bad = false;
-#pragma unroll
-        for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
-        if (bad) atomicOr(P.flag, 1);
         if (PK == PRE_MASK || (PK == PRE_MASKS && P.ops.pre == PRE_MASK)) {
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
@@ -654,10 +652,15 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
                 if (colok && y >= oy && y < oy + ny) {
                     cpx<R> w;
                     w = at32(tr, (unsigned)(y - oy) * nx + kx);
-                    v[m] = cmul(v[m], w);
+                    v[m] = tmul(v[m], w);
                 }
             }
         }
+        // the engine validates the masked field
+        bool bad = false;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
+        if (bad) atomicOr(P.flag, 1);
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
         fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
         const cpx<R>* hy = static_cast<const cpx<R>*>(P.eng.hy);

@@ -62,6 +62,15 @@ template <typename R>
 __device__ __forceinline__ cpx<R> cmul(cpx<R> a, cpx<R> b) {
     return {fmadd(a.re, b.re, -(a.im * b.im)), fmadd(a.re, b.im, a.im * b.re)};
 }
+// Product with a phase-mask transmission t: t is e^{i phi} (|t| = 1) inside
+// the stop and exactly 0 outside, where the apply sets the sample to 0
+// whatever it held (a non-finite sample there included).
+template <typename R>
+__device__ __forceinline__ cpx<R> tmul(cpx<R> v, cpx<R> t) {
+    const bool stop = t.re == R(0) && t.im == R(0);
+    const cpx<R> o = cmul(v, t);
+    return {stop ? R(0) : o.re, stop ? R(0) : o.im};
+}
 template <typename R>
 __device__ __forceinline__ cpx<R> cadd(cpx<R> a, cpx<R> b) {
     return {a.re + b.re, a.im + b.im};

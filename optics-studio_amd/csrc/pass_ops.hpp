@@ -413,15 +413,19 @@ __device__ __forceinline__ cpx<R> load_pre(const ProbDev& P, int ix, int iy, boo
         return {a * e.re, a * e.im};
     }
     cpx<R> v = gp(static_cast<const cpx<R>*>(P.in))[k];
-    bad |= !finite_c(v);
+    // a mask is Element::apply before the engine: the engine validates the
+    // masked field, in which samples outside the stop are zero
+    if (pre != PRE_MASK && pre != PRE_TMUL) bad |= !finite_c(v);
     if (pre == PRE_MASK) {
         v = apply_mask(P.mask, ix, iy, v);
+        bad |= !finite_c(v);
     } else if (pre == PRE_FRAUN) {
         v = cmul(v, fraun_pre<R>(P.eng, ix, iy, P.nx, P.ny));
     } else if (pre == PRE_TMUL) {
         cpx<R> t;
         t = gp(static_cast<const cpx<R>*>(o.trans))[k];
-        v = cmul(v, t);
+        v = tmul(v, t);
+        bad |= !finite_c(v);
     } else if (pre == PRE_SEED_MSE) {  // w = (dM/dI) U, dM/dI = 2 (I - T)/n (merit.rs:143-165)
         const double re = v.re, im = v.im;
         const double d = (re * re + im * im) - gp(o.seed_target)[k];

@@ -255,15 +255,19 @@ __global__ void k_rs_kgen(const ProbDev* __restrict__ probs, int nprob, int tota
 /* Elementwise helpers.                                                 */
 /* ------------------------------------------------------------------ */
 template <typename R>
-__global__ void k_mask_apply(const cpx<R>* __restrict__ in, cpx<R>* __restrict__ out, MaskDev m, int nx, int ny,
-                             int* flag) {
+__global__ void k_mask_apply(const cpx<R>* in, cpx<R>* out, MaskDev m, int nx, int ny, int* flag) {
+    // `flag` (optional) receives bit 0 for a non-finite sample of the masked
+    // field (what a propagation after the apply validates: samples outside
+    // the stop are zero); out == nullptr only checks.
     const size_t n = (size_t)nx * ny;
+    bool bad = false;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const int iy = (int)(i / nx), ix = (int)(i - (size_t)iy * nx);
-        cpx<R> v = in[i];
-        if (!finite_c(v) && flag) atomicOr(flag, 1);
-        out[i] = apply_mask(m, ix, iy, v);
+        const cpx<R> o = apply_mask(m, ix, iy, in[i]);
+        bad |= !finite_c(o);
+        if (out) out[i] = o;
     }
+    if (bad && flag) atomicOr(flag, 1);
 }
 
 // Transmission of a phase mask over the field grid (the mask applied to 1).

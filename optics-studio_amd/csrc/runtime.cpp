@@ -819,9 +819,12 @@ void evict_tables(oc_context* ctx) {
 // their finiteness flags in dflags[0..n).  No synchronisation.
 int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags, bool zero_flags) {
     evict_tables(ctx);
+    // A masked problem is Element::apply then the engine (beamline.rs:
+    // 118-123): the mask's footprint error comes first, also for the z == 0
+    // identity and a failed guard.
     for (auto& p : probs) {
         prepare(p);
-        if (p.status == OC_OK && p.mask && !p.check_only) {
+        if (p.status == OC_OK && p.mask) {
             int st = mask_footprint(p.g, *p.mask);
             if (st) p.status = st;
         }
@@ -832,6 +835,19 @@ int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* df
         Prob& p = probs[i];
         if (p.status != OC_OK || !p.check_only) continue;
         const size_t n = p.g.nx * p.g.ny;
+        if (p.mask) {
+            // the engine validates the masked field; the identity returns it
+            // (a failed guard writes nothing)
+            MaskDev md;
+            int st = mask_tables(ctx, p.g, *p.mask, p.mask->values, &md);
+            if (st) return st;
+            void* o = p.pending == OC_OK ? p.out : nullptr;
+            HIP_TRY(ctx, dbl ? ocl::launch_mask_apply_c128(p.in, o, md, (int)p.g.nx, (int)p.g.ny, dflags + i,
+                                                           ctx->stream)
+                             : ocl::launch_mask_apply_c64(p.in, o, md, (int)p.g.nx, (int)p.g.ny, dflags + i,
+                                                          ctx->stream));
+            continue;
+        }
         HIP_TRY(ctx, ocl::launch_check_finite(dbl, p.in, n, dflags + i, ctx->stream));
         if (p.pending == OC_OK && p.out != p.in) HIP_TRY(ctx, ocl::launch_copy(dbl, p.in, p.out, n, ctx->stream));
     }

@@ -506,28 +506,30 @@ class DevEngine:
         return torch.as_tensor(np.ascontiguousarray(u, self.ndt)).cuda()
 
     @staticmethod
-    def _down(t):
-        return t.cpu().numpy().astype(np.complex128)
+    def _down(t, st=0):
+        """The output as complex128, or None after a failed call (the output
+        buffer is unspecified then and is not read)."""
+        return t.cpu().numpy().astype(np.complex128) if st == 0 else None
 
     def asm_propagate(self, u, dx, dy, lam, z, cfg):
         st, out, rep = asm_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
-        return st, self._down(out), rep
+        return st, self._down(out, st), rep
 
     def fresnel_propagate(self, u, dx, dy, lam, z, cfg):
         st, out, rep = fresnel_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
-        return st, self._down(out), rep
+        return st, self._down(out, st), rep
 
     def rs_propagate(self, u, dx, dy, lam, z, cfg):
         st, out, rep = rs_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
-        return st, self._down(out), rep
+        return st, self._down(out, st), rep
 
     def fresnel_propagate_convolution(self, u, dx, dy, lam, z, cfg):
         st, out, rep = fresnel_propagate_convolution_dev(self._up(u), dx, dy, lam, z, cfg)
-        return st, self._down(out), rep
+        return st, self._down(out, st), rep
 
     def fraunhofer_propagate(self, u, dx, dy, lam, z, cfg):
         st, out, rep = fraunhofer_propagate_dev(self._up(u), dx, dy, lam, z, cfg)
-        return st, self._down(out), rep
+        return st, self._down(out, st), rep
 
     @staticmethod
     def fraunhofer_regime_report(shape, dx, dy, lam, z, cfg):
@@ -540,7 +542,7 @@ class DevEngine:
 
     def phasemask_apply(self, u, dx, dy, lam, mask):
         st, out = phasemask_apply_dev(self._up(u), dx, dy, lam, mask.struct)
-        return st, self._down(out)
+        return st, self._down(out, st)
 
 
 # ---------------------------------------------------------------------------
@@ -618,6 +620,17 @@ def adjoint_push_back(steps, u, dx, dy, lam, asm_cfg, dmdi, ctx=None):
     st = lib.oc_adjoint_push_back(ctx.h, _steps(steps), len(steps), C.byref(_geom(u, dx, dy, lam)), u.ctypes.data,
                                   C.byref(asm_cfg), d.ctypes.data, out.ctypes.data)
     return st, out
+
+
+def adjoint_asm(u, dx, dy, lam, z, cfg, ctx=None):
+    """adjoint::asm_adjoint (adjoint.rs:90-92): A(z)^dagger = A(-z), host f64."""
+    ctx = ctx or context()
+    u = _c128(u)
+    out = np.empty_like(u)
+    rep = T.AsmReport()
+    st = lib.oc_adjoint_asm(ctx.h, C.byref(_geom(u, dx, dy, lam)), u.ctypes.data, z, C.byref(cfg),
+                            out.ctypes.data, C.byref(rep))
+    return st, out, rep
 
 
 def adjoint_transmission(mask, shape, dx, dy, lam, ctx=None):

@@ -6,7 +6,7 @@
  * contraction: built with -ffp-contract=off) so that it reproduces the
  * reference's f64 results up to the FFT library's own rounding.
  *
- * Parity pins (tests/test_oracle.py, tests/test_oracle_golden.py):
+ * Parity pins (tests/test_oracle_*.py):
  *   fft2/ifft2/fftfreq ........ numpy.fft (pocketfft) vectors at odd, even and
  *                               prime sizes; reference fft.rs unit tests
  *                               (round trip, Parseval, delta, convolution,
@@ -15,8 +15,11 @@
  *                               ThinLens chain, padding 1); reference asm.rs
  *                               and tests/asm.rs closed forms (z=0 identity,
  *                               e^{ikz}, e^{inkt}, round trip, strict guard,
- *                               energy); brute-force direct-sum ASM on small
- *                               padded grids.
+ *                               energy); padded, band-limited ASM against a
+ *                               direct DFT-matrix evaluation on small even
+ *                               and odd padded grids (limit active and not)
+ *                               and against numpy's pocketfft at the
+ *                               benchmark geometries.
  *   fresnel TF / convolution .. exact bin phase, round trip, closed-form step
  *                               vs sweep, TF == convolution at critical
  *                               sampling, delta -> sampled kernel.

@@ -250,3 +250,56 @@ def test_grid_gradient_dev_matches_fd_and_oracle(G, orc):
     rel = np.linalg.norm(fd - ad) / np.linalg.norm(fd)
     say("  fd rel", rel)
     assert rel < 1e-4
+
+
+def test_adjoint_asm_entry(G, orc):
+    """oc_adjoint_asm (adjoint.rs:90-92): propagate by -z with the same
+    config, bit for bit the drop-in at -z; the inner-product identity
+    <w, A(z) d> = <A(z)^dagger w, d> (adjoint.rs:406-427) to 1e-12 with the
+    band limit on and off; a failing propagation maps to InverseError::Adjoint."""
+    n, d, lam, z = 48, 4e-6, 633e-9, 3e-3
+    rng = np.random.default_rng(4)
+    dd = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    w = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    for bl in (1, 0):
+        cfg = T.AsmConfig(2.0, bl, 0, 2.0, 1.0)
+        st, back, rep = G.adjoint_asm(w, d, d, lam, z, cfg)
+        assert st == 0, T.STATUS[st]
+        st, ref, rref = G.asm_propagate(w, d, d, lam, -z, cfg)
+        assert st == 0
+        assert np.array_equal(back, ref)
+        assert T.report_dict(rep) == T.report_dict(rref)
+        st, fwd, _ = G.asm_propagate(dd, d, d, lam, z, cfg)
+        assert st == 0
+        lhs = np.vdot(w, fwd)
+        rhs = np.vdot(back, dd)
+        say("inner product, band limit", bl, abs(lhs - rhs) / abs(lhs))
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+        st, o_ref, _ = orc.asm_propagate(w, d, d, lam, -z, cfg)
+        assert np.linalg.norm(back - o_ref) / np.linalg.norm(o_ref) <= 1e-12
+    st, _, _ = G.adjoint_asm(w, d, d, lam, float("nan"), T.AsmConfig.default())
+    assert st == T.CODE["OC_INV_ADJOINT"]
+
+
+def test_adjoint_transmission_entry(G, orc):
+    """oc_adjoint_transmission (adjoint.rs:102-108): apply(1), bit for bit the
+    host phase-mask apply of a unit field and e^{i phi} inside the stop (0
+    outside) against the oracle to 1e-12; a stop beyond the grid is
+    InverseError::Adjoint."""
+    n, d, lam = 40, 5e-6, 550e-9
+    vals = 2 * math.pi * W.splitmix_uniform(8, n * n).reshape(n, n)
+    ap = (T.APERTURE_CIRCLE, 80e-6, 0.0)
+    gm = G.Mask(vals, d, d, orientation=0.3, aperture=ap, center=(5e-6, 0.0))
+    st, t = G.adjoint_transmission(gm, (n, n), d, d, lam)
+    assert st == 0, T.STATUS[st]
+    st, t2 = G.phasemask_apply(np.ones((n, n), complex), d, d, lam, gm)
+    assert st == 0
+    assert np.array_equal(t, t2)
+    st, tr = orc.phasemask_apply(np.ones((n, n), complex), d, d, lam,
+                                 orc.Mask(vals, d, d, orientation=0.3, aperture=ap, center=(5e-6, 0.0)))
+    assert st == 0
+    assert np.abs(t - tr).max() <= 1e-12
+    assert np.count_nonzero(t) == np.count_nonzero(tr) > 0
+    big = G.Mask(vals, d, d, aperture=(T.APERTURE_CIRCLE, 150e-6, 0.0))
+    st, _ = G.adjoint_transmission(big, (n, n), d, d, lam)
+    assert st == T.CODE["OC_INV_ADJOINT"]
