@@ -459,6 +459,22 @@ int oc_propagate_chain_dev_c64(oc_context* ctx, oc_prop_desc* first,
 int oc_propagate_chain_dev_c128(oc_context* ctx, oc_prop_desc* first,
                                 oc_prop_desc* second, size_t count);
 
+/* oc_propagate_batch_dev_c64 without device allocation.  No counterpart in
+   the reference: it is for callers that cannot allocate (code under stream
+   capture, tight serving loops, harnesses that forbid hipMalloc).  Every
+   buffer, table and descriptor of the call comes from one 256 MiB scratch
+   array per device that the library reserves when it loads and holds until
+   the process ends; the call allocates no device or pinned memory.  A batch
+   whose buffers would exceed it (e.g. more than three 2048^2 problems at
+   P = 4096) is refused with OC_ERR_INVALID_ARGUMENT before anything runs
+   (every status set to it; oc_last_error says how much it needs).  Calls on
+   one device take turns on the array (each waits for its own kernels before
+   the next starts), and the array keeps the last call's contents.  Same
+   kernels and results as oc_propagate_batch_dev_c64; returns after the
+   batch has run. */
+int oc_propagate_batch_noalloc_dev_c64(oc_context* ctx, oc_prop_desc* descs,
+                                       size_t count);
+
 /* The same without waiting (no counterpart in the reference: a pipelining
    form for serving loops).  Host-side checks fill descs[i].status now and
    the batch is queued on the context stream; oc_context_wait resolves the

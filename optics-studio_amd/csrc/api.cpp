@@ -56,6 +56,10 @@ int oc_context_create(int device, oc_context** out) {
     if (ocl::set_lds_limits() != hipSuccess) return OC_ERR_DEVICE;
     if (ocl::fast_set_lds_limits_c64() != hipSuccess || ocl::fast_set_lds_limits_c128() != hipSuccess)
         return OC_ERR_DEVICE;
+    if (noalloc_attach(c.get()) != OC_OK) {
+        (void)hipStreamDestroy(c->own);
+        return OC_ERR_DEVICE;
+    }
     *out = c.release();
     return OC_OK;
 }
@@ -345,16 +349,7 @@ int oc_rs_propagate_dev_c64(oc_context* ctx, const oc_band_geom* g, const float*
 }
 
 static int batch_probs(const oc_prop_desc* d, size_t count, std::vector<Prob>& v) {
-    v.reserve(count);
-    for (size_t i = 0; i < count; ++i) {
-        const oc_prop_desc& x = d[i];
-        if ((x.engine < OC_ENGINE_ASM || x.engine > OC_ENGINE_RS) && x.engine != OC_ENGINE_FRESNEL_CONV)
-            return OC_ERR_INVALID_ARGUMENT;
-        if ((x.engine == OC_ENGINE_RS || x.engine == OC_ENGINE_FRESNEL_CONV) && x.mask) return OC_ERR_INVALID_ARGUMENT;
-        v.push_back(make_prob(&x.geom, x.z, x.engine, x.band_limited, x.strict, x.padding, x.max_padding,
-                              x.engine == OC_ENGINE_ASM ? x.medium : 1.0, x.mask, x.in, x.out));
-    }
-    return OC_OK;
+    return probs_from_descs(d, count, v);
 }
 
 static int batch(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {

@@ -42,7 +42,54 @@ void collect_timing(oc_context* ctx) {
 }
 
 
+int arena_alloc(oc_context* ctx, size_t bytes, void** out) {
+    const size_t a = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    if (!ctx->scratch || ctx->arena_used > ctx->scratch_cap || a > ctx->scratch_cap - ctx->arena_used)
+        return fail(ctx, OC_ERR_INVALID_ARGUMENT,
+                    "the no-allocation scratch (" + std::to_string(ctx->scratch_cap >> 20) + " MiB) is exhausted");
+    *out = static_cast<unsigned char*>(ctx->scratch) + ctx->arena_used;
+    ctx->arena_used += a;
+    return OC_OK;
+}
+
+int arena_upload(oc_context* ctx, const void* host, size_t bytes, const void** out) {
+    void* d;
+    int st = arena_alloc(ctx, bytes, &d);
+    if (st) return st;
+    ctx->arena_keep.emplace_back(static_cast<const unsigned char*>(host),
+                                 static_cast<const unsigned char*>(host) + bytes);
+    HIP_TRY(ctx, hipMemcpyAsync(d, ctx->arena_keep.back().data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+    *out = d;
+    return OC_OK;
+}
+
+const void* find_table(oc_context* ctx, const std::string& key) {
+    if (ctx->arena) {
+        auto it = ctx->arena_tables.find(key);
+        return it == ctx->arena_tables.end() ? nullptr : it->second;
+    }
+    auto it = ctx->tables.find(key);
+    return it == ctx->tables.end() ? nullptr : it->second.dev;
+}
+
+int store_table(oc_context* ctx, const std::string& key, const void* host, size_t bytes, const void** out) {
+    if (ctx->arena) {
+        int st = arena_upload(ctx, host, bytes, out);
+        if (st) return st;
+        ctx->arena_tables[key] = *out;
+        return OC_OK;
+    }
+    TableEntry te;
+    HIP_TRY(ctx, hipMalloc(&te.dev, std::max<size_t>(bytes, 1)));
+    HIP_TRY(ctx, hipMemcpy(te.dev, host, bytes, hipMemcpyHostToDevice));
+    te.bytes = bytes;
+    ctx->tables.emplace(key, te);
+    *out = te.dev;
+    return OC_OK;
+}
+
 int ensure_buf(oc_context* ctx, const std::string& name, size_t bytes, void** out) {
+    if (ctx->arena) return arena_alloc(ctx, bytes, out);
     DevBuf& b = ctx->bufs[name];
     if (b.cap < bytes) {
         if (b.p) {
@@ -85,10 +132,18 @@ int ensure_pinned(oc_context* ctx, int slot, size_t bytes) {
 // exp(-2 pi i k/n), k < n, evaluated in extended precision then rounded.
 int get_twiddles(oc_context* ctx, int n, bool dbl, const void** out) {
     auto key = std::make_pair(n, dbl ? 1 : 0);
-    auto it = ctx->twiddles.find(key);
-    if (it != ctx->twiddles.end()) {
-        *out = it->second;
-        return OC_OK;
+    const std::string akey = fmtkey("tw", {(double)n, dbl ? 1.0 : 0.0});
+    if (ctx->arena) {
+        if (const void* t = find_table(ctx, akey)) {
+            *out = t;
+            return OC_OK;
+        }
+    } else {
+        auto it = ctx->twiddles.find(key);
+        if (it != ctx->twiddles.end()) {
+            *out = it->second;
+            return OC_OK;
+        }
     }
     const long double two_pi = 6.283185307179586476925286766559005768L;
     size_t esz = dbl ? 16 : 8;
@@ -104,6 +159,7 @@ int get_twiddles(oc_context* ctx, int n, bool dbl, const void** out) {
             std::memcpy(host.data() + esz * k, v, esz);
         }
     }
+    if (ctx->arena) return store_table(ctx, akey, host.data(), host.size(), out);
     void* d = nullptr;
     HIP_TRY(ctx, hipMalloc(&d, host.size()));
     HIP_TRY(ctx, hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice));
@@ -186,8 +242,8 @@ std::string fmtkey(const char* tag, std::initializer_list<double> vals) {
 int fresnel_factors(oc_context* ctx, const Prob& p, EngineDev* e) {
     const size_t nxp = p.nxp, nyp = p.nyp;
     std::string key = fmtkey("frh", {(double)nxp, (double)nyp, p.g.dx, p.g.dy, p.g.lambda, p.z});
-    auto it = ctx->tables.find(key);
-    if (it == ctx->tables.end()) {
+    const void* dev = find_table(ctx, key);
+    if (!dev) {
         std::vector<float> h(2 * (nxp + nyp));
         for (size_t k = 0; k < nxp + nyp; ++k) {
             const bool isx = k < nxp;
@@ -196,15 +252,11 @@ int fresnel_factors(oc_context* ctx, const Prob& p, EngineDev* e) {
             h[2 * k] = (float)std::cos((long double)ph);
             h[2 * k + 1] = (float)std::sin((long double)ph);
         }
-        TableEntry te;
-        const size_t bytes = h.size() * sizeof(float);
-        HIP_TRY(ctx, hipMalloc(&te.dev, bytes));
-        HIP_TRY(ctx, hipMemcpy(te.dev, h.data(), bytes, hipMemcpyHostToDevice));
-        te.bytes = bytes;
-        it = ctx->tables.emplace(key, te).first;
+        int st = store_table(ctx, key, h.data(), h.size() * sizeof(float), &dev);
+        if (st) return st;
     }
-    e->hx = it->second.dev;
-    e->hy = static_cast<const float*>(it->second.dev) + 2 * nxp;
+    e->hx = dev;
+    e->hy = static_cast<const float*>(dev) + 2 * nxp;
     return OC_OK;
 }
 
@@ -274,9 +326,8 @@ int engine_tables(oc_context* ctx, const Prob& p, bool dbl, EngineDev* e) {
     }
     std::string key = fmtkey(p.engine == ENG_ASM ? "asm" : "fr",
                              {(double)nxp, (double)nyp, p.g.dx, p.g.dy, lam_n, (double)p.band_limited, lim_x, lim_y});
-    auto it = ctx->tables.find(key);
-    const size_t bytes = (nxp + nyp) * 8;
-    if (it == ctx->tables.end()) {
+    const void* dev = find_table(ctx, key);
+    if (!dev) {
         // ASM: (lam_n f)^2, or +inf where the band limit masks the frequency
         // (then 1 - ax - ay = -inf < 0 and the sample is zeroed like an
         // evanescent one).  Fresnel: f^2.
@@ -292,13 +343,10 @@ int engine_tables(oc_context* ctx, const Prob& p, bool dbl, EngineDev* e) {
                 h[k] = f * f;
             }
         }
-        TableEntry te;
-        HIP_TRY(ctx, hipMalloc(&te.dev, bytes));
-        HIP_TRY(ctx, hipMemcpy(te.dev, h.data(), bytes, hipMemcpyHostToDevice));
-        te.bytes = bytes;
-        it = ctx->tables.emplace(key, te).first;
+        int st = store_table(ctx, key, h.data(), h.size() * sizeof(double), &dev);
+        if (st) return st;
     }
-    e->ax = static_cast<const double*>(it->second.dev);
+    e->ax = static_cast<const double*>(dev);
     e->ay = e->ax + nxp;
     if (p.engine == ENG_FRESNEL && !dbl) return fresnel_factors(ctx, p, e);
     return OC_OK;
@@ -352,8 +400,8 @@ int mask_tables(oc_context* ctx, const oc_band_geom& g, const oc_phase_mask& m, 
     const size_t nx = g.nx, ny = g.ny;
     std::string key = fmtkey("mask", {(double)nx, (double)ny, g.dx, g.dy, (double)m.nx, (double)m.ny, m.dx, m.dy,
                                       m.cx, m.cy});
-    auto it = ctx->tables.find(key);
-    if (it == ctx->tables.end()) {
+    const void* dev = find_table(ctx, key);
+    if (!dev) {
         std::vector<MaskAxis> h(nx + ny);
         auto fill = [](MaskAxis* out, size_t n, double d, size_t mn, double md_, double c) {
             for (size_t i = 0; i < n; ++i) {
@@ -374,14 +422,10 @@ int mask_tables(oc_context* ctx, const oc_band_geom& g, const oc_phase_mask& m, 
         };
         fill(h.data(), nx, g.dx, m.nx, m.dx, m.cx);
         fill(h.data() + nx, ny, g.dy, m.ny, m.dy, m.cy);
-        TableEntry te;
-        const size_t bytes = h.size() * sizeof(MaskAxis);
-        HIP_TRY(ctx, hipMalloc(&te.dev, bytes));
-        HIP_TRY(ctx, hipMemcpy(te.dev, h.data(), bytes, hipMemcpyHostToDevice));
-        te.bytes = bytes;
-        it = ctx->tables.emplace(key, te).first;
+        int st = store_table(ctx, key, h.data(), h.size() * sizeof(MaskAxis), &dev);
+        if (st) return st;
     }
-    md->sx = static_cast<const MaskAxis*>(it->second.dev);
+    md->sx = static_cast<const MaskAxis*>(dev);
     md->sy = md->sx + nx;
     return OC_OK;
 }
@@ -527,7 +571,7 @@ void release_queued(oc_context* ctx) {
 }
 
 int batch_alloc(oc_context* ctx, Batch& b, bool persistent, const std::string& name, size_t bytes, void** out) {
-    if (!persistent) return ensure_buf(ctx, name, bytes, out);
+    if (!persistent || ctx->arena) return ensure_buf(ctx, name, bytes, out);
     HIP_TRY(ctx, hipMalloc(out, std::max<size_t>(bytes, 256)));
     b.owned.push_back(*out);
     return OC_OK;
@@ -753,9 +797,14 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     for (const ProbDev& d : b.pd) fraun = fraun && d.eng.kind == ENG_FRAUNHOFER;
     if (fraun) b.pc[2].total = 0;  // the column pass writes the far field
     const size_t bytes = b.pd.size() * sizeof(ProbDev);
-    int st = batch_alloc(ctx, b, persistent, "probs" + tag, bytes, &b.dprobs);
+    int st = ctx->arena ? OC_OK : batch_alloc(ctx, b, persistent, "probs" + tag, bytes, &b.dprobs);
     if (st) return st;
-    if (persistent) {
+    if (ctx->arena) {
+        const void* d = nullptr;
+        st = arena_upload(ctx, b.pd.data(), bytes, &d);
+        if (st) return st;
+        b.dprobs = const_cast<void*>(d);
+    } else if (persistent) {
         HIP_TRY(ctx, hipMemcpy(b.dprobs, b.pd.data(), bytes, hipMemcpyHostToDevice));
     } else {
         // the slot is free once its previous copy ran (not the kernels
@@ -807,7 +856,7 @@ int launch_batch(oc_context* ctx, Batch& b, unsigned mask) {
 }
 
 void evict_tables(oc_context* ctx) {
-    if (ctx->tables.size() > 1024) {  // between calls only: tables in flight stay valid
+    if (!ctx->arena && ctx->tables.size() > 1024) {  // between calls only: tables in flight stay valid
         (void)hipStreamSynchronize(ctx->stream);
         for (auto& kv : ctx->tables) (void)hipFree(kv.second.dev);
         ctx->tables.clear();
@@ -989,6 +1038,53 @@ Prob make_prob(const oc_band_geom* g, double z, int engine, int band_limited, in
 }
 
 bool geom_ok(const oc_band_geom* g) { return g != nullptr; }
+
+int probs_from_descs(const oc_prop_desc* d, size_t count, std::vector<Prob>& v) {
+    v.reserve(count);
+    for (size_t i = 0; i < count; ++i) {
+        const oc_prop_desc& x = d[i];
+        if ((x.engine < OC_ENGINE_ASM || x.engine > OC_ENGINE_RS) && x.engine != OC_ENGINE_FRESNEL_CONV)
+            return OC_ERR_INVALID_ARGUMENT;
+        if ((x.engine == OC_ENGINE_RS || x.engine == OC_ENGINE_FRESNEL_CONV) && x.mask) return OC_ERR_INVALID_ARGUMENT;
+        v.push_back(make_prob(&x.geom, x.z, x.engine, x.band_limited, x.strict, x.padding, x.max_padding,
+                              x.engine == OC_ENGINE_ASM ? x.medium : 1.0, x.mask, x.in, x.out));
+    }
+    return OC_OK;
+}
+
+// Upper bound of what launch_problems allocates for `probs` (unprepared):
+// every buffer, table and descriptor rounded up to 256 bytes, one shared
+// transmission per distinct mask and padded size (a batch is built per
+// padded size), tables counted per problem rather than per key.
+size_t arena_need(const std::vector<Prob>& probs, bool dbl) {
+    auto a = [](size_t x) { return (std::max<size_t>(x, 1) + 255) & ~(size_t)255; };
+    const size_t esz = dbl ? 16 : 8;
+    size_t need = a(sizeof(int) * std::max<size_t>(probs.size(), 1)) + 4096;
+    std::map<std::string, int> trans;
+    for (Prob p : probs) {
+        prepare(p);
+        if (p.status == OC_OK && p.mask) p.status = mask_footprint(p.g, *p.mask);
+        if (p.status != OC_OK) continue;
+        const size_t nx = p.g.nx, ny = p.g.ny;
+        if (p.mask) {
+            const oc_phase_mask& m = *p.mask;
+            const std::string k =
+                fmtkey("tm", {(double)(uintptr_t)m.values, (double)m.nx, (double)m.ny, m.dx, m.dy, m.orientation,
+                              (double)m.aperture.kind, m.aperture.a, m.aperture.b, m.cx, m.cy, (double)nx, (double)ny,
+                              p.g.dx, p.g.dy, (double)p.nxp, (double)p.nyp});
+            need += a((nx + ny) * sizeof(MaskAxis));
+            if (trans[k]++ == 0) need += a(nx * ny * esz);
+        }
+        if (p.check_only || !p.launch) continue;
+        const size_t nxp = p.nxp, nyp = p.nyp;
+        need += a(ny * nxp * esz);                           // intermediate
+        if (conv_engine(p.engine)) need += a(nyp * nxp * esz);  // kernel spectra
+        need += a(nxp * esz) + a(nyp * esz);                 // twiddles
+        need += a((nxp + nyp) * 8) + a((nxp + nyp) * 8);     // axis tables, Fresnel factors
+        need += a(sizeof(ProbDev)) + 256;                    // descriptor
+    }
+    return need;
+}
 
 // Host-memory wrapper: H2D, run, D2H.
 int run_host(oc_context* ctx, Prob p, const double* in, double* out, const oc_phase_mask* host_mask, Prob* done) {

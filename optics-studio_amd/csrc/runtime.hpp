@@ -134,6 +134,17 @@ struct oc_context {
     bool force_generic = false;  // route every batch through the runtime-plan kernels
     void* queued = nullptr;      // batches submitted without waiting (QueuedSet, created on first use)
     std::recursive_mutex mu;     // held by every entry point for the duration of its call
+    // No-allocation mode (oc_propagate_batch_noalloc_dev_c64): while `arena`
+    // is set, every device buffer, table and descriptor of the call is carved
+    // from the device's scratch array [scratch, scratch + scratch_cap) and
+    // filled from host copies kept in arena_keep until the call has
+    // synchronised; arena_tables holds this call's tables by key.
+    void* scratch = nullptr;  // this device's scratch array (hipGetSymbolAddress at creation)
+    size_t scratch_cap = 0;
+    bool arena = false;
+    size_t arena_used = 0;
+    std::vector<std::vector<unsigned char>> arena_keep;
+    std::map<std::string, const void*> arena_tables;
     std::map<std::pair<int, int>, int> fft_variant;  // (n, double?) -> fast-kernel layout variant
     std::map<int, std::pair<int, int>> sep_variant;  // n -> (column, row) layouts of the two-pass Fresnel
     // The autotuned layout if there is one, else the layout that timed
@@ -197,6 +208,23 @@ void collect_timing(oc_context* ctx);
     } while (0)
 
 int ensure_buf(oc_context* ctx, const std::string& name, size_t bytes, void** out);
+// No-allocation mode: `bytes` (256-byte aligned) of the scratch array, or
+// OC_ERR_INVALID_ARGUMENT when it is exhausted.
+int arena_alloc(oc_context* ctx, size_t bytes, void** out);
+// No-allocation mode: a scratch copy of host data (queued on the stream; the
+// host copy is kept until the call ends).
+int arena_upload(oc_context* ctx, const void* host, size_t bytes, const void** out);
+// Scratch bytes a no-allocation batch of these (unprepared) problems needs at
+// most; every allocation of the call fits in it.
+size_t arena_need(const std::vector<struct Prob>& probs, bool dbl);
+// Looks up / stores a device table by key: the context's cache, or this
+// call's scratch copies in the no-allocation mode.
+const void* find_table(oc_context* ctx, const std::string& key);
+int store_table(oc_context* ctx, const std::string& key, const void* host, size_t bytes, const void** out);
+// Problems of a batch call from its descriptors (engine and mask checks).
+int probs_from_descs(const oc_prop_desc* d, size_t count, std::vector<struct Prob>& v);
+// Attaches this device's scratch array to a new context (noalloc.hip).
+int noalloc_attach(oc_context* ctx);
 // Staging slot `slot` with room for `bytes` (the caller has waited for the
 // slot's previous copy).
 int ensure_pinned(oc_context* ctx, int slot, size_t bytes);

@@ -104,6 +104,7 @@ EXPORTS = {
     "oc_adam_step_dev": ([vp, c_sz, vp, vp, vp, vp, vp, vp, C.c_uint64, P(T.AdamConfig)], C.c_int),
     "oc_propagate_chain_dev_c64": ([vp, P(T.PropDesc), P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_chain_dev_c128": ([vp, P(T.PropDesc), P(T.PropDesc), c_sz], C.c_int),
+    "oc_propagate_batch_noalloc_dev_c64": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_batch_submit_dev_c64": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_batch_submit_dev_c128": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_context_wait": ([vp], C.c_int),
@@ -464,6 +465,14 @@ def propagate_batch_dev(descs, double=False, ctx=None):
     _torch().cuda.synchronize()
     fn = lib.oc_propagate_batch_dev_c128 if double else lib.oc_propagate_batch_dev_c64
     return fn(ctx.h, descs, len(descs))
+
+
+def propagate_batch_noalloc_dev(descs, ctx=None):
+    """oc_propagate_batch_noalloc_dev_c64: the complex64 batch on the
+    library's scratch array (no device allocation).  Returns status."""
+    ctx = ctx or context()
+    _torch().cuda.synchronize()
+    return lib.oc_propagate_batch_noalloc_dev_c64(ctx.h, descs, len(descs))
 
 
 def propagate_chain_dev(first, second, double=False, ctx=None):
