@@ -621,7 +621,9 @@ int oc_adjoint_push_back(oc_context* ctx, const oc_design_step* steps,
    scene with MseToTarget (merit.rs:112-166): render, merit, seed and
    push_back in one forward and one reverse sweep.  Device pointers (mask
    values included).  dphi_out (device, n_elements*nx*ny) may be NULL: then
-   only the forward render and merit run.  merit_out (host) may be NULL. */
+   only the forward render and merit run.  merit_out (host) may be NULL.
+   OC_INV_RENDER for a failing forward trace, OC_INV_ADJOINT for a failing
+   adjoint propagation. */
 int oc_adjoint_grid_gradient_dev_c128(oc_context* ctx,
                                       const oc_design_step* steps,
                                       size_t nsteps, const oc_band_geom* geom,
@@ -629,6 +631,39 @@ int oc_adjoint_grid_gradient_dev_c128(oc_context* ctx,
                                       const oc_asm_config* cfg,
                                       const double* target, double* dphi_out,
                                       double* merit_out);
+
+/* Replaces adjoint::grid_gradient (adjoint.rs:268-340) with MseToTarget
+   (merit.rs:112-166) over every (band, scene) frame of a render: the forward
+   trace of all frames as one batch per beamline step (per-plane
+   checkpoints), the seeds, one reverse sweep for all frames, and the
+   per-frame planes added in frame order, band-major (adjoint.rs:310-332).
+   As the reference, each selected frame is seeded with obs_gradient's
+   2 (I_f - T)/n (merit.rs:143-165), so with several frames selected the
+   gradient is that of the sum of the frames' own MSEs; an unselected frame
+   contributes nothing, and a selector that picks no frame (out of range)
+   gives all-zero gradients (the reference returns no planes).
+   designs: [band][nsteps] (every band's design has the same structure;
+   mask values on the device); inputs: [band][scene] fields (device,
+   complex128 / complex64); target: nx*ny f64 (device); sel_band,
+   sel_scene: the selector (-1 = all); dphi_out: n_elements*nx*ny f64
+   (device) or NULL for the render and merit only; merit_out (host, may be
+   NULL): MseToTarget::evaluate of the rendered observation (the selected
+   frames' intensities summed), NaN for a selector out of range.  Errors:
+   OC_INV_RENDER for an empty band or scene list or a failing forward trace,
+   OC_INV_ADJOINT for a failing adjoint propagation, OC_ERR_UNSUPPORTED when
+   the bands' designs differ in structure. */
+int oc_adjoint_grid_gradient_frames_dev_c128(
+    oc_context* ctx, const oc_band_geom* geom, const double* bands,
+    size_t n_bands, size_t n_scenes, const oc_design_step* designs,
+    size_t nsteps, const oc_asm_config* cfg, const void* inputs,
+    const double* target, int32_t sel_band, int32_t sel_scene,
+    double* dphi_out, double* merit_out);
+int oc_adjoint_grid_gradient_frames_dev_c64(
+    oc_context* ctx, const oc_band_geom* geom, const double* bands,
+    size_t n_bands, size_t n_scenes, const oc_design_step* designs,
+    size_t nsteps, const oc_asm_config* cfg, const void* inputs,
+    const double* target, int32_t sel_band, int32_t sel_scene,
+    double* dphi_out, double* merit_out);
 
 /* Renderer::render (crates/optics-inverse/src/observe.rs:213-314) for
    designs whose elements are PhaseMasks.  The caller keeps the two closures

@@ -53,7 +53,7 @@ int oc_context_create(int device, oc_context** out) {
         c->num_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) return OC_ERR_DEVICE;
     c->stream = c->own;
-    if (ocl::set_lds_limits() != hipSuccess) return OC_ERR_DEVICE;
+    if (ocl::set_lds_limits() != hipSuccess || ocl::big_set_lds_limits() != hipSuccess) return OC_ERR_DEVICE;
     if (ocl::fast_set_lds_limits_c64() != hipSuccess || ocl::fast_set_lds_limits_c128() != hipSuccess)
         return OC_ERR_DEVICE;
     if (noalloc_attach(c.get()) != OC_OK) {

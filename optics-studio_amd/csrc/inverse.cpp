@@ -1,4 +1,6 @@
 // Gerchberg-Saxton (drop-in and FFT form) and the adjoint gradient.
+#include <climits>
+
 #include "runtime.hpp"
 
 using namespace ocr;
@@ -410,45 +412,81 @@ int oc_adjoint_transmission(oc_context* ctx, const oc_band_geom* g, const oc_pha
 
 namespace {
 
-// Forward tape + reverse sweep on device (adjoint.rs:140-246).  Mask values
-// are device pointers.  `dmdi` (device, f64) seeds w = dmdi * U_det; when
-// null, the MseToTarget seed 2(|U|^2 - target)/n is formed from `target` and
-// the merit is returned in *merit.  grads: n_elements * n (device, zeroed
-// here); may be null for a forward render + merit only.
-int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, const oc_band_geom& g, const void* d_input,
-                 const oc_asm_config& cfg, const double* dmdi, const double* target, double* grads, double* merit) {
-    const size_t n = g.nx * g.ny;
+// The frames an adjoint run traces and pushes back: frame f has wavelength
+// lam[f], design steps step(f, s) (every frame's design has frame 0's
+// structure: element presence and zero gaps) and input field input(f)
+// (device).  Frames are (band, scene) pairs, f = b * ns + s.
+struct AdjFrames {
+    size_t F = 1, nb = 1, ns = 1;
+    std::vector<double> lam;
+    std::function<const oc_design_step&(size_t, size_t)> step;
+    std::function<const void*(size_t)> input;
+    int sel_band = -1, sel_scene = -1;  // MseToTarget selector (-1 = all)
+};
+
+// Forward tape + reverse sweep on device (adjoint.rs:140-246) for every
+// frame at once: each propagation step is one batch over the frames.  Mask
+// values are device pointers.  `dmdi` (device, f64, one frame) seeds
+// w = dmdi * U_det; when null, the MseToTarget seeds of the selected frames
+// (merit.rs:143-165) are formed from `target` and the merit of their summed
+// intensity (merit.rs:124-137) is returned in *merit.  grads: n_elements * n
+// (device); several frames are pushed back into per-frame planes that are
+// then added in frame order (adjoint.rs:310-332).  May be null for a forward
+// render + merit only.  fwd_err: the status of a failing forward trace
+// (InverseError::Render for the renderer's trace, ::Adjoint for push_back's
+// tape).
+int adjoint_impl(oc_context* ctx, bool dbl, const oc_band_geom& g, const AdjFrames& fr, size_t nsteps,
+                 const oc_asm_config& cfg, const double* dmdi, const double* target, double* grads, double* merit,
+                 int fwd_err) {
+    const size_t n = g.nx * g.ny, F = fr.F, esz = dbl ? 16 : 8, fb = n * esz;
     size_t ne = 0;
-    for (size_t s = 0; s < nsteps; ++s) ne += steps[s].has_element ? 1 : 0;
+    for (size_t s = 0; s < nsteps; ++s) ne += fr.step(0, s).has_element ? 1 : 0;
+    for (size_t f = 1; f < F; ++f)
+        for (size_t s = 0; s < nsteps; ++s) {
+            const oc_design_step &d = fr.step(f, s), &d0 = fr.step(0, s);
+            if (d.has_element != d0.has_element || (d.z_after == 0.0) != (d0.z_after == 0.0))
+                return fail(ctx, OC_ERR_UNSUPPORTED, "the bands' designs differ in structure");
+        }
     // the context's cached buffers (a gradient per call in an inverse loop
     // must not pay an allocation and a free)
-    void *cur, *nxt, *ckpt = nullptr, *sums, *flags;
+    void *cur, *nxt, *ckpt = nullptr, *sums, *flags, *slab = nullptr;
     int st;
-    if ((st = ensure_buf(ctx, "adj_cur", n * 16, &cur)) || (st = ensure_buf(ctx, "adj_nxt", n * 16, &nxt)) ||
+    if ((st = ensure_buf(ctx, "adj_cur", F * fb, &cur)) || (st = ensure_buf(ctx, "adj_nxt", F * fb, &nxt)) ||
         (st = ensure_buf(ctx, "adj_sums", sizeof(double) * (8 + ocd::kSeedParts), &sums)) ||
-        (st = ensure_buf(ctx, "adj_flags", sizeof(int) * (2 * nsteps + 1), &flags)))
+        (st = ensure_buf(ctx, "adj_flags", sizeof(int) * (2 * nsteps + 1) * F, &flags)))
         return st;
-    if (ne && (st = ensure_buf(ctx, "adj_ckpt", ne * n * 16, &ckpt))) return st;
+    if (ne && (st = ensure_buf(ctx, "adj_ckpt", ne * F * fb, &ckpt))) return st;
+    if (ne && grads && F > 1 && (st = ensure_buf(ctx, "adj_slab", sizeof(double) * F * ne * n, &slab))) return st;
+    // A field set: frame f's field at base + f * fb, or the inputs (base null).
+    auto at = [&](const unsigned char* base, size_t f) -> const void* { return base ? base + f * fb : fr.input(f); };
+    auto geom_f = [&](size_t f) {
+        oc_band_geom gi = g;
+        gi.lambda = fr.lam[f];
+        return gi;
+    };
     // Every step is queued without waiting; the finiteness flags of all
     // propagations (zeroed once here) and the merit are read back once at
     // the end.
     std::vector<std::vector<Prob>> queued;
     queued.reserve(2 * nsteps);
     int* dflags = static_cast<int*>(flags);
-    HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, (int)(2 * nsteps + 1), ctx->stream));
-    auto propagate = [&](Prob p) -> int {
-        queued.push_back({p});
-        int s2 = launch_problems(ctx, true, queued.back(), dflags + (queued.size() - 1), false);
-        if (s2 == OC_OK) s2 = queued.back()[0].status;
-        return s2;
+    HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, (int)((2 * nsteps + 1) * F), ctx->stream));
+    auto propagate = [&](std::vector<Prob> ps, int err) -> int {
+        queued.push_back(std::move(ps));
+        int s2 = launch_problems(ctx, dbl, queued.back(), dflags + (queued.size() - 1) * F, false);
+        if (s2 == OC_ERR_DEVICE || s2 == OC_ERR_OUT_OF_MEMORY) return s2;
+        for (const Prob& p : queued.back())
+            if (s2 != OC_OK || p.status != OC_OK) return fail(ctx, err, "propagation failed");
+        return OC_OK;
     };
-    // forward, the mask fused into the propagation.  The field arriving at
-    // element e is its checkpoint: the input itself for a first-step
-    // element, else the previous step writes it straight into slot e.
+    // forward, the mask fused into the propagation.  The fields arriving at
+    // element e are its checkpoints: the inputs themselves for a first-step
+    // element, else the previous step writes them straight into slot e.
     // Other intermediates alternate between two scratch buffers.
-    std::vector<const void*> ck_at(ne, nullptr);
-    void* scratch[2] = {cur, nxt};
-    const void* src = d_input;
+    unsigned char* const sa = static_cast<unsigned char*>(cur);
+    unsigned char* const sb = static_cast<unsigned char*>(nxt);
+    std::vector<const unsigned char*> ck_at(ne, nullptr);
+    const unsigned char* src = nullptr;  // the inputs
     size_t e = 0;
     // consecutive forward propagations are launched together, linked
     // through their rows when they qualify (each checkpoint still written)
@@ -456,97 +494,175 @@ int adjoint_impl(oc_context* ctx, const oc_design_step* steps, size_t nsteps, co
     auto flush = [&]() -> int {
         if (run.empty()) return OC_OK;
         std::vector<int> off;
-        for (size_t k = 0; k < run.size(); ++k) off.push_back((int)(queued.size() + k));
-        int sl = run.size() > 1 ? launch_linked(ctx, true, run, dflags, off, queued, true) : 1;
+        for (size_t k = 0; k < run.size(); ++k) off.push_back((int)((queued.size() + k) * F));
+        int sl = run.size() > 1 ? launch_linked(ctx, dbl, run, dflags, off, queued, true) : 1;
         if (sl < 0) return -sl;
         if (sl > 0)
             for (auto& r : run)
-                if (propagate(r[0]) != OC_OK) return fail(ctx, OC_INV_ADJOINT, "forward propagation failed");
+                if (int s2 = propagate(r, fwd_err)) return s2;
         for (size_t k = queued.size() - run.size(); k < queued.size(); ++k)
-            if (queued[k][0].status != OC_OK) return fail(ctx, OC_INV_ADJOINT, "forward propagation failed");
+            for (const Prob& p : queued[k])
+                if (p.status != OC_OK) return fail(ctx, fwd_err, "forward propagation failed");
         run.clear();
         return OC_OK;
     };
     for (size_t s = 0; s < nsteps; ++s) {
-        const oc_design_step& stp = steps[s];
-        if (stp.has_element) ck_at[e++] = src;
-        const bool next_elem = s + 1 < nsteps && steps[s + 1].has_element;
-        void* dst = next_elem ? static_cast<unsigned char*>(ckpt) + e * n * 16
-                              : (src == scratch[0] ? scratch[1] : scratch[0]);
-        if (stp.z_after != 0.0) {
-            run.push_back({make_prob(&g, stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
-                                     cfg.max_padding, stp.medium, stp.has_element ? &stp.mask : nullptr, src, dst)});
+        const oc_design_step& d0 = fr.step(0, s);
+        if (d0.has_element) ck_at[e++] = src;
+        const bool next_elem = s + 1 < nsteps && fr.step(0, s + 1).has_element;
+        unsigned char* dst = next_elem ? static_cast<unsigned char*>(ckpt) + e * F * fb : (src == sa ? sb : sa);
+        if (d0.z_after != 0.0) {
+            std::vector<Prob> ps;
+            for (size_t f = 0; f < F; ++f) {
+                const oc_design_step& d = fr.step(f, s);
+                const oc_band_geom gi = geom_f(f);
+                ps.push_back(make_prob(&gi, d.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
+                                       cfg.max_padding, d.medium, d.has_element ? &d.mask : nullptr, at(src, f),
+                                       dst + f * fb));
+            }
+            run.push_back(std::move(ps));
             src = dst;
             continue;
         }
         if ((st = flush())) return st;
-        if (stp.has_element) {
-            const int fs = mask_footprint(g, stp.mask);
-            if (fs) return OC_INV_ADJOINT;
-            MaskDev md;
-            if ((st = mask_tables(ctx, g, stp.mask, stp.mask.values, &md))) return st;
-            HIP_TRY(ctx, ocl::launch_mask_apply_c128(src, dst, md, (int)g.nx, (int)g.ny, nullptr, ctx->stream));
+        if (d0.has_element) {
+            for (size_t f = 0; f < F; ++f) {
+                const oc_design_step& d = fr.step(f, s);
+                if (mask_footprint(g, d.mask)) return fail(ctx, fwd_err, "phase mask does not fit the grid");
+                MaskDev md;
+                if ((st = mask_tables(ctx, g, d.mask, d.mask.values, &md))) return st;
+                HIP_TRY(ctx, dbl ? ocl::launch_mask_apply_c128(at(src, f), dst + f * fb, md, (int)g.nx, (int)g.ny,
+                                                               nullptr, ctx->stream)
+                                 : ocl::launch_mask_apply_c64(at(src, f), dst + f * fb, md, (int)g.nx, (int)g.ny,
+                                                              nullptr, ctx->stream));
+            }
             src = dst;
         } else if (next_elem) {  // an empty step before an element: its field is that element's checkpoint
-            HIP_TRY(ctx, hipMemcpyAsync(dst, src, n * 16, hipMemcpyDeviceToDevice, ctx->stream));
+            for (size_t f = 0; f < F; ++f)
+                HIP_TRY(ctx, hipMemcpyAsync(dst + f * fb, at(src, f), fb, hipMemcpyDeviceToDevice, ctx->stream));
             src = dst;
         }
     }
     if ((st = flush())) return st;
-    // the detector field into a scratch buffer of its own for the sweep
-    cur = const_cast<void*>(src);
-    nxt = (src == scratch[0]) ? scratch[1] : scratch[0];
-    // seed at the detector (into a scratch buffer: the detector field may be
-    // the caller's input or a checkpoint)
+    // the detector fields, contiguous in a scratch buffer
+    const unsigned char* det = src;
+    if (!det) {
+        for (size_t f = 0; f < F; ++f)
+            HIP_TRY(ctx, hipMemcpyAsync(sa + f * fb, fr.input(f), fb, hipMemcpyDeviceToDevice, ctx->stream));
+        det = sa;
+    }
+    unsigned char* co = det == sa ? sb : sa;
+    // seed at the detector
     if (dmdi) {
         HIP_TRY(ctx, hipMemsetAsync(sums, 0, 64, ctx->stream));
-        HIP_TRY(ctx, ocl::launch_seed(true, cur, dmdi, nxt, n, ctx->stream));
+        HIP_TRY(ctx, ocl::launch_seed(dbl, det, dmdi, co, n, ctx->stream));
+    } else if (F == 1 && (fr.sel_band == -1 || fr.sel_band == 0) && (fr.sel_scene == -1 || fr.sel_scene == 0)) {
+        HIP_TRY(ctx, ocl::launch_seed_mse(dbl, det, target, co, n, static_cast<double*>(sums), ctx->stream));
     } else {
-        HIP_TRY(ctx, ocl::launch_seed_mse(true, cur, target, nxt, n, static_cast<double*>(sums), ctx->stream));
+        HIP_TRY(ctx, ocl::launch_seed_frames(dbl, det, target, co, n, (int)fr.nb, (int)fr.ns, fr.sel_band,
+                                             fr.sel_scene, static_cast<double*>(sums), ctx->stream));
     }
-    std::swap(cur, nxt);
     auto finish = [&]() -> int {
         double hs[4];
-        std::vector<int> hf(queued.size() + 1, 0);
+        std::vector<int> hf(queued.size() * F + 1, 0);
         HIP_TRY(ctx, hipMemcpyAsync(hs, sums, 32, hipMemcpyDeviceToHost, ctx->stream));
         if (!queued.empty())
-            HIP_TRY(ctx, hipMemcpyAsync(hf.data(), dflags, sizeof(int) * queued.size(), hipMemcpyDeviceToHost,
+            HIP_TRY(ctx, hipMemcpyAsync(hf.data(), dflags, sizeof(int) * queued.size() * F, hipMemcpyDeviceToHost,
                                         ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         collect_timing(ctx);
         if (merit) *merit = hs[0] / (double)n;
-        for (size_t i = 0; i < queued.size(); ++i)
-            if (hf[i]) return fail(ctx, OC_INV_ADJOINT, "non-finite samples in a propagation");
+        for (size_t i = 0; i < queued.size() * F; ++i)
+            if (hf[i]) return fail(ctx, fwd_err, "non-finite samples in a propagation");
         return OC_OK;
     };
     if (!grads) return finish();
-    HIP_TRY(ctx, hipMemsetAsync(grads, 0, ne * n * 8, ctx->stream));
+    double* const gbase = slab ? static_cast<double*>(slab) : grads;
+    if (ne) HIP_TRY(ctx, hipMemsetAsync(gbase, 0, (slab ? F : 1) * ne * n * 8, ctx->stream));
+    auto gplane = [&](size_t f, size_t el) { return gbase + ((slab ? f * ne : 0) + el) * n; };
     // reverse sweep
+    unsigned char* c0 = co;
+    unsigned char* c1 = co == sa ? sb : sa;
     e = ne;
     for (size_t s = nsteps; s-- > 0;) {
-        const oc_design_step& stp = steps[s];
-        if (stp.has_element) --e;
-        const void* ck = stp.has_element ? ck_at[e] : nullptr;
-        double* gr = stp.has_element ? grads + e * n : nullptr;
-        if (stp.z_after != 0.0) {
-            Prob pb = make_prob(&g, -stp.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding, cfg.max_padding,
-                                stp.medium, nullptr, cur, nxt);
-            if (stp.has_element) {
-                pb.mask = &stp.mask;  // mask tables for the epilogue (no pre-op)
-                pb.ops.pre = PRE_NONE;
-                pb.ops.post = POST_ADJ_GRAD;
-                pb.ops.ckpt = ck;
-                pb.ops.grad = gr;
+        const oc_design_step& d0 = fr.step(0, s);
+        if (d0.has_element) --e;
+        if (d0.z_after != 0.0) {
+            std::vector<Prob> ps;
+            for (size_t f = 0; f < F; ++f) {
+                const oc_design_step& d = fr.step(f, s);
+                const oc_band_geom gi = geom_f(f);
+                Prob pb = make_prob(&gi, -d.z_after, ENG_ASM, cfg.band_limited, cfg.strict, cfg.padding,
+                                    cfg.max_padding, d.medium, nullptr, c0 + f * fb, c1 + f * fb);
+                if (d.has_element) {
+                    pb.mask = &d.mask;  // mask tables for the epilogue (no pre-op)
+                    pb.ops.pre = PRE_NONE;
+                    pb.ops.post = POST_ADJ_GRAD;
+                    pb.ops.ckpt = at(ck_at[e], f);
+                    pb.ops.grad = gplane(f, e);
+                }
+                ps.push_back(pb);
             }
-            if (propagate(pb) != OC_OK) return fail(ctx, OC_INV_ADJOINT, "adjoint propagation failed");
-            std::swap(cur, nxt);
-        } else if (stp.has_element) {
-            MaskDev md;
-            if ((st = mask_tables(ctx, g, stp.mask, stp.mask.values, &md))) return st;
-            HIP_TRY(ctx, ocl::launch_adj_elem(true, cur, ck, gr, md, (int)g.nx, (int)g.ny, ctx->stream));
+            if (int s2 = propagate(std::move(ps), OC_INV_ADJOINT)) return s2;
+            std::swap(c0, c1);
+        } else if (d0.has_element) {
+            for (size_t f = 0; f < F; ++f) {
+                const oc_design_step& d = fr.step(f, s);
+                MaskDev md;
+                if ((st = mask_tables(ctx, g, d.mask, d.mask.values, &md))) return st;
+                HIP_TRY(ctx, ocl::launch_adj_elem(dbl, c0 + f * fb, at(ck_at[e], f), gplane(f, e), md, (int)g.nx,
+                                                  (int)g.ny, ctx->stream));
+            }
         }
     }
+    if (slab && ne) HIP_TRY(ctx, ocl::launch_frame_sum(static_cast<double*>(slab), grads, ne * n, (int)F, ctx->stream));
     return finish();
+}
+
+// One frame (band and scene) of geometry g.
+AdjFrames one_frame(const oc_band_geom& g, const oc_design_step* steps, const void* input) {
+    AdjFrames fr;
+    fr.lam = {g.lambda};
+    fr.step = [steps](size_t, size_t s) -> const oc_design_step& { return steps[s]; };
+    fr.input = [input](size_t) { return input; };
+    return fr;
+}
+
+int grid_gradient_frames(oc_context* ctx, bool dbl, const oc_band_geom* g, const double* bands, size_t nb,
+                         size_t ns, const oc_design_step* designs, size_t nsteps, const oc_asm_config* cfg,
+                         const void* inputs, const double* target, int32_t sel_band, int32_t sel_scene,
+                         double* dphi_out, double* merit_out) {
+    OC_CONTEXT_LOCK(ctx);
+    if (!ctx || !g || !cfg || !target || (!designs && nsteps) || (nb && ns && (!bands || !inputs)))
+        return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    evict_tables(ctx);
+    // observe.rs:214-218 (Renderer::render), then the frame length
+    if (nb == 0 || ns == 0) return fail(ctx, OC_INV_RENDER, "renderer needs at least one band and one scene");
+    const size_t n = g->nx * g->ny;
+    if (n == 0) return fail(ctx, OC_INV_RENDER, "empty grid");
+    const size_t esz = dbl ? 16 : 8;
+    AdjFrames fr;
+    fr.F = nb * ns;
+    fr.nb = nb;
+    fr.ns = ns;
+    for (size_t f = 0; f < fr.F; ++f) fr.lam.push_back(bands[f / ns]);
+    fr.step = [designs, nsteps, ns](size_t f, size_t s) -> const oc_design_step& {
+        return designs[(f / ns) * nsteps + s];
+    };
+    fr.input = [inputs, n, esz](size_t f) -> const void* {
+        return static_cast<const unsigned char*>(inputs) + f * n * esz;
+    };
+    // a selector that picks no frame: obs_gradient seeds nothing, so no
+    // plane (adjoint.rs:297-304) and all-zero gradients; evaluate would fail
+    const bool in_range = sel_band >= -1 && sel_scene >= -1 && (sel_band < 0 || (size_t)sel_band < nb) &&
+                          (sel_scene < 0 || (size_t)sel_scene < ns);
+    fr.sel_band = in_range ? sel_band : INT_MAX;  // matches no frame
+    fr.sel_scene = in_range ? sel_scene : INT_MAX;
+    double m = 0.0;
+    int st = adjoint_impl(ctx, dbl, *g, fr, nsteps, *cfg, nullptr, target, dphi_out, &m, OC_INV_RENDER);
+    if (merit_out) *merit_out = in_range ? m : std::nan("");
+    return st;
 }
 
 }  // namespace
@@ -581,8 +697,8 @@ int oc_adjoint_push_back(oc_context* ctx, const oc_design_step* steps, size_t ns
         HIP_TRY(ctx, hipMemcpy(mv, s.mask.values, mn * 8, hipMemcpyHostToDevice));
         s.mask.values = static_cast<const double*>(mv);
     }
-    st = adjoint_impl(ctx, ds.data(), nsteps, *g, din, *cfg, static_cast<double*>(ddm), nullptr,
-                      static_cast<double*>(dgr), nullptr);
+    st = adjoint_impl(ctx, true, *g, one_frame(*g, ds.data(), din), nsteps, *cfg, static_cast<double*>(ddm), nullptr,
+                      static_cast<double*>(dgr), nullptr, OC_INV_ADJOINT);
     if (st) return st;
     if (ne) HIP_TRY(ctx, hipMemcpy(dphi_out, dgr, ne * n * 8, hipMemcpyDeviceToHost));
     return OC_OK;
@@ -597,9 +713,28 @@ int oc_adjoint_grid_gradient_dev_c128(oc_context* ctx, const oc_design_step* ste
     evict_tables(ctx);
     if (g->nx * g->ny == 0) return OC_INV_ADJOINT;
     double m = 0.0;
-    int st = adjoint_impl(ctx, steps, nsteps, *g, input, *cfg, nullptr, target, dphi_out, &m);
+    int st = adjoint_impl(ctx, true, *g, one_frame(*g, steps, input), nsteps, *cfg, nullptr, target, dphi_out, &m,
+                          OC_INV_RENDER);
     if (merit_out) *merit_out = m;
     return st;
+}
+
+int oc_adjoint_grid_gradient_frames_dev_c128(oc_context* ctx, const oc_band_geom* geom, const double* bands,
+                                             size_t n_bands, size_t n_scenes, const oc_design_step* designs,
+                                             size_t nsteps, const oc_asm_config* cfg, const void* inputs,
+                                             const double* target, int32_t sel_band, int32_t sel_scene,
+                                             double* dphi_out, double* merit_out) {
+    return grid_gradient_frames(ctx, true, geom, bands, n_bands, n_scenes, designs, nsteps, cfg, inputs, target,
+                                sel_band, sel_scene, dphi_out, merit_out);
+}
+
+int oc_adjoint_grid_gradient_frames_dev_c64(oc_context* ctx, const oc_band_geom* geom, const double* bands,
+                                            size_t n_bands, size_t n_scenes, const oc_design_step* designs,
+                                            size_t nsteps, const oc_asm_config* cfg, const void* inputs,
+                                            const double* target, int32_t sel_band, int32_t sel_scene,
+                                            double* dphi_out, double* merit_out) {
+    return grid_gradient_frames(ctx, false, geom, bands, n_bands, n_scenes, designs, nsteps, cfg, inputs, target,
+                                sel_band, sel_scene, dphi_out, merit_out);
 }
 
 }  // extern "C"

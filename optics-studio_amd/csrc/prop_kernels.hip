@@ -517,6 +517,57 @@ __global__ __launch_bounds__(256) void k_seed_mse(const cpx<R>* __restrict__ u, 
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
+// MseToTarget over the (band, scene) frames of a render, [frame][n] fields
+// with frame f = b ns + s (merit.rs:112-166): a selected frame's intensity
+// joins the per-pixel sum Selector::frame takes (frames in order) and gets
+// its own seed 2 (I_f - T)/n times its field (obs_gradient); an unselected
+// frame gets a zero seed.  part[workgroup] = the workgroup's share of
+// sum_k (frame_k - T_k)^2.
+template <typename R>
+__global__ __launch_bounds__(256) void k_seed_frames(const cpx<R>* __restrict__ u, const double* __restrict__ target,
+                                                     cpx<R>* __restrict__ out, size_t n, int nb, int ns, int sel_b,
+                                                     int sel_s, double nf, double* __restrict__ part) {
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double t = target[i];
+        double frame = 0.0;
+        for (int b = 0; b < nb; ++b)
+            for (int s = 0; s < ns; ++s) {
+                const size_t f = (size_t)b * ns + s;
+                const cpx<R> v = u[f * n + i];
+                cpx<R> o = {R(0), R(0)};
+                if ((sel_b < 0 || sel_b == b) && (sel_s < 0 || sel_s == s)) {
+                    const double re = v.re, im = v.im;
+                    const double inten = re * re + im * im;
+                    frame += inten;
+                    const double g = 2.0 * (inten - t) / nf;
+                    o = {(R)(g * re), (R)(g * im)};
+                }
+                out[f * n + i] = o;
+            }
+        const double d = frame - t;
+        acc += d * d;
+    }
+    __shared__ double sm[256];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] += sm[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// out[i] = slab[0][i] + slab[1][i] + ... in frame order (the per-frame
+// gradient planes added as adjoint.rs:310-332 does).
+__global__ void k_frame_sum(const double* __restrict__ slab, double* __restrict__ out, size_t m, int frames) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
+        double s = slab[i];
+        for (int f = 1; f < frames; ++f) s += slab[(size_t)f * m + i];
+        out[i] = s;
+    }
+}
+
 // sums[0] = the np workgroup partials added in a fixed order (thread t adds
 // t, t + 256, ... in order, then an LDS tree).
 __global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int np,
@@ -699,6 +750,27 @@ hipError_t launch_seed_mse(bool dbl, const void* u, const double* target, void* 
         hipLaunchKernelGGL(k_seed_mse<float>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<float>*>(u),
                            target, static_cast<cpx<float>*>(out), n, (double)n, sums + 8);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, sums + 8, grid, sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_seed_frames(bool dbl, const void* u, const double* target, void* out, size_t n, int nb, int ns,
+                              int sel_b, int sel_s, double* sums, hipStream_t st) {
+    int grid = (int)std::min<size_t>((n + 255) / 256, kSeedParts);
+    if (grid < 1) grid = 1;
+    if (dbl)
+        hipLaunchKernelGGL(k_seed_frames<double>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<double>*>(u),
+                           target, static_cast<cpx<double>*>(out), n, nb, ns, sel_b, sel_s, (double)n, sums + 8);
+    else
+        hipLaunchKernelGGL(k_seed_frames<float>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<float>*>(u),
+                           target, static_cast<cpx<float>*>(out), n, nb, ns, sel_b, sel_s, (double)n, sums + 8);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, sums + 8, grid, sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_frame_sum(const double* slab, double* out, size_t m, int frames, hipStream_t st) {
+    int grid = (int)std::min<size_t>((m + 255) / 256, 16384);
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_frame_sum, dim3(grid), dim3(256), 0, st, slab, out, m, frames);
     return hipGetLastError();
 }
 

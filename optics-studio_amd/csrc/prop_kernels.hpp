@@ -157,4 +157,21 @@ struct ProbDev {
     int* flag;  // bit 0: non-finite input sample; bit 1: non-finite output
 };
 
+// One batch of 1-D transforms over a buffer (the multi-pass path for padded
+// lengths beyond the on-chip transform).  Line l starts at
+// (l / inner) * outer + (l % inner) * inner_step and its element e sits e *
+// elem further on (all in elements), in the input and output views.
+struct LineView {
+    long long outer, inner_step, elem;
+};
+struct LineSpec {
+    int n;             // line length
+    int cnt;           // lines per workgroup iteration
+    long long count;   // lines
+    long long inner;   // lines per outer index
+    LineView in, out;
+    int tw_len;        // > 0: output e times W_{tw_len}^{m e}, m = l / inner (tw_outer) or l % inner
+    int tw_outer;
+};
+
 }  // namespace ocd

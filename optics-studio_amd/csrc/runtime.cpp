@@ -863,6 +863,147 @@ void evict_tables(oc_context* ctx) {
     }
 }
 
+// A length the generic LDS transform holds on chip: a plan within the
+// stage limit and the LDS budget (ping-pong plans twice over).
+static bool line_fits(int n, size_t esz) {
+    std::vector<int> rad;
+    bool generic;
+    if (n <= 0 || !make_radices(n, rad, generic)) return false;
+    return (size_t)lds_stride(n) * (generic ? 2 : 1) * esz <= kLdsBudget;
+}
+
+bool fits_on_chip(const Prob& p, bool dbl) {
+    const size_t esz = dbl ? 16 : 8;
+    std::vector<int> rad;
+    bool gy;
+    if (!line_fits((int)p.nxp, esz) || !make_radices((int)p.nyp, rad, gy)) return false;
+    return (size_t)lds_stride((int)p.nyp) * (gy ? 2 : 1) * esz * (conv_engine(p.engine) ? 2 : 1) <= kLdsBudget;
+}
+
+// n = a * (n / a) with both factors on chip and a as close to sqrt(n) as
+// possible; 0 when n has no such split (a prime beyond the on-chip size).
+static int split_len(int n, size_t esz) {
+    int best = 0;
+    double bd = 1e300;
+    for (int a = 2; (long long)a * a <= n; ++a) {
+        if (n % a) continue;
+        const int b = n / a;
+        if (!line_fits(a, esz) || !line_fits(b, esz)) continue;
+        const double d = std::fabs(std::log((double)b / (double)a));
+        if (d < bd) {
+            bd = d;
+            best = a;
+        }
+    }
+    return best;
+}
+
+// The 1-D transforms of every line of `cur` along axis x (rows) or y
+// (columns) of the px x py buffer, direction dir; a length beyond the
+// on-chip transform runs as four steps through `tmp`, and the result then
+// sits in tmp (the two are swapped).
+static int big_axis(oc_context* ctx, bool dbl, void*& cur, void*& tmp, long long px, long long py, bool along_x,
+                    int dir) {
+    const size_t esz = dbl ? 16 : 8;
+    const int L = (int)(along_x ? px : py);
+    const long long lines = along_x ? py : px;  // transforms along the axis
+    // one LDS pass of `count` lines of length n
+    auto pass = [&](const void* in, void* out, int n, long long count, long long inner, LineView vin, LineView vout,
+                    int tw_len, int tw_outer) -> int {
+        FftPlanDev pl;
+        int st = make_plan(ctx, n, dbl, &pl);
+        if (st) return st;
+        const void* twl = nullptr;
+        if (tw_len > 0 && (st = get_twiddles(ctx, tw_len, dbl, &twl))) return st;
+        const size_t line_bytes = (size_t)lds_stride(n) * (pl.generic ? 2 : 1) * esz;
+        int cnt = (int)std::min<size_t>(kLdsBudget / line_bytes, (size_t)std::max(1, 1024 / pl.threads));
+        cnt = std::max(1, std::min(cnt, 16));
+        LineSpec s{n, cnt, count, inner, vin, vout, tw_len, tw_outer};
+        HIP_TRY(ctx, timed(ctx, 5, [&] {
+                    return ocl::big_line(dbl, dir, in, out, s, pl, twl, line_bytes * cnt, ctx->num_cu * 8,
+                                         ctx->stream);
+                }));
+        return OC_OK;
+    };
+    if (line_fits(L, esz)) {
+        // rows: line r at r px; columns: line c at c, element stride px
+        const LineView v = along_x ? LineView{px, 0, 1} : LineView{0, 1, px};
+        return pass(cur, cur, L, lines, along_x ? 1 : px, v, v, 0, 0);
+    }
+    const int L1 = split_len(L, esz);
+    if (!L1) return fail(ctx, OC_ERR_UNSUPPORTED, "a padded length with a prime factor beyond the on-chip transform");
+    const int L2 = L / L1;
+    int st;
+    if (along_x) {
+        // step 1: row r, n2 < L2: elements n1 L2 + n2 (in place), twiddle W_L^{n2 k1}
+        const LineView a{px, 1, L2};
+        if ((st = pass(cur, cur, L1, py * L2, L2, a, a, L, 0))) return st;
+        // step 2: row r, k1 < L1: elements k1 L2 + n2 -> k1 + L1 k2
+        if ((st = pass(cur, tmp, L2, py * L1, L1, LineView{px, L2, 1}, LineView{px, 1, L1}, 0, 0))) return st;
+    } else {
+        // columns, lines numbered (n2 or k1) * px + c so neighbouring lines are neighbouring columns
+        const LineView a{px, 1, (long long)L2 * px};
+        if ((st = pass(cur, cur, L1, px * L2, px, a, a, L, 1))) return st;
+        if ((st = pass(cur, tmp, L2, px * L1, px, LineView{(long long)L2 * px, 1, px},
+                       LineView{px, 1, (long long)L1 * px}, 0, 0)))
+            return st;
+    }
+    std::swap(cur, tmp);
+    return OC_OK;
+}
+
+int launch_big(oc_context* ctx, bool dbl, Prob& p, int* flag) {
+    if (p.engine != ENG_ASM && p.engine != ENG_FRESNEL && !conv_engine(p.engine)) {
+        p.status = fail(ctx, OC_ERR_UNSUPPORTED, "padded length exceeds the on-chip transform size");
+        return OC_OK;
+    }
+    const size_t esz = dbl ? 16 : 8;
+    const long long px = (long long)p.nxp, py = (long long)p.nyp;
+    if ((!line_fits((int)px, esz) && !split_len((int)px, esz)) || (!line_fits((int)py, esz) && !split_len((int)py, esz))) {
+        p.status = fail(ctx, OC_ERR_UNSUPPORTED, "a padded length with a prime factor beyond the on-chip transform");
+        return OC_OK;
+    }
+    const size_t bytes = (size_t)px * (size_t)py * esz;
+    const bool conv = conv_engine(p.engine);
+    void *a, *b, *k = nullptr;
+    int st;
+    if ((st = ensure_buf(ctx, "big_a", bytes, &a)) || (st = ensure_buf(ctx, "big_b", bytes, &b))) return st;
+    if (conv && (st = ensure_buf(ctx, "big_k", bytes, &k))) return st;
+    ProbDev d;
+    std::memset(&d, 0, sizeof d);
+    d.nx = (int)p.g.nx;
+    d.ny = (int)p.g.ny;
+    d.px = (int)px;
+    d.py = (int)py;
+    d.ox = (int)((p.nxp - p.g.nx) / 2);
+    d.oy = (int)((p.nyp - p.g.ny) / 2);
+    d.in = p.in;
+    d.out = p.out;
+    d.scale = 1.0 / (double)(p.nxp * p.nyp);
+    d.flag = flag;
+    d.ops = p.ops;
+    if ((st = engine_tables(ctx, p, dbl, &d.eng))) return st;
+    if (p.mask && (st = mask_tables(ctx, p.g, *p.mask, p.mask->values, &d.mask))) return st;
+    void* kspec = nullptr;
+    void *fld = a, *tmp = b;
+    if (conv) {  // the kernel's spectrum first, in k or a; the field then takes the other
+        void *kc = k, *kt = a;
+        HIP_TRY(ctx, ocl::big_kernel(dbl, d, kc, ctx->stream));
+        if ((st = big_axis(ctx, dbl, kc, kt, px, py, true, -1)) || (st = big_axis(ctx, dbl, kc, kt, px, py, false, -1)))
+            return st;
+        kspec = kc;
+        fld = kt;
+    }
+    HIP_TRY(ctx, ocl::big_embed(dbl, d, fld, ctx->stream));
+    if ((st = big_axis(ctx, dbl, fld, tmp, px, py, true, -1)) || (st = big_axis(ctx, dbl, fld, tmp, px, py, false, -1)))
+        return st;
+    HIP_TRY(ctx, ocl::big_transfer(dbl, d, fld, kspec, ctx->stream));
+    if ((st = big_axis(ctx, dbl, fld, tmp, px, py, false, 1)) || (st = big_axis(ctx, dbl, fld, tmp, px, py, true, 1)))
+        return st;
+    HIP_TRY(ctx, ocl::big_crop(dbl, d, fld, ctx->stream));
+    return OC_OK;
+}
+
 // Queues all problems on the context stream (host-side validation, the
 // standalone finiteness checks, then one batch per padded-size group) with
 // their finiteness flags in dflags[0..n).  No synchronisation.
@@ -901,12 +1042,18 @@ int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* df
         if (p.pending == OC_OK && p.out != p.in) HIP_TRY(ctx, ocl::launch_copy(dbl, p.in, p.out, n, ctx->stream));
     }
     // One batch per padded size that has specialised kernels (uneven grids
-    // then each get the power-of-two kernels), one for all other sizes.
+    // then each get the power-of-two kernels), one for all other sizes that
+    // fit on chip; the rest run one by one through the multi-pass path.
     std::map<std::pair<size_t, size_t>, std::vector<int>> groups;
+    std::vector<int> big;
     for (int i = 0; i < np; ++i) {
         const Prob& p = probs[i];
         if (p.status != OC_OK || !p.launch) continue;
         const bool fast = !ctx->force_generic && p.nxp == p.nyp && ocl::fast_variants((int)p.nxp) > 0;
+        if (!fast && p.engine != ENG_FRAUNHOFER && !fits_on_chip(p, dbl)) {
+            big.push_back(i);
+            continue;
+        }
         // Fraunhofer problems (forward transforms only) never share a batch
         // with the others: their batch has no inverse row pass
         const size_t fh = p.engine == ENG_FRAUNHOFER ? ((size_t)1 << 40) : 0;
@@ -920,6 +1067,14 @@ int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* df
         st = launch_batch(ctx, b);
         if (st) return st;
         ++gi;
+    }
+    for (int i : big) {
+        if (probs[i].nxp > (1u << 20) || probs[i].nyp > (1u << 20)) {
+            probs[i].status = fail(ctx, OC_ERR_UNSUPPORTED, "grid too large");
+            continue;
+        }
+        int st = launch_big(ctx, dbl, probs[i], dflags + i);
+        if (st) return st;
     }
     return OC_OK;
 }
@@ -1077,6 +1232,11 @@ size_t arena_need(const std::vector<Prob>& probs, bool dbl) {
         }
         if (p.check_only || !p.launch) continue;
         const size_t nxp = p.nxp, nyp = p.nyp;
+        if (!fits_on_chip(p, dbl)) {  // the multi-pass path: two or three padded buffers, six plans
+            need += a(nxp * nyp * esz) * (conv_engine(p.engine) ? 3 : 2) + 6 * a(std::max(nxp, nyp) * esz);
+            need += a((nxp + nyp) * 8) * 2 + 256;
+            continue;
+        }
         need += a(ny * nxp * esz);                           // intermediate
         if (conv_engine(p.engine)) need += a(nyp * nxp * esz);  // kernel spectra
         need += a(nxp * esz) + a(nyp * esz);                 // twiddles

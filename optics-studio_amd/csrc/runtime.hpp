@@ -73,6 +73,20 @@ hipError_t launch_block_reorder(bool dbl, const void* src, void* dst, int nx, in
 // sums: 8 + ocd::kSeedParts doubles; sums[0] receives the merit sum.
 hipError_t launch_seed_mse(bool dbl, const void* u, const double* target, void* out, size_t n, double* sums,
                            hipStream_t st);
+// MseToTarget seeds of the [frame][n] fields (frame = b * ns + s) for the
+// band/scene selector (-1 = all); sums as launch_seed_mse (sums[0] = the sum
+// of the squared differences of the selected frames' summed intensity).
+hipError_t launch_seed_frames(bool dbl, const void* u, const double* target, void* out, size_t n, int nb, int ns,
+                              int sel_b, int sel_s, double* sums, hipStream_t st);
+// out[i] = sum over frames of slab[f][i], in frame order (m per frame).
+hipError_t launch_frame_sum(const double* slab, double* out, size_t m, int frames, hipStream_t st);
+hipError_t big_set_lds_limits();
+hipError_t big_line(bool dbl, int dir, const void* in, void* out, const ocd::LineSpec& s, const ocd::FftPlanDev& pl,
+                    const void* twl, size_t lds, int max_grid, hipStream_t st);
+hipError_t big_embed(bool dbl, const ocd::ProbDev& P, void* a, hipStream_t st);
+hipError_t big_kernel(bool dbl, const ocd::ProbDev& P, void* k, hipStream_t st);
+hipError_t big_transfer(bool dbl, const ocd::ProbDev& P, void* a, const void* kspec, hipStream_t st);
+hipError_t big_crop(bool dbl, const ocd::ProbDev& P, const void* a, hipStream_t st);
 hipError_t launch_render_obs(bool dbl, const void* u, double* obs, size_t n, int frames, int nr, int cx, int cy,
                              int cl, double hs, double hl, int* flag, hipStream_t st);
 hipError_t launch_mse(bool dbl, const void* u, const double* target, size_t n, int nr, double* part, double* merit,
@@ -336,6 +350,12 @@ int finish_problems(oc_context* ctx, std::vector<Prob>& probs, const int* dflags
 int launch_linked(oc_context* ctx, bool dbl, std::vector<std::vector<Prob>>& run, int* dflags,
                   const std::vector<int>& off, std::vector<std::vector<Prob>>& queued, bool keep_mid);
 int run_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs);
+// true when the batched passes hold problem p's row and column transforms on
+// chip (else it runs through launch_big)
+bool fits_on_chip(const Prob& p, bool dbl);
+// One prepared problem through the multi-pass path (padded transforms beyond
+// the on-chip size) with its finiteness flag at *flag; 0 when queued.
+int launch_big(oc_context* ctx, bool dbl, Prob& p, int* flag);
 Prob make_prob(const oc_band_geom* g, double z, int engine, int band_limited, int strict, double padding,
                double max_padding, double medium, const oc_phase_mask* mask, const void* in, void* out);
 bool geom_ok(const oc_band_geom* g);

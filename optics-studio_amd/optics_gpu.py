@@ -97,6 +97,10 @@ EXPORTS = {
     "oc_adjoint_push_back": ([vp, P(T.DesignStep), c_sz, P(T.BandGeom), vp, P(T.AsmConfig), vp, vp], C.c_int),
     "oc_adjoint_grid_gradient_dev_c128": ([vp, P(T.DesignStep), c_sz, P(T.BandGeom), vp, P(T.AsmConfig), vp, vp,
                                            P(c_f64)], C.c_int),
+    "oc_adjoint_grid_gradient_frames_dev_c128": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.DesignStep), c_sz,
+                                                  P(T.AsmConfig), vp, vp, c_i32, c_i32, vp, P(c_f64)], C.c_int),
+    "oc_adjoint_grid_gradient_frames_dev_c64": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.DesignStep), c_sz,
+                                                 P(T.AsmConfig), vp, vp, c_i32, c_i32, vp, P(c_f64)], C.c_int),
     "oc_fd_gradient_dev_c128": ([vp, P(T.BandGeom), P(T.DesignStep), c_sz, c_sz, vp, vp, vp, vp, P(T.AsmConfig), vp,
                                  vp, vp, vp, P(c_sz)], C.c_int),
     "oc_fd_gradient_dev_c64": ([vp, P(T.BandGeom), P(T.DesignStep), c_sz, c_sz, vp, vp, vp, vp, P(T.AsmConfig), vp,
@@ -671,6 +675,31 @@ def adjoint_grid_gradient_dev(dev_steps, u_dev, dx, dy, lam, asm_cfg, target_dev
                                                u_dev.data_ptr(), C.byref(asm_cfg), target_dev.data_ptr(),
                                                grads_dev.data_ptr() if grads_dev is not None else None,
                                                C.byref(merit))
+    return st, merit.value
+
+
+def adjoint_grid_gradient_frames_dev(designs, inputs_dev, dx, dy, bands, asm_cfg, target_dev, grads_dev=None,
+                                     sel_band=-1, sel_scene=-1, ctx=None):
+    """adjoint::grid_gradient (adjoint.rs:268-340) with MseToTarget over every
+    (band, scene) frame.  designs[band] = step list (mask structs with device
+    lattices); inputs_dev complex128/complex64 (bands, scenes, ny, nx);
+    target_dev float64 (ny, nx); grads_dev float64 (n_elements, ny, nx) or
+    None.  Returns (status, merit)."""
+    torch = _torch()
+    ctx = ctx or context()
+    nb, ns, ny, nx = inputs_dev.shape
+    if not inputs_dev.is_contiguous() or target_dev.dtype != torch.float64 or not target_dev.is_contiguous():
+        raise ValueError("inputs_dev and target_dev must be contiguous (target float64)")
+    fn = {torch.complex128: lib.oc_adjoint_grid_gradient_frames_dev_c128,
+          torch.complex64: lib.oc_adjoint_grid_gradient_frames_dev_c64}[inputs_dev.dtype]
+    nsteps = len(designs[0])
+    table = _design_table([[steps] for steps in designs])
+    b = np.ascontiguousarray(bands, np.float64)
+    merit = c_f64()
+    torch.cuda.synchronize()
+    st = fn(ctx.h, C.byref(T.geom(nx, ny, dx, dy, 0.0)), b.ctypes.data, nb, ns, table, nsteps, C.byref(asm_cfg),
+            inputs_dev.data_ptr(), target_dev.data_ptr(), sel_band, sel_scene,
+            grads_dev.data_ptr() if grads_dev is not None else None, C.byref(merit))
     return st, merit.value
 
 

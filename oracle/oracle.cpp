@@ -1403,6 +1403,86 @@ int or_render(const oc_band_geom* g, const double* bands, uint64_t nb, uint64_t 
     return OC_OK;
 }
 
+// adjoint.rs:268-340 with MseToTarget as the merit (merit.rs:112-166):
+// render every (band, scene) frame (observe.rs:195-210, the band's design
+// on its input), seed each frame with obs_gradient (merit.rs:143-165:
+// 2 (I_f - T)/n for a frame the selector picks, 0 otherwise), return no
+// planes when the whole seed is zero (adjoint.rs:297-304), else push every
+// frame back through its band's design and add the planes frame by frame,
+// band-major (adjoint.rs:310-332: the first frame's plane, then += the
+// next).  merit_out = MseToTarget::evaluate (merit.rs:124-137: the
+// selected frames' intensities summed in frame order, then the mean
+// squared difference), NaN for a selector out of range (evaluate's error).
+int or_grid_gradient(const oc_band_geom* g, const double* bands, uint64_t nb, uint64_t ns,
+                     const or_design_step* designs, uint64_t nsteps, const oc_asm_config* asm_cfg,
+                     const double* inputs, const double* target, int32_t sel_band, int32_t sel_scene,
+                     double* dphi_out, uint64_t* n_planes, double* merit_out) {
+    const size_t n = g->nx * g->ny;
+    if (nb == 0 || ns == 0 || n == 0) return OC_INV_RENDER;
+    size_t ne = 0;
+    for (uint64_t s = 0; s < nsteps; ++s) ne += designs[s].has_element ? 1 : 0;
+    const size_t frames = nb * ns;
+    std::vector<double> obs(frames * n);
+    std::vector<double> fld(2 * n);
+    for (uint64_t b = 0; b < nb; ++b) {
+        oc_band_geom gb = *g;
+        gb.lambda = bands[b];
+        for (uint64_t sc = 0; sc < ns; ++sc) {
+            const double* in = inputs + 2 * n * (b * ns + sc);
+            if (or_trace(designs + b * nsteps, nsteps, &gb, in, asm_cfg, fld.data())) return OC_INV_RENDER;
+            double* o = obs.data() + (b * ns + sc) * n;
+            for (size_t k = 0; k < n; ++k) o[k] = fld[2 * k] * fld[2 * k] + fld[2 * k + 1] * fld[2 * k + 1];
+        }
+    }
+    auto picked = [&](uint64_t b, uint64_t sc) {
+        return (sel_band < 0 || (uint64_t)sel_band == b) && (sel_scene < 0 || (uint64_t)sel_scene == sc);
+    };
+    if (merit_out) {
+        const bool in_range = (sel_band < 0 || (uint64_t)sel_band < nb) && (sel_scene < 0 || (uint64_t)sel_scene < ns);
+        if (!in_range) {
+            *merit_out = std::numeric_limits<double>::quiet_NaN();
+        } else {
+            std::vector<double> acc(n, 0.0);
+            for (uint64_t b = 0; b < nb; ++b)
+                for (uint64_t sc = 0; sc < ns; ++sc)
+                    if (picked(b, sc))
+                        for (size_t k = 0; k < n; ++k) acc[k] += obs[(b * ns + sc) * n + k];
+            double m = 0.0;
+            for (size_t k = 0; k < n; ++k) m += (acc[k] - target[k]) * (acc[k] - target[k]);
+            *merit_out = m / (double)n;
+        }
+    }
+    std::vector<double> seed(frames * n, 0.0);
+    bool any = false;
+    for (uint64_t b = 0; b < nb; ++b)
+        for (uint64_t sc = 0; sc < ns; ++sc) {
+            if (!picked(b, sc)) continue;
+            for (size_t k = 0; k < n; ++k) {
+                const double v = 2.0 * (obs[(b * ns + sc) * n + k] - target[k]) / (double)n;
+                seed[(b * ns + sc) * n + k] = v;
+                any = any || v != 0.0;
+            }
+        }
+    *n_planes = 0;
+    if (!any) return OC_OK;
+    std::vector<double> plane(ne * n);
+    for (uint64_t b = 0; b < nb; ++b) {
+        oc_band_geom gb = *g;
+        gb.lambda = bands[b];
+        for (uint64_t sc = 0; sc < ns; ++sc) {
+            const size_t f = b * ns + sc;
+            int st = or_adjoint_push_back(designs + b * nsteps, nsteps, &gb, inputs + 2 * n * f, asm_cfg,
+                                          seed.data() + f * n, plane.data());
+            if (st) return st;
+            if (f == 0) std::memcpy(dphi_out, plane.data(), ne * n * sizeof(double));
+            else
+                for (size_t k = 0; k < ne * n; ++k) dphi_out[k] += plane[k];
+        }
+    }
+    *n_planes = ne;
+    return OC_OK;
+}
+
 // adjoint.rs:140-246
 int or_adjoint_push_back(const or_design_step* steps, uint64_t nsteps, const oc_band_geom* g, const double* input,
                          const oc_asm_config* asm_cfg, const double* dmdi, double* dphi_out) {

@@ -154,6 +154,19 @@ int or_adjoint_push_back(const or_design_step* steps, uint64_t nsteps,
                          const oc_band_geom* g, const double* input,
                          const oc_asm_config* asm_cfg, const double* dmdi,
                          double* dphi_out);
+/* grid_gradient (adjoint.rs:268-340) with MseToTarget (merit.rs:112-166)
+   over every (band, scene) frame: designs [band][nsteps], inputs
+   [band][scene] fields, selector band/scene (-1 = all).  dphi_out:
+   n_elements * n, the per-frame planes added band-major; *n_planes = 0
+   (dphi_out unwritten) when the seed is zero everywhere.  merit_out (may be
+   NULL): MseToTarget::evaluate of the observation, NaN for a selector out of
+   range. */
+int or_grid_gradient(const oc_band_geom* g, const double* bands, uint64_t nb,
+                     uint64_t ns, const or_design_step* designs,
+                     uint64_t nsteps, const oc_asm_config* asm_cfg,
+                     const double* inputs, const double* target,
+                     int32_t sel_band, int32_t sel_scene, double* dphi_out,
+                     uint64_t* n_planes, double* merit_out);
 /* Beamline::trace with ASM (beamline.rs:109-130, observe.rs:195-210):
    output field (complex) of the design. */
 int or_trace(const or_design_step* steps, uint64_t nsteps,

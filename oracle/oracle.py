@@ -81,6 +81,8 @@ def _declare(L):
     L.or_gs_fft.argtypes = [c_u64, vp, vp, vp, c_u64, c_u64, vp]
     L.or_adjoint_push_back.argtypes = [P(DesignStep), c_u64, G, vp, A, vp, vp]
     L.or_trace.argtypes = [P(DesignStep), c_u64, G, vp, A, vp]
+    L.or_grid_gradient.argtypes = [G, vp, c_u64, c_u64, P(DesignStep), c_u64, A, vp, vp, c_i32, c_i32, vp,
+                                   P(c_u64), P(c_f64)]
     L.or_fd_gradient.argtypes = [G, P(DesignStep), c_u64, c_u64, vp, vp, vp, vp, A, vp, vp, vp, vp, vp]
     L.or_adam_step.argtypes = [c_u64, vp, vp, vp, vp, vp, vp, c_u64, P(T.AdamConfig)]
     L.or_mtf.argtypes = [vp, c_u64, c_u64, c_f64, vp, vp]
@@ -427,3 +429,23 @@ def adjoint_push_back(steps, u, dx, dy, lam, asm_cfg, dmdi):
     st = lib().or_adjoint_push_back(_steps(steps), len(steps), C.byref(_geom(u, dx, dy, lam)), _ptr(u),
                                     C.byref(asm_cfg), _ptr(d), _ptr(out))
     return st, out
+
+
+def grid_gradient(designs, inputs, dx, dy, bands, asm_cfg, target, sel_band=-1, sel_scene=-1):
+    """adjoint::grid_gradient (adjoint.rs:268-340) with MseToTarget over every
+    (band, scene) frame.  designs[band] = step list; inputs complex (bands,
+    scenes, ny, nx).  Returns (status, dphi (elements, ny, nx), planes,
+    merit); planes = 0 when the seed is zero everywhere (dphi all zero)."""
+    u = _c128(inputs)
+    nb, ns, ny, nx = u.shape
+    nsteps = len(designs[0])
+    flat = [st for steps in designs for st in steps]
+    ne = sum(1 for st in designs[0] if st[0] is not None)
+    out = np.zeros((max(ne, 1), ny, nx))
+    b = np.ascontiguousarray(bands, np.float64)
+    t = np.ascontiguousarray(target, np.float64)
+    planes, merit = c_u64(0), c_f64(0.0)
+    st = lib().or_grid_gradient(C.byref(T.geom(nx, ny, dx, dy, 0.0)), _ptr(b), nb, ns, _steps(flat), nsteps,
+                                C.byref(asm_cfg), _ptr(u), _ptr(t), sel_band, sel_scene, _ptr(out),
+                                C.byref(planes), C.byref(merit))
+    return st, out[:ne], planes.value, merit.value
