@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--bands", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the f64 and host drop-in legs")
-    ap.add_argument("--only", choices=["gs", "adjoint", "rs", "render", "fd"],
+    ap.add_argument("--only", choices=["gs", "gs_asm", "adjoint", "rs", "render", "fd"],
                     help="run only that secondary workload and print its record (for profiling)")
     ap.add_argument("--autotune", action="store_true",
                     help="time every workgroup layout at P = 4096 first and run the fastest (default: the layouts "
@@ -103,6 +103,53 @@ def bench_gs(G, W, torch, np):
          "final_error_mean": float(np.mean(err)), "final_error_best": float(np.min(err))}
     log(f"gs: {r['iterations_per_s']:.0f} restart-iterations/s, error mean {r['final_error_mean']:.4f} "
         f"best {r['final_error_best']:.4f}")
+    return r
+
+
+def bench_gs_asm(G, W, T, np):
+    """The ASM form of GS (gs::run with the MseToTarget merit, complex128):
+    512^2 at 8 um, 633 nm, z = 20 mm (each half padded 2x to 1024^2), a
+    Gaussian source and a flat-top disk target, 8 restarts x 200 iterations
+    run to their cap.  Timed as one oc_gs_run_batch call (graph chunks of 20)
+    and as 8 oc_gs_run calls; both calls include the host uploads, the
+    illumination propagation and the outputs' read-back."""
+    n, d, lam, z = 512, 8e-6, 633e-9, 20e-3
+    restarts, iters, chunk = 8, 200, 20
+    u = W.gaussian_beam(n, d, lam, 1.2e-3)
+    x = W.coords(n, d)
+    r2 = x[None, :] ** 2 + x[:, None] ** 2
+    disk = r2 <= (1.0e-3) ** 2
+    target = np.where(disk, np.sqrt(np.sum(np.abs(u) ** 2) / disk.sum()), 0.0)
+    cfg = T.AsmConfig(2.0, 0, 0, 8.0, 1.0)
+    probs = []
+    for k in range(restarts):
+        init = -np.pi + 2 * np.pi * W.splitmix_uniform(900 + k, n * n).reshape(n, n)
+        probs.append(dict(u_in=u, dx=d, dy=d, lam=lam, z_total=z, target_amplitude=target,
+                          merit_target=target ** 2, asm_cfg=cfg, max_iterations=iters, patience=5,
+                          merit_tolerance=0.0, init_phase=init, aperture=(T.APERTURE_CIRCLE, 1.8e-3, 0.0)))
+    st, _ = G.gs_run_batch(probs[:2], graph_chunk=chunk)  # warm-up (plans, tables, graph)
+    if st:
+        raise RuntimeError(f"gs asm warm-up failed: {st}")
+    t0 = time.perf_counter()
+    st, sols = G.gs_run_batch(probs, graph_chunk=chunk)
+    t1 = time.perf_counter()
+    if st:
+        raise RuntimeError(f"gs asm batch failed: {st}")
+    single = [k for k in ("u_in", "dx", "dy", "lam", "z_total", "target_amplitude", "merit_target", "asm_cfg")]
+    s0 = time.perf_counter()
+    for p in probs:
+        st, _ = G.gs_run(*[p[k] for k in single], **{k: v for k, v in p.items() if k not in single})
+        if st:
+            raise RuntimeError(f"gs asm single failed: {st}")
+    s1 = time.perf_counter()
+    its = restarts * iters
+    r = {"workload": f"GS ASM form (gs::run) {n}^2 complex128, P = {2 * n}, {restarts} restarts x {iters} "
+                     f"iterations, graph chunk {chunk}",
+         "iterations_per_s": its / (t1 - t0), "seconds": t1 - t0,
+         "single_calls_iterations_per_s": its / (s1 - s0),
+         "merit_best": min(s["merit"] for s in sols), "merit_mean": float(np.mean([s["merit"] for s in sols]))}
+    log(f"gs asm: {r['iterations_per_s']:.0f} restart-iterations/s batched, "
+        f"{r['single_calls_iterations_per_s']:.0f}/s as single calls")
     return r
 
 
@@ -375,6 +422,7 @@ def main():
 
     if args.only:
         runners = {"gs": lambda: bench_gs(G, W, torch, np),
+                   "gs_asm": lambda: bench_gs_asm(G, W, T, np),
                    "adjoint": lambda: bench_adjoint(G, W, T, torch, np, cpu=not args.no_cpu),
                    "rs": lambda: bench_rs(G, W, T, torch, np),
                    "render": lambda: bench_render(G, W, T, torch, np, cpu=not args.no_cpu),
@@ -619,6 +667,7 @@ def main():
             f"{extra['f64_host_dropin_propagations_per_s']:.2f}/s")
         del u_in, u_asm, u_fr
         extra["gs"] = bench_gs(G, W, torch, np)
+        extra["gs_asm"] = bench_gs_asm(G, W, T, np)
         extra["adjoint"] = bench_adjoint(G, W, T, torch, np, cpu=not args.no_cpu)
         extra["rs_batch"] = bench_rs(G, W, T, torch, np)
         extra["render"] = bench_render(G, W, T, torch, np, cpu=not args.no_cpu)

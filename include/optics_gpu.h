@@ -565,6 +565,26 @@ typedef struct oc_gs_solution {
 int oc_gs_run(oc_context* ctx, const oc_gs_problem* problem,
               oc_gs_solution* solution);
 
+/* gs::run for `count` problems (restarts: typically one problem with
+   different initial phases) on one grid size, together.  No single
+   reference counterpart: it replaces `count` calls of gs::run
+   (gs.rs:347-597) and returns the same solutions.  Every pass runs once for
+   all restarts (persistent launches); the merit, best-so-far, history and
+   best-phase copy are kept on the device and the GS sums are reduced in a
+   fixed order, so a restart's history does not depend on the batch or the
+   chunking.  graph_chunk > 1 queues that many iterations as one captured
+   hipGraph and reads the histories back once per chunk; the patience stop is
+   applied per restart and iteration, and a restart that stops inside a chunk
+   is rolled back to the chunk's start and replayed alone up to its stop.
+   graph_chunk <= 1 reads back every iteration.  A restart that has finished
+   keeps running with the others until all have (its outputs were taken at
+   its stop).  Errors: the first failing problem's validation status
+   (nothing runs), OC_ERR_INVALID_ARGUMENT for unequal grid sizes,
+   OC_INV_RENDER for a failing propagation. */
+int oc_gs_run_batch(oc_context* ctx, const oc_gs_problem* problems,
+                    oc_gs_solution* solutions, size_t count,
+                    uint64_t graph_chunk);
+
 /* Gerchberg-Saxton in the Fourier (FFT) form, `restarts` independent runs
    batched in one persistent launch per pass (no reference counterpart: the
    reference loop uses ASM half-transforms).  Projection semantics of

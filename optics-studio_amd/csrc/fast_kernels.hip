@@ -460,7 +460,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
         }
         if (bad) atomicOr(P.flag, 2);
         if (QK == POST_GS_PHASE || (QK < 0 && (P.ops.post == POST_GS_TARGET || P.ops.post == POST_GS_PHASE)))
-            accumulate4(acc0, acc1, acc2, 0.0, P.ops.sums);
+            accumulate4_tile(acc0, acc1, acc2, 0.0, P.ops.sums, P.ops.parts, tile - P.base_row);
     }
 }
 

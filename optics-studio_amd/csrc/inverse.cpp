@@ -222,14 +222,13 @@ int oc_gs_fft_run_dev_c128(oc_context* ctx, uint64_t n, uint64_t restarts, const
     return gs_fft_impl(ctx, true, n, restarts, source_amp, target_amp, phases, iterations, graph_chunk, err_out);
 }
 
-// gs.rs:347-597
-int oc_gs_run(oc_context* ctx, const oc_gs_problem* pb, oc_gs_solution* sol) {
-    OC_CONTEXT_LOCK(ctx);
-    if (!ctx || !pb || !sol) return OC_ERR_INVALID_ARGUMENT;
-    (void)hipSetDevice(ctx->device);
-    evict_tables(ctx);
-    const oc_band_geom g = pb->geom;
-    const size_t nx = g.nx, ny = g.ny, n = nx * ny;
+}  // extern "C"
+
+namespace {
+
+// gs.rs:378-418: the problem checks of gs::run.
+int gs_validate(const oc_gs_problem* pb) {
+    const size_t n = pb->geom.nx * pb->geom.ny;
     if (n == 0 || !pb->input || !pb->target_amplitude || !pb->merit_target) return OC_ERR_INVALID_ARGUMENT;
     for (size_t k = 0; k < n; ++k) {
         const double t = pb->target_amplitude[k];
@@ -243,152 +242,312 @@ int oc_gs_run(oc_context* ctx, const oc_gs_problem* pb, oc_gs_solution* sol) {
     if (pb->init_kind != 0 && pb->init_kind != 2) return OC_ERR_UNSUPPORTED;
     if (pb->init_kind == 0 && !pb->carrier_values) return OC_ERR_INVALID_ARGUMENT;
     if (pb->init_kind == 2 && !pb->init_phase) return OC_ERR_INVALID_ARGUMENT;
-    const double half_z = pb->z_total / 2.0;
-    const double lo = pb->lo, hi = pb->hi;
-    const double beta_f = pb->variant == 2 ? pb->beta : 1.0;
-    const double beta_p = pb->variant == 1 ? pb->beta : 1.0;
-    Scoped mem(ctx);
-    void *d_in, *d_arr, *d_illum, *d_phase, *d_best, *d_open, *d_tamp, *d_mt, *d_proj, *d_err, *d_dphi, *d_sums,
-        *d_flags;
-    int st;
-    if ((st = mem.alloc(n * 16, &d_in)) || (st = mem.alloc(n * 16, &d_arr)) || (st = mem.alloc(n * 8, &d_illum)) ||
-        (st = mem.alloc(n * 8, &d_phase)) || (st = mem.alloc(n * 8, &d_best)) || (st = mem.alloc(n, &d_open)) ||
-        (st = mem.alloc(n * 8, &d_tamp)) || (st = mem.alloc(n * 8, &d_mt)) || (st = mem.alloc(n * 16, &d_proj)) ||
-        (st = mem.alloc(n * 8, &d_err)) || (st = mem.alloc(n * 8, &d_dphi)) || (st = mem.alloc(64, &d_sums)) ||
-        (st = mem.alloc(64, &d_flags)))
-        return st;
-    HIP_TRY(ctx, hipMemcpy(d_in, pb->input, n * 16, hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMemcpy(d_tamp, pb->target_amplitude, n * 8, hipMemcpyHostToDevice));
-    HIP_TRY(ctx, hipMemcpy(d_mt, pb->merit_target, n * 8, hipMemcpyHostToDevice));
-    // aperture indicator (gs.rs:427-433)
-    std::vector<unsigned char> open(n);
-    for (size_t iy = 0; iy < ny; ++iy)
-        for (size_t ix = 0; ix < nx; ++ix) {
-            const double x = ((double)ix - ((double)nx - 1.0) / 2.0) * g.dx;
-            const double y = ((double)iy - ((double)ny - 1.0) / 2.0) * g.dy;
-            open[iy * nx + ix] = contains_h(pb->aperture, x - pb->cx, y - pb->cy) ? 1 : 0;
-        }
-    HIP_TRY(ctx, hipMemcpy(d_open, open.data(), n, hipMemcpyHostToDevice));
-    // illumination arriving at the DOE plane (gs.rs:436-444)
+    return OC_OK;
+}
+
+Prob gs_half(const oc_gs_problem* pb, double z, const void* in, void* out) {
     const oc_asm_config& c = pb->asm_cfg;
-    {
-        std::vector<Prob> v{make_prob(&g, half_z, ENG_ASM, c.band_limited, c.strict, c.padding, c.max_padding,
-                                      c.medium, nullptr, d_in, d_arr)};
-        if (run_batch(ctx, true, v) != OC_OK) return OC_INV_RENDER;
+    return make_prob(&pb->geom, z, ENG_ASM, c.band_limited, c.strict, c.padding, c.max_padding, c.medium, nullptr, in,
+                     out);
+}
+
+// gs::run (gs.rs:347-597) for R problems (restarts) on one grid size, run
+// together: one persistent launch per pass over all restarts, the merit,
+// best-so-far, history and best-phase copy on the device (k_gs_after_fwd,
+// k_gs_keep_best, k_gs_after_bwd), the GS sums reduced in a fixed order.
+// `chunk` > 1 queues that many iterations at once (captured in a hipGraph
+// and replayed) and reads the histories back once per chunk; the patience
+// stop is applied to them on the host, iteration by iteration, and a
+// restart that stopped inside a chunk is rolled back to the chunk's start
+// (phase, best phase and state were saved) and replayed alone up to its
+// stop, so every output is that of the iteration-by-iteration loop.
+int gs_batch_impl(oc_context* ctx, const oc_gs_problem* pbs, oc_gs_solution* sols, size_t R, uint64_t chunk) {
+    if (!ctx || !pbs || !sols || R == 0) return OC_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    evict_tables(ctx);
+    const size_t nx = pbs[0].geom.nx, ny = pbs[0].geom.ny, n = nx * ny;
+    uint64_t max_it = 0;
+    for (size_t r = 0; r < R; ++r) {
+        if (pbs[r].geom.nx != nx || pbs[r].geom.ny != ny)
+            return fail(ctx, OC_ERR_INVALID_ARGUMENT, "the restarts of a batch share one grid size");
+        const int v = gs_validate(&pbs[r]);
+        if (v) return v;
+        max_it = std::max<uint64_t>(max_it, pbs[r].max_iterations);
     }
-    std::vector<double> arr(2 * n), illum(n);
-    HIP_TRY(ctx, hipMemcpy(arr.data(), d_arr, n * 16, hipMemcpyDeviceToHost));
-    for (size_t k = 0; k < n; ++k) illum[k] = std::hypot(arr[2 * k], arr[2 * k + 1]);
-    HIP_TRY(ctx, hipMemcpy(d_illum, illum.data(), n * 8, hipMemcpyHostToDevice));
-    // initial phase (gs.rs:447-475)
+    Scoped mem(ctx);
+    const size_t rn = R * n;
+    void *d_in, *d_arr, *d_illum, *d_phase, *d_best, *d_open, *d_tamp, *d_mt, *d_proj, *d_err, *d_dphi, *d_snap,
+        *d_state, *d_snap_state, *d_hist, *d_flags, *d_parts_f, *d_parts_b, *d_nparts;
+    // partials: at most one row per tile and 16 waves per workgroup
+    const size_t pstride = ny * 16 * 4;
+    int st;
+    if ((st = mem.alloc(rn * 16, &d_in)) || (st = mem.alloc(rn * 16, &d_arr)) || (st = mem.alloc(rn * 8, &d_illum)) ||
+        (st = mem.alloc(rn * 8, &d_phase)) || (st = mem.alloc(rn * 8, &d_best)) || (st = mem.alloc(rn, &d_open)) ||
+        (st = mem.alloc(rn * 8, &d_tamp)) || (st = mem.alloc(rn * 8, &d_mt)) || (st = mem.alloc(rn * 16, &d_proj)) ||
+        (st = mem.alloc(rn * 8, &d_err)) || (st = mem.alloc(rn * 8, &d_dphi)) || (st = mem.alloc(2 * rn * 8, &d_snap)) ||
+        (st = mem.alloc(R * sizeof(GsState), &d_state)) || (st = mem.alloc(R * sizeof(GsState), &d_snap_state)) ||
+        (st = mem.alloc(R * max_it * sizeof(GsRec), &d_hist)) || (st = mem.alloc(2 * R * sizeof(int), &d_flags)) ||
+        (st = mem.alloc(R * pstride * 8, &d_parts_f)) || (st = mem.alloc(R * pstride * 8, &d_parts_b)) ||
+        (st = mem.alloc(2 * R * sizeof(int), &d_nparts)))
+        return st;
+    auto at = [n](void* base, size_t r, size_t esz) { return static_cast<unsigned char*>(base) + r * n * esz; };
+    GsState* state = static_cast<GsState*>(d_state);
+    GsRec* hist = static_cast<GsRec*>(d_hist);
+    int* flags = static_cast<int*>(d_flags);
+    // inputs, targets, aperture indicators (gs.rs:427-433), initial phases (gs.rs:447-475)
+    std::vector<unsigned char> open(n);
     std::vector<double> phase(n);
-    const double* src = pb->init_kind == 0 ? pb->carrier_values : pb->init_phase;
-    for (size_t k = 0; k < n; ++k) phase[k] = encode_bounds_h(src[k], lo, hi);
-    HIP_TRY(ctx, hipMemcpy(d_phase, phase.data(), n * 8, hipMemcpyHostToDevice));
-    // fused forward / backward half-transforms
-    std::vector<Prob> fw{make_prob(&g, half_z, ENG_ASM, c.band_limited, c.strict, c.padding, c.max_padding, c.medium,
-                                   nullptr, nullptr, d_proj)};
-    fw[0].ops.pre = PRE_POLAR;
-    fw[0].ops.amp = d_illum;
-    fw[0].ops.phase = d_phase;
-    fw[0].ops.open = static_cast<const unsigned char*>(d_open);
-    fw[0].ops.post = POST_GS_TARGET;
-    fw[0].ops.tamp = static_cast<const double*>(d_tamp);
-    fw[0].ops.merit_target = static_cast<const double*>(d_mt);
-    fw[0].ops.beta_f = beta_f;
-    fw[0].ops.err_out = static_cast<double*>(d_err);
-    fw[0].ops.sums = static_cast<double*>(d_sums);
-    std::vector<Prob> bw{make_prob(&g, -half_z, ENG_ASM, c.band_limited, c.strict, c.padding, c.max_padding, c.medium,
-                                   nullptr, d_proj, nullptr)};
-    bw[0].ops.post = POST_GS_PHASE;
-    bw[0].ops.phase_io = d_phase;
-    bw[0].ops.beta_p = beta_p;
-    bw[0].ops.lo = lo;
-    bw[0].ops.hi = hi;
-    bw[0].ops.dphi_out = static_cast<double*>(d_dphi);
-    bw[0].ops.sums = static_cast<double*>(d_sums);
-    prepare(fw[0]);
-    prepare(bw[0]);
-    if (fw[0].status || bw[0].status || fw[0].check_only || bw[0].check_only) return OC_INV_RENDER;
-    Batch bf, bb;
-    st = build_batch(ctx, true, fw, static_cast<int*>(d_flags), bf, true);
-    if (!st) st = build_batch(ctx, true, bw, static_cast<int*>(d_flags) + 1, bb, true);
-    if (st || bf.pd.empty() || bb.pd.empty()) {
-        free_batch(ctx, bf);
-        free_batch(ctx, bb);
-        return st ? st : OC_INV_RENDER;
+    for (size_t r = 0; r < R; ++r) {
+        const oc_gs_problem* pb = &pbs[r];
+        const oc_band_geom& g = pb->geom;
+        HIP_TRY(ctx, hipMemcpy(at(d_in, r, 16), pb->input, n * 16, hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(at(d_tamp, r, 8), pb->target_amplitude, n * 8, hipMemcpyHostToDevice));
+        HIP_TRY(ctx, hipMemcpy(at(d_mt, r, 8), pb->merit_target, n * 8, hipMemcpyHostToDevice));
+        for (size_t iy = 0; iy < ny; ++iy)
+            for (size_t ix = 0; ix < nx; ++ix) {
+                const double x = ((double)ix - ((double)nx - 1.0) / 2.0) * g.dx;
+                const double y = ((double)iy - ((double)ny - 1.0) / 2.0) * g.dy;
+                open[iy * nx + ix] = contains_h(pb->aperture, x - pb->cx, y - pb->cy) ? 1 : 0;
+            }
+        HIP_TRY(ctx, hipMemcpy(at(d_open, r, 1), open.data(), n, hipMemcpyHostToDevice));
+        const double* src = pb->init_kind == 0 ? pb->carrier_values : pb->init_phase;
+        for (size_t k = 0; k < n; ++k) phase[k] = encode_bounds_h(src[k], pb->lo, pb->hi);
+        HIP_TRY(ctx, hipMemcpy(at(d_phase, r, 8), phase.data(), n * 8, hipMemcpyHostToDevice));
     }
-    bool have_best = false;
-    double best_merit = 0.0;
-    uint64_t best_it = 0;
-    std::vector<oc_gs_record> hist;
-    bool converged = false;
-    double hsum[4];
-    int hfl[2];
-    for (uint64_t it = 0; it < pb->max_iterations; ++it) {
-        HIP_TRY(ctx, hipMemsetAsync(d_sums, 0, 64, ctx->stream));
-        HIP_TRY(ctx, ocl::launch_fill_int(static_cast<int*>(d_flags), 0, 2, ctx->stream));
-        if ((st = launch_batch(ctx, bf))) break;
-        HIP_TRY(ctx, hipMemcpyAsync(hsum, d_sums, 32, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(ctx, hipMemcpyAsync(hfl, d_flags, 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        if (hfl[0]) { st = OC_INV_RENDER; break; }
-        const double amp_mse = hsum[0] / (double)n;
-        const double merit = hsum[1] / (double)n;
-        const bool improved = !have_best || merit < best_merit;
-        if (improved) {
-            have_best = true;
-            best_merit = merit;
-            best_it = it;
-            HIP_TRY(ctx, hipMemcpyAsync(d_best, d_phase, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    // illumination arriving at the DOE plane (gs.rs:436-444), every restart at once
+    std::vector<std::vector<double>> illum(R, std::vector<double>(n));
+    {
+        std::vector<Prob> v;
+        for (size_t r = 0; r < R; ++r) v.push_back(gs_half(&pbs[r], pbs[r].z_total / 2.0, at(d_in, r, 16), at(d_arr, r, 16)));
+        if (run_batch(ctx, true, v) != OC_OK) return OC_INV_RENDER;
+        std::vector<double> arr(2 * n);
+        for (size_t r = 0; r < R; ++r) {
+            HIP_TRY(ctx, hipMemcpy(arr.data(), at(d_arr, r, 16), n * 16, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < n; ++k) illum[r][k] = std::hypot(arr[2 * k], arr[2 * k + 1]);
+            HIP_TRY(ctx, hipMemcpy(at(d_illum, r, 8), illum[r].data(), n * 8, hipMemcpyHostToDevice));
         }
-        hist.push_back({it, merit, best_merit, amp_mse, std::sqrt(amp_mse), 0.0});
-        if ((st = launch_batch(ctx, bb))) break;
-        HIP_TRY(ctx, hipMemcpyAsync(hsum, d_sums, 32, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(ctx, hipMemcpyAsync(hfl, d_flags, 8, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    // the two half-transforms of every restart, projections fused
+    std::vector<Prob> fw, bw;
+    for (size_t r = 0; r < R; ++r) {
+        const oc_gs_problem* pb = &pbs[r];
+        Prob f = gs_half(pb, pb->z_total / 2.0, nullptr, at(d_proj, r, 16));
+        f.ops.pre = PRE_POLAR;
+        f.ops.amp = at(d_illum, r, 8);
+        f.ops.phase = at(d_phase, r, 8);
+        f.ops.open = at(d_open, r, 1);
+        f.ops.post = POST_GS_TARGET;
+        f.ops.tamp = reinterpret_cast<const double*>(at(d_tamp, r, 8));
+        f.ops.merit_target = reinterpret_cast<const double*>(at(d_mt, r, 8));
+        f.ops.beta_f = pb->variant == 2 ? pb->beta : 1.0;
+        f.ops.err_out = reinterpret_cast<double*>(at(d_err, r, 8));
+        f.ops.parts = static_cast<double*>(d_parts_f) + r * pstride;
+        Prob b = gs_half(pb, -pb->z_total / 2.0, at(d_proj, r, 16), nullptr);
+        b.ops.post = POST_GS_PHASE;
+        b.ops.phase_io = at(d_phase, r, 8);
+        b.ops.beta_p = pb->variant == 1 ? pb->beta : 1.0;
+        b.ops.lo = pb->lo;
+        b.ops.hi = pb->hi;
+        b.ops.dphi_out = reinterpret_cast<double*>(at(d_dphi, r, 8));
+        b.ops.parts = static_cast<double*>(d_parts_b) + r * pstride;
+        prepare(f);
+        prepare(b);
+        if (f.status || b.status || f.check_only || b.check_only || !fits_on_chip(f, true)) return OC_INV_RENDER;
+        fw.push_back(f);
+        bw.push_back(b);
+    }
+    Batch bf, bb;
+    st = build_batch(ctx, true, fw, flags, bf, true);
+    if (!st) st = build_batch(ctx, true, bw, flags + R, bb, true);
+    struct Cleanup {
+        oc_context* c;
+        Batch *a, *b;
+        ~Cleanup() {
+            free_batch(c, *a);
+            free_batch(c, *b);
+        }
+    } cleanup{ctx, &bf, &bb};
+    if (st || bf.pd.size() != R || bb.pd.size() != R) return st ? st : OC_INV_RENDER;
+    // fixed-order partial counts: row tiles x waves of the last pass
+    std::vector<int> np(2 * R);
+    for (size_t r = 0; r < R; ++r) {
+        np[r] = bf.pd[r].tiles_row * ((bf.pc[2].block + 63) / 64);
+        np[R + r] = bb.pd[r].tiles_row * ((bb.pc[2].block + 63) / 64);
+        if ((size_t)np[r] * 4 > pstride || (size_t)np[R + r] * 4 > pstride) return fail(ctx, OC_ERR_DEVICE, "GS partials");
+    }
+    HIP_TRY(ctx, hipMemcpy(d_nparts, np.data(), np.size() * sizeof(int), hipMemcpyHostToDevice));
+    const int* np_f = static_cast<const int*>(d_nparts);
+    const int* np_b = np_f + R;
+    HIP_TRY(ctx, hipMemsetAsync(d_state, 0, R * sizeof(GsState), ctx->stream));
+    HIP_TRY(ctx, ocl::launch_fill_int(flags, 0, (int)(2 * R), ctx->stream));
+    double* phase_d = static_cast<double*>(d_phase);
+    double* best_d = static_cast<double*>(d_best);
+    // one iteration of the restarts of batches (fwd, bwd) whose state,
+    // partial counts and history start at restart r0 (`cnt` restarts)
+    auto iterate = [&](Batch& f, Batch& b, size_t r0, int cnt) -> int {
+        int s2 = launch_batch(ctx, f);
+        if (s2) return s2;
+        HIP_TRY(ctx, ocl::launch_gs_after_fwd(state + r0, static_cast<double*>(d_parts_f) + r0 * pstride, np_f + r0,
+                                              pstride, n, hist + r0 * max_it, max_it, cnt, ctx->stream));
+        HIP_TRY(ctx, ocl::launch_gs_keep_best(state + r0, phase_d + r0 * n, best_d + r0 * n, n, cnt, ctx->stream));
+        if ((s2 = launch_batch(ctx, b))) return s2;
+        HIP_TRY(ctx, ocl::launch_gs_after_bwd(state + r0, static_cast<double*>(d_parts_b) + r0 * pstride, np_b + r0,
+                                              pstride, n, hist + r0 * max_it, max_it, cnt, ctx->stream));
+        return OC_OK;
+    };
+    hipGraphExec_t exec = nullptr;
+    struct GraphGuard {
+        hipGraphExec_t* e;
+        ~GraphGuard() {
+            if (*e) (void)hipGraphExecDestroy(*e);
+        }
+    } graph_guard{&exec};
+    if (chunk > 1 && max_it >= chunk) {
+        const bool was = ctx->timing;
+        ctx->timing = false;
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+        if (e != hipSuccess) {
+            ctx->timing = was;
+            return fail(ctx, OC_ERR_DEVICE, "begin capture");
+        }
+        for (uint64_t k = 0; k < chunk && st == OC_OK; ++k) st = iterate(bf, bb, 0, (int)R);
+        e = hipStreamEndCapture(ctx->stream, &graph);
+        ctx->timing = was;
+        if (st || e != hipSuccess) return st ? st : fail(ctx, OC_ERR_DEVICE, "end capture");
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(ctx, OC_ERR_DEVICE, "graph instantiate");
+    }
+    std::vector<char> done(R, 0), conv(R, 0);
+    std::vector<uint64_t> stop_at(R, 0);
+    std::vector<GsRec> h(R * max_it);
+    std::vector<GsState> hs(R);
+    size_t live = R;
+    // copies a finished restart's outputs (its buffers hold the state of its
+    // last iteration)
+    auto finalize = [&](size_t r, uint64_t last) -> int {
+        const oc_gs_problem* pb = &pbs[r];
+        oc_gs_solution* sol = &sols[r];
+        HIP_TRY(ctx, hipMemcpyAsync(h.data() + r * max_it, hist + r * max_it, (last + 1) * sizeof(GsRec),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(&hs[r], state + r, sizeof(GsState), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(sol->phase, at(d_best, r, 8), n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        if (sol->target_amplitude_error)
+            HIP_TRY(ctx, hipMemcpyAsync(sol->target_amplitude_error, at(d_err, r, 8), n * 8, hipMemcpyDeviceToHost,
+                                        ctx->stream));
+        if (sol->phase_error)
+            HIP_TRY(ctx, hipMemcpyAsync(sol->phase_error, at(d_dphi, r, 8), n * 8, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-        if (hfl[1]) { st = OC_INV_RENDER; break; }
-        hist.back().phase_update_rms = std::sqrt(hsum[2]) / std::sqrt((double)n);
-        if (it + 1 > pb->patience) {
-            const double earlier = hist[hist.size() - 1 - pb->patience].best_merit;
-            const double scale = std::fmax(std::fabs(earlier), 1e-30);
-            if ((earlier - best_merit) / scale < pb->merit_tolerance) {
-                converged = true;
-                break;
+        const GsRec* hr = h.data() + r * max_it;
+        if (sol->history) std::memcpy(sol->history, hr, (last + 1) * sizeof(oc_gs_record));
+        if (sol->illumination_amplitude) std::memcpy(sol->illumination_amplitude, illum[r].data(), n * 8);
+        sol->merit = hs[r].best_merit;
+        sol->best_iteration = hs[r].best_it;
+        sol->amplitude_mse = hr[hs[r].best_it].amplitude_mse;
+        sol->converged = conv[r];
+        sol->iterations = last + 1;
+        if (sol->target_intensity) {  // |U_t|^2 of the best phase (the render of its iteration)
+            std::vector<Prob> v{gs_half(pb, pb->z_total / 2.0, nullptr, at(d_arr, r, 16))};
+            v[0].ops.pre = PRE_POLAR;
+            v[0].ops.amp = at(d_illum, r, 8);
+            v[0].ops.phase = at(d_best, r, 8);
+            v[0].ops.open = at(d_open, r, 1);
+            if (run_batch(ctx, true, v) != OC_OK) return OC_INV_RENDER;
+            std::vector<double> arr(2 * n);
+            HIP_TRY(ctx, hipMemcpy(arr.data(), at(d_arr, r, 16), n * 16, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < n; ++k) {
+                const double a = std::hypot(arr[2 * k], arr[2 * k + 1]);
+                sol->target_intensity[k] = a * a;
             }
         }
-    }
-    collect_timing(ctx);
-    free_batch(ctx, bf);
-    free_batch(ctx, bb);
-    if (st) return st;
-    // outputs
-    HIP_TRY(ctx, hipMemcpy(sol->phase, d_best, n * 8, hipMemcpyDeviceToHost));
-    if (sol->target_amplitude_error)
-        HIP_TRY(ctx, hipMemcpy(sol->target_amplitude_error, d_err, n * 8, hipMemcpyDeviceToHost));
-    if (sol->phase_error) HIP_TRY(ctx, hipMemcpy(sol->phase_error, d_dphi, n * 8, hipMemcpyDeviceToHost));
-    if (sol->illumination_amplitude) std::memcpy(sol->illumination_amplitude, illum.data(), n * 8);
-    if (sol->history) std::memcpy(sol->history, hist.data(), hist.size() * sizeof(oc_gs_record));
-    sol->merit = best_merit;
-    sol->best_iteration = best_it;
-    sol->amplitude_mse = hist[best_it].amplitude_mse;
-    sol->converged = converged ? 1 : 0;
-    sol->iterations = hist.size();
-    if (sol->target_intensity) {  // |U_t|^2 of the best phase (same render as its iteration)
-        std::vector<Prob> v{make_prob(&g, half_z, ENG_ASM, c.band_limited, c.strict, c.padding, c.max_padding,
-                                      c.medium, nullptr, nullptr, d_proj)};
-        v[0].ops.pre = PRE_POLAR;
-        v[0].ops.amp = d_illum;
-        v[0].ops.phase = d_best;
-        v[0].ops.open = static_cast<const unsigned char*>(d_open);
-        if (run_batch(ctx, true, v) != OC_OK) return OC_INV_RENDER;
-        HIP_TRY(ctx, hipMemcpy(arr.data(), d_proj, n * 16, hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < n; ++k) {
-            const double a = std::hypot(arr[2 * k], arr[2 * k + 1]);
-            sol->target_intensity[k] = a * a;
+        done[r] = 1;
+        --live;
+        return OC_OK;
+    };
+    uint64_t it0 = 0;
+    while (live) {
+        const uint64_t c = (exec && it0 + chunk <= max_it) ? chunk : 1;
+        if (c > 1) {  // the chunk's starting point, for restarts that stop inside it
+            HIP_TRY(ctx, hipMemcpyAsync(d_snap, d_phase, rn * 8, hipMemcpyDeviceToDevice, ctx->stream));
+            HIP_TRY(ctx, hipMemcpyAsync(static_cast<unsigned char*>(d_snap) + rn * 8, d_best, rn * 8,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+            HIP_TRY(ctx, hipMemcpyAsync(d_snap_state, d_state, R * sizeof(GsState), hipMemcpyDeviceToDevice,
+                                        ctx->stream));
+            const hipError_t e = timed(ctx, 4, [&] { return hipGraphLaunch(exec, ctx->stream); });
+            if (e != hipSuccess) return fail(ctx, OC_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
+        } else if ((st = iterate(bf, bb, 0, (int)R))) {
+            return st;
         }
+        std::vector<int> hflags(2 * R);
+        for (size_t r = 0; r < R; ++r)
+            if (!done[r])
+                HIP_TRY(ctx, hipMemcpyAsync(h.data() + r * max_it + it0, hist + r * max_it + it0, c * sizeof(GsRec),
+                                            hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipMemcpyAsync(hflags.data(), flags, 2 * R * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        collect_timing(ctx);
+        // a non-finite sample in a propagation (gs.rs:488, 554); restarts that
+        // already stopped keep running in the batched launches, their flags
+        // are not theirs to raise any more
+        for (size_t r = 0; r < R; ++r)
+            if (!done[r] && (hflags[r] || hflags[R + r])) return OC_INV_RENDER;
+        for (size_t r = 0; r < R; ++r) {
+            if (done[r]) continue;
+            const oc_gs_problem* pb = &pbs[r];
+            const GsRec* hr = h.data() + r * max_it;
+            // the loop's exit test, iteration by iteration (gs.rs:482, 570-577)
+            uint64_t last = 0;
+            bool stop = false;
+            for (uint64_t it = it0; it < it0 + c && !stop; ++it) {
+                if (it + 1 > pb->patience) {
+                    const double earlier = hr[it - pb->patience].best_merit;
+                    const double scale = std::fmax(std::fabs(earlier), 1e-30);
+                    if ((earlier - hr[it].best_merit) / scale < pb->merit_tolerance) {
+                        conv[r] = 1;
+                        stop = true;
+                    }
+                }
+                if (it + 1 == pb->max_iterations) stop = true;
+                last = it;
+            }
+            if (!stop) continue;
+            if (last + 1 < it0 + c) {
+                // roll this restart back to the chunk's start and run it alone to its stop
+                HIP_TRY(ctx, hipMemcpyAsync(at(d_phase, r, 8), static_cast<unsigned char*>(d_snap) + r * n * 8, n * 8,
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+                HIP_TRY(ctx, hipMemcpyAsync(at(d_best, r, 8), static_cast<unsigned char*>(d_snap) + (rn + r * n) * 8,
+                                            n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+                HIP_TRY(ctx, hipMemcpyAsync(state + r, static_cast<GsState*>(d_snap_state) + r, sizeof(GsState),
+                                            hipMemcpyDeviceToDevice, ctx->stream));
+                std::vector<int> one{(int)r};
+                Batch sf, sb;
+                st = build_batch(ctx, true, fw, flags, sf, false, &one, "gsf");
+                if (!st) st = build_batch(ctx, true, bw, flags + R, sb, false, &one, "gsb");
+                for (uint64_t it = it0; !st && it <= last; ++it) st = iterate(sf, sb, r, 1);
+                if (st) return st;
+            }
+            if ((st = finalize(r, last))) return st;
+        }
+        it0 += c;
     }
     return OC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// gs.rs:347-597
+int oc_gs_run(oc_context* ctx, const oc_gs_problem* pb, oc_gs_solution* sol) {
+    OC_CONTEXT_LOCK(ctx);
+    return gs_batch_impl(ctx, pb, sol, 1, 0);
+}
+
+int oc_gs_run_batch(oc_context* ctx, const oc_gs_problem* problems, oc_gs_solution* solutions, size_t count,
+                    uint64_t graph_chunk) {
+    OC_CONTEXT_LOCK(ctx);
+    return gs_batch_impl(ctx, problems, solutions, count, graph_chunk);
 }
 
 int oc_adjoint_asm(oc_context* ctx, const oc_band_geom* g, const double* in, double z, const oc_asm_config* cfg,

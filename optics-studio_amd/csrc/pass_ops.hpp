@@ -397,6 +397,28 @@ __device__ __forceinline__ void accumulate4(double a0, double a1, double a2, dou
     }
 }
 
+// accumulate4 into a problem's fixed-order partials when it has them:
+// wave w of the problem's row tile `slot` writes its four sums to
+// parts[(slot * waves + w) * 4 + i].  Every thread of the block must call it.
+__device__ __forceinline__ void accumulate4_tile(double a0, double a1, double a2, double a3, double* sums,
+                                                 double* parts, int slot) {
+    if (!parts) {
+        accumulate4(a0, a1, a2, a3, sums);
+        return;
+    }
+    double v[4] = {a0, a1, a2, a3};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_down(v[i], off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const int waves = ((int)blockDim.x + 63) / 64, w = (int)threadIdx.x / 64;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) parts[((size_t)slot * waves + w) * 4 + i] = v[i];
+    }
+}
+
 // First-pass load of input sample (ix, iy) with the problem's pre-op.
 // PK: the pre-op when known at compile time (PRE_NONE / PRE_MASK /
 // PRE_TMUL), else -1 and the descriptor's op decides.

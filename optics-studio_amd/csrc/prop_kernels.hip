@@ -210,7 +210,7 @@ __global__ void k_row_inv(const ProbDev* __restrict__ probs, int nprob, int tota
         }
         if (bad) atomicOr(P.flag, 2);
         if (P.ops.post == POST_GS_TARGET || P.ops.post == POST_GS_PHASE)
-            accumulate4(acc0, acc1, acc2, 0.0, P.ops.sums);
+            accumulate4_tile(acc0, acc1, acc2, 0.0, P.ops.sums, P.ops.parts, tile - P.base_row);
         __syncthreads();
     }
 }
@@ -568,6 +568,88 @@ __global__ void k_frame_sum(const double* __restrict__ slab, double* __restrict_
     }
 }
 
+// The device-side bookkeeping of gs::run (gs.rs:482-578) for restart
+// r = blockIdx.x, between the passes of an iteration, so that iterations can
+// be queued (and graph-captured) without reading anything back.  sum_gs_parts
+// adds a restart's fixed-order partials (accumulate4_tile) in index order.
+__device__ void sum_gs_parts(const double* __restrict__ parts, int nparts, double* out) {
+    __shared__ double sm[4][256];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < nparts; j += 256) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] += parts[(size_t)j * 4 + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sm[i][threadIdx.x] = a[i];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sm[i][threadIdx.x] += sm[i][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[i] = sm[i][0];
+    }
+}
+
+// After the forward half (gs.rs:494-532): amplitude MSE and merit, the
+// best-so-far, the history record; st.improved tells k_gs_keep_best to
+// keep this iteration's phase.
+__global__ __launch_bounds__(256) void k_gs_after_fwd(GsState* __restrict__ st, const double* __restrict__ parts,
+                                                      const int* __restrict__ nparts, size_t pstride, double nf,
+                                                      GsRec* __restrict__ hist, size_t hstride) {
+    const int r = blockIdx.x;
+    __shared__ double s[4];
+    sum_gs_parts(parts + (size_t)r * pstride, nparts[r], s);
+    if (threadIdx.x == 0) {
+        GsState& S = st[r];
+        const unsigned long long it = S.it;
+        const double amp_mse = s[0] / nf;
+        const double merit = s[1] / nf;
+        const int improved = !S.have_best || merit < S.best_merit;
+        if (improved) {
+            S.have_best = 1;
+            S.best_merit = merit;
+            S.best_it = it;
+        }
+        S.improved = improved;
+        GsRec h;
+        h.iteration = it;
+        h.merit = merit;
+        h.best_merit = S.best_merit;
+        h.amplitude_mse = amp_mse;
+        h.target_error_rms = sqrt(amp_mse);
+        h.phase_update_rms = 0.0;
+        hist[(size_t)r * hstride + it] = h;
+    }
+}
+
+// best = phase where the iteration improved (the phase of its forward half).
+__global__ void k_gs_keep_best(const GsState* __restrict__ st, const double* __restrict__ phase,
+                               double* __restrict__ best, size_t n, int restarts) {
+    const size_t total = n * (size_t)restarts;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+        if (st[i / n].improved) best[i] = phase[i];
+}
+
+// After the backward half (gs.rs:557-567): the phase-update RMS of the
+// record, then the next iteration.
+__global__ __launch_bounds__(256) void k_gs_after_bwd(GsState* __restrict__ st, const double* __restrict__ parts,
+                                                      const int* __restrict__ nparts, size_t pstride, double nf,
+                                                      GsRec* __restrict__ hist, size_t hstride) {
+    const int r = blockIdx.x;
+    __shared__ double s[4];
+    sum_gs_parts(parts + (size_t)r * pstride, nparts[r], s);
+    if (threadIdx.x == 0) {
+        GsState& S = st[r];
+        hist[(size_t)r * hstride + S.it].phase_update_rms = sqrt(s[2]) / sqrt(nf);
+        S.it += 1;
+    }
+}
+
 // sums[0] = the np workgroup partials added in a fixed order (thread t adds
 // t, t + 256, ... in order, then an LDS tree).
 __global__ __launch_bounds__(256) void k_sum_parts(const double* __restrict__ part, int np,
@@ -764,6 +846,25 @@ hipError_t launch_seed_frames(bool dbl, const void* u, const double* target, voi
         hipLaunchKernelGGL(k_seed_frames<float>, dim3(grid), dim3(256), 0, st, static_cast<const cpx<float>*>(u),
                            target, static_cast<cpx<float>*>(out), n, nb, ns, sel_b, sel_s, (double)n, sums + 8);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, sums + 8, grid, sums);
+    return hipGetLastError();
+}
+
+hipError_t launch_gs_after_fwd(GsState* st, const double* parts, const int* nparts, size_t pstride, size_t n,
+                               GsRec* hist, size_t hstride, int restarts, hipStream_t stream) {
+    hipLaunchKernelGGL(k_gs_after_fwd, dim3(restarts), dim3(256), 0, stream, st, parts, nparts, pstride, (double)n,
+                       hist, hstride);
+    return hipGetLastError();
+}
+hipError_t launch_gs_after_bwd(GsState* st, const double* parts, const int* nparts, size_t pstride, size_t n,
+                               GsRec* hist, size_t hstride, int restarts, hipStream_t stream) {
+    hipLaunchKernelGGL(k_gs_after_bwd, dim3(restarts), dim3(256), 0, stream, st, parts, nparts, pstride, (double)n,
+                       hist, hstride);
+    return hipGetLastError();
+}
+hipError_t launch_gs_keep_best(const GsState* st, const double* phase, double* best, size_t n, int restarts,
+                               hipStream_t stream) {
+    const int grid = (int)std::max<size_t>(1, std::min<size_t>((n * restarts + 255) / 256, 16384));
+    hipLaunchKernelGGL(k_gs_keep_best, dim3(grid), dim3(256), 0, stream, st, phase, best, n, restarts);
     return hipGetLastError();
 }
 

@@ -131,6 +131,10 @@ struct OpsDev {
     // like the intermediate ([P_x/B][P_y][B], element type R)
     const void* tamp_blk;
     double* sums;  // accumulators (4 per problem)
+    // when set, the last pass's GS sums go here in a fixed order instead of
+    // into `sums`: wave w of row tile t (counted from the problem's first)
+    // writes its four sums to parts[(t * waves + w) * 4 + i]
+    double* parts;
 };
 
 struct ProbDev {
@@ -155,6 +159,21 @@ struct ProbDev {
     EngineDev eng;
     OpsDev ops;
     int* flag;  // bit 0: non-finite input sample; bit 1: non-finite output
+};
+
+// Per-restart state of the device-side gs::run loop (gs.rs:482-578): the
+// next iteration, the best-so-far and whether the last iteration improved.
+struct GsState {
+    unsigned long long it;
+    unsigned long long best_it;
+    double best_merit;
+    int have_best;
+    int improved;
+};
+// GsRecord (gs.rs:254-268), the layout of oc_gs_record.
+struct GsRec {
+    unsigned long long iteration;
+    double merit, best_merit, amplitude_mse, target_error_rms, phase_update_rms;
 };
 
 // One batch of 1-D transforms over a buffer (the multi-pass path for padded

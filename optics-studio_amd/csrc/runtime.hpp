@@ -80,6 +80,14 @@ hipError_t launch_seed_frames(bool dbl, const void* u, const double* target, voi
                               int sel_b, int sel_s, double* sums, hipStream_t st);
 // out[i] = sum over frames of slab[f][i], in frame order (m per frame).
 hipError_t launch_frame_sum(const double* slab, double* out, size_t m, int frames, hipStream_t st);
+// gs::run bookkeeping between the passes of an iteration, one workgroup
+// per restart (parts: pstride doubles per restart, nparts[r] entries of 4).
+hipError_t launch_gs_after_fwd(ocd::GsState* st, const double* parts, const int* nparts, size_t pstride, size_t n,
+                               ocd::GsRec* hist, size_t hstride, int restarts, hipStream_t stream);
+hipError_t launch_gs_after_bwd(ocd::GsState* st, const double* parts, const int* nparts, size_t pstride, size_t n,
+                               ocd::GsRec* hist, size_t hstride, int restarts, hipStream_t stream);
+hipError_t launch_gs_keep_best(const ocd::GsState* st, const double* phase, double* best, size_t n, int restarts,
+                               hipStream_t stream);
 hipError_t big_set_lds_limits();
 hipError_t big_line(bool dbl, int dir, const void* in, void* out, const ocd::LineSpec& s, const ocd::FftPlanDev& pl,
                     const void* twl, size_t lds, int max_grid, hipStream_t st);

@@ -90,6 +90,7 @@ EXPORTS = {
     "oc_phasemask_apply_dev_c128": ([vp, P(T.BandGeom), vp, P(T.PhaseMask), vp], C.c_int),
     "oc_phasemask_apply_dev_c64": ([vp, P(T.BandGeom), vp, P(T.PhaseMask), vp], C.c_int),
     "oc_gs_run": ([vp, P(T.GsProblem), P(T.GsSolution)], C.c_int),
+    "oc_gs_run_batch": ([vp, P(T.GsProblem), P(T.GsSolution), c_sz, c_u64], C.c_int),
     "oc_gs_fft_run_dev_c64": ([vp, c_u64, c_u64, vp, vp, vp, c_u64, c_u64, P(c_f64)], C.c_int),
     "oc_gs_fft_run_dev_c128": ([vp, c_u64, c_u64, vp, vp, vp, c_u64, c_u64, P(c_f64)], C.c_int),
     "oc_adjoint_asm": ([vp, P(T.BandGeom), vp, c_f64, P(T.AsmConfig), vp, P(T.AsmReport)], C.c_int),
@@ -590,6 +591,52 @@ def gs_run(u_in, dx, dy, lam, z_total, target_amplitude, merit_target, asm_cfg, 
     return st, dict(phase=phase, merit=sol.merit, best_iteration=sol.best_iteration,
                     amplitude_mse=sol.amplitude_mse, converged=bool(sol.converged), iterations=k, history=history,
                     target_intensity=ti, target_amplitude_error=tae, phase_error=pe, illumination_amplitude=il)
+
+
+def gs_run_batch(problems, graph_chunk=0, ctx=None):
+    """oc_gs_run_batch: problems = list of dicts with gs_run's arguments
+    (u_in, dx, dy, lam, z_total, target_amplitude, merit_target, asm_cfg and
+    the keywords); returns (status, [solution dict per problem])."""
+    ctx = ctx or context()
+    keep, probs, sols, outs = [], (T.GsProblem * len(problems))(), (T.GsSolution * len(problems))(), []
+    for i, p in enumerate(problems):
+        u_in = _c128(p["u_in"])
+        ny, nx = u_in.shape
+        kw = dict(aperture=(0, 0.0, 0.0), center=(0.0, 0.0), lo=-np.pi, hi=np.pi, carrier_values=None,
+                  max_iterations=100, patience=10, merit_tolerance=1e-4, variant=0, beta=1.0, init_phase=None)
+        kw.update({k: v for k, v in p.items() if k in kw})
+        ta = np.ascontiguousarray(p["target_amplitude"], np.float64)
+        mt = np.ascontiguousarray(p["merit_target"], np.float64)
+        cv = np.ascontiguousarray(kw["carrier_values"] if kw["carrier_values"] is not None else np.zeros((ny, nx)),
+                                  np.float64)
+        ip = np.ascontiguousarray(kw["init_phase"] if kw["init_phase"] is not None else np.zeros((ny, nx)),
+                                  np.float64)
+        ap = kw["aperture"]
+        probs[i] = T.GsProblem(T.geom(nx, ny, p["dx"], p["dy"], p["lam"]), u_in.ctypes.data, p["z_total"],
+                               ta.ctypes.data, mt.ctypes.data, p["asm_cfg"], T.ClearAperture(ap[0], 0, ap[1], ap[2]),
+                               kw["center"][0], kw["center"][1], kw["lo"], kw["hi"], cv.ctypes.data,
+                               kw["max_iterations"], kw["patience"], kw["merit_tolerance"], kw["variant"],
+                               2 if kw["init_phase"] is not None else 0, kw["beta"], ip.ctypes.data)
+        arrays = [np.empty((ny, nx)) for _ in range(5)]
+        hist = (T.GsRecord * kw["max_iterations"])()
+        sols[i] = T.GsSolution(arrays[0].ctypes.data, 0.0, 0, 0.0, 0, 0, 0, C.cast(hist, vp), arrays[1].ctypes.data,
+                               arrays[2].ctypes.data, arrays[3].ctypes.data, arrays[4].ctypes.data)
+        keep += [u_in, ta, mt, cv, ip]
+        outs.append((arrays, hist))
+    st = lib.oc_gs_run_batch(ctx.h, probs, sols, len(problems), graph_chunk)
+    if st != 0:
+        return st, None
+    res = []
+    for i, (arrays, hist) in enumerate(outs):
+        sol = sols[i]
+        k = sol.iterations
+        history = [(h.iteration, h.merit, h.best_merit, h.amplitude_mse, h.target_error_rms, h.phase_update_rms)
+                   for h in hist[:k]]
+        res.append(dict(phase=arrays[0], merit=sol.merit, best_iteration=sol.best_iteration,
+                        amplitude_mse=sol.amplitude_mse, converged=bool(sol.converged), iterations=k,
+                        history=history, target_intensity=arrays[1], target_amplitude_error=arrays[2],
+                        phase_error=arrays[3], illumination_amplitude=arrays[4]))
+    return st, res
 
 
 def gs_fft_run_dev(source_amp, target_amp, phases, iterations, graph_chunk=0, ctx=None):
