@@ -51,6 +51,9 @@ def parse():
                          "measured fastest on MI355X, so that every run, profiled or not, runs the same kernels)")
     ap.add_argument("--layout", type=str, default=None,
                     help="V or V,C,R: run layout variant V (and the separable Fresnel's column/row layouts C,R)")
+    ap.add_argument("--schedule-cache", type=str, default=None,
+                    help="the RS batch loads its schedules from this file when it exists, else tunes and saves "
+                         "them there (default: tune, keep nothing)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of 1 GiB (8 launches) for PMC counter calibration")
     return ap.parse_args()
@@ -245,7 +248,7 @@ def bench_adjoint(G, W, T, torch, np, cpu=True):
     return r
 
 
-def bench_rs(G, W, T, torch, np):
+def bench_rs(G, W, T, torch, np, schedule_cache=None):
     """Workload 5: 64 uneven Rayleigh-Sommerfeld problems (sizes 256..2048,
     padding 2) in one batched call, complex64."""
     ps = W.workload5_params()
@@ -260,21 +263,43 @@ def bench_rs(G, W, T, torch, np):
         arr[i].strict = 0
         arr[i].in_ = ins[i].data_ptr()
         arr[i].out = outs[i].data_ptr()
-    st = G.propagate_batch_dev(arr)
-    if st:
-        raise RuntimeError(f"rs batch failed: {st}")
-    k = 5
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(k):
-        G.propagate_batch_dev(arr)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    ctx = G.Context(0)  # a context of its own: the tuner's picks stay in it
+
+    def timed(k=5):
+        st = G.propagate_batch_dev(arr, ctx=ctx)
+        if st:
+            raise RuntimeError(f"rs batch failed: {st}")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            G.propagate_batch_dev(arr, ctx=ctx)
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / k
+
+    ms_default = timed()
+    # the batch schedule tuner on this batch: pad length, layout or radix plan per length
+    cache = schedule_cache
+    loaded = 0
+    if cache and os.path.exists(cache):
+        st, loaded = G.schedule_load(cache, ctx=ctx)
+        if st:
+            raise RuntimeError(f"schedule load failed: {st}")
+        picks, search_ms = G.schedules(ctx=ctx), 0.0
+    else:
+        st, picks, search_ms = G.autotune_batch_dev(arr, sweeps=1, ctx=ctx)
+        if st:
+            raise RuntimeError(f"rs tuning failed: {st}")
+        if cache:
+            G.schedule_save(cache, ctx=ctx)
+    ms = timed()
+    ctx.close()
     px = sum(p["n"] ** 2 for p in ps)
-    r = {"workload": "64 uneven RS problems (256..2048, padding 2), complex64, one call",
-         "problems_per_s": len(ps) * k / (t1 - t0), "ms_per_batch": 1e3 * (t1 - t0) / k,
+    r = {"workload": "64 uneven RS problems (256..2048, padding 2), complex64, one call, tuned schedules",
+         "problems_per_s": len(ps) / (ms * 1e-3), "ms_per_batch": ms, "ms_per_batch_default_schedule": ms_default,
+         "tuner": {"search_ms": search_ms, "loaded_from_cache": loaded, "cache": cache, "picks": picks},
          "field_samples": px, "sizes": sorted({p["n"] for p in ps})}
-    log(f"rs batch: {r['ms_per_batch']:.2f} ms per 64 problems")
+    log(f"rs batch: {ms:.2f} ms per 64 problems tuned ({ms_default:.2f} default), "
+        f"search {search_ms:.0f} ms, {len(picks)} picks")
     return r
 
 
@@ -424,7 +449,7 @@ def main():
         runners = {"gs": lambda: bench_gs(G, W, torch, np),
                    "gs_asm": lambda: bench_gs_asm(G, W, T, np),
                    "adjoint": lambda: bench_adjoint(G, W, T, torch, np, cpu=not args.no_cpu),
-                   "rs": lambda: bench_rs(G, W, T, torch, np),
+                   "rs": lambda: bench_rs(G, W, T, torch, np, args.schedule_cache),
                    "render": lambda: bench_render(G, W, T, torch, np, cpu=not args.no_cpu),
                    "fd": lambda: bench_fd(G, W, T, torch, np)}
         print(json.dumps({"only": args.only, "record": runners[args.only]()}))
@@ -669,7 +694,7 @@ def main():
         extra["gs"] = bench_gs(G, W, torch, np)
         extra["gs_asm"] = bench_gs_asm(G, W, T, np)
         extra["adjoint"] = bench_adjoint(G, W, T, torch, np, cpu=not args.no_cpu)
-        extra["rs_batch"] = bench_rs(G, W, T, torch, np)
+        extra["rs_batch"] = bench_rs(G, W, T, torch, np, args.schedule_cache)
         extra["render"] = bench_render(G, W, T, torch, np, cpu=not args.no_cpu)
         extra["fd"] = bench_fd(G, W, T, torch, np)
 

@@ -279,6 +279,71 @@ int oc_context_fresnel_sep_layout(oc_context* ctx, uint64_t n, int* v_col,
                                   int* v_row);
 int oc_context_force_generic(oc_context* ctx, int enable);
 
+/* Batch schedule tuner (no reference counterpart; the reference runs
+   rustfft's plan for every length).  A propagation batch
+   (oc_propagate_*_dev_* on square padded grids) makes two kinds of choice,
+   each kept per (padded length, precision, batch size class) where the
+   class is the number of problems of that length in the call rounded up to
+   a power of two:
+     OC_SCHED_PAD   a convolution-engine length (Rayleigh-Sommerfeld,
+                    Fresnel convolution) runs at `run_length`: its own
+                    length, or the next power of two when that is at most
+                    8192 (with P >= 2N - 1 on both axes every cropped
+                    output sample is the same linear convolution for any
+                    such P, so the reports keep the reference's P);
+     OC_SCHED_FAST  a power-of-two run length uses the register-resident
+                    kernels in layout `variant`;
+     OC_SCHED_LDS   a run length uses the LDS kernels with the radix plan
+                    radix[0..nradix) (stages in order; radices 2, 3, 4, 5,
+                    7, 8, 16, product = the length).
+   oc_autotune_batch_dev_* times the caller's batch (it runs it several
+   times: outputs and statuses are those of the last run) under every
+   candidate of every choice, one choice at a time with the others fixed
+   (coordinate descent, `sweeps` rounds, 1 if 0), keeping the fastest; the
+   picks stay in the context and, up to max_picks, are copied to picks
+   (npicks receives their count); search_ms receives the wall time spent.
+   Lengths the search did not see use the defaults (power-of-two lengths:
+   next power of two for convolutions, the layout of oc_context_fft_variant;
+   others: the radix plan 16.., 8/4/2, 3, 5, 7).  A pick found for another
+   size class applies when its own class has none (the nearest class).
+   oc_schedule_save writes the context's picks as text tagged with the
+   device (name and compute units); oc_schedule_load reads such a file and
+   installs the entries tagged with this context's device (loaded receives
+   their count), OC_ERR_INVALID_ARGUMENT for a malformed file.
+   oc_schedule_set installs one entry (ms ignored), oc_schedule_get copies
+   entry i (OC_ERR_INVALID_ARGUMENT past the end), oc_schedule_clear drops
+   all.  The tuner's picks affect performance only: every schedule computes
+   the same propagation to rounding. */
+#define OC_SCHED_PAD 0
+#define OC_SCHED_FAST 1
+#define OC_SCHED_LDS 2
+typedef struct oc_schedule {
+    uint64_t length;          /* padded length (square grids) */
+    int32_t double_precision; /* 0: complex64, 1: complex128 */
+    int32_t kind;             /* OC_SCHED_* */
+    uint64_t problems;        /* batch size class (a power of two) */
+    uint64_t run_length;      /* OC_SCHED_PAD */
+    int32_t variant;          /* OC_SCHED_FAST */
+    int32_t nradix;           /* OC_SCHED_LDS */
+    int32_t radix[12];
+    double ms;                /* batch time with this pick when tuned */
+} oc_schedule;
+struct oc_prop_desc; /* the batch descriptor, defined with the batch calls */
+int oc_autotune_batch_dev_c64(oc_context* ctx, struct oc_prop_desc* descs,
+                              size_t count, int sweeps, oc_schedule* picks,
+                              size_t max_picks, size_t* npicks,
+                              double* search_ms);
+int oc_autotune_batch_dev_c128(oc_context* ctx, struct oc_prop_desc* descs,
+                               size_t count, int sweeps, oc_schedule* picks,
+                               size_t max_picks, size_t* npicks,
+                               double* search_ms);
+int oc_schedule_set(oc_context* ctx, const oc_schedule* s);
+int oc_schedule_get(oc_context* ctx, size_t i, oc_schedule* out);
+size_t oc_schedule_count(oc_context* ctx);
+int oc_schedule_clear(oc_context* ctx);
+int oc_schedule_save(oc_context* ctx, const char* path);
+int oc_schedule_load(oc_context* ctx, const char* path, size_t* loaded);
+
 /* ------------------------------------------------------------------------ */
 /* Angular spectrum (src/propagation/asm.rs).                                */
 /* ------------------------------------------------------------------------ */

@@ -207,10 +207,27 @@ bool make_radices(int n, std::vector<int>& rad, bool& generic) {
     return true;
 }
 
-int make_plan(oc_context* ctx, int n, bool dbl, FftPlanDev* pl) {
+bool valid_radices(int n, const int* rad, int nrad) {
+    if (n <= 0 || nrad <= 0 || nrad > kMaxStages) return false;
+    long long prod = 1;
+    for (int i = 0; i < nrad; ++i) {
+        const int r = rad[i];
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 16) return false;
+        prod *= r;
+        if (prod > n) return false;
+    }
+    return prod == n;
+}
+
+int make_plan(oc_context* ctx, int n, bool dbl, FftPlanDev* pl, const int* radices, int nrad) {
     std::vector<int> rad;
-    bool generic;
-    if (!make_radices(n, rad, generic)) return fail(ctx, OC_ERR_UNSUPPORTED, "too many FFT stages");
+    bool generic = false;
+    if (radices && nrad > 0) {
+        if (!valid_radices(n, radices, nrad)) return fail(ctx, OC_ERR_INVALID_ARGUMENT, "radix plan does not fit n");
+        rad.assign(radices, radices + nrad);
+    } else if (!make_radices(n, rad, generic)) {
+        return fail(ctx, OC_ERR_UNSUPPORTED, "too many FFT stages");
+    }
     std::memset(pl, 0, sizeof *pl);
     pl->n = n;
     pl->nstages = (int)rad.size();
@@ -515,6 +532,8 @@ void prepare(Prob& p) {
         p.nxp = r.nx_pad;
         p.nyp = r.ny_pad;
     }
+    p.ref_nxp = p.nxp;
+    p.ref_nyp = p.nyp;
     if (conv_engine(p.engine) && p.nxp + 1 >= 2 * g.nx && p.nyp + 1 >= 2 * g.ny) {
         // Sampled-kernel convolution with P >= 2N - 1 on both axes: every
         // cropped output sample is sum_m u[m] h((n - m) d) over offsets
@@ -614,8 +633,11 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         bool ok = n0 >= 32 && n0 <= 8192 && (n0 & (n0 - 1)) == 0;
         const bool rs0 = conv_engine(probs[cand[0]].engine);
         for (int i : cand)
-            ok = ok && probs[i].nxp == n0 && probs[i].nyp == n0 && conv_engine(probs[i].engine) == rs0;
-        const int v = force_v >= 0 ? force_v : ctx->variant_for((int)n0, dbl);
+            ok = ok && probs[i].nxp == n0 && probs[i].nyp == n0 && conv_engine(probs[i].engine) == rs0 &&
+                 probs[i].path != 0;
+        // a scheduled layout (every problem of a batch length carries the same pick)
+        const int sv0 = probs[cand[0]].path == 1 ? probs[cand[0]].variant : -1;
+        const int v = force_v >= 0 ? force_v : (sv0 >= 0 ? sv0 : ctx->variant_for((int)n0, dbl));
         if (ok && ocl::fast_supported(dbl, (int)n0, v, &fG, &fB, &fC)) {
             b.fast = true;
             b.fast_n = (int)n0;
@@ -631,7 +653,8 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     // x embedding offset must fall on a block boundary)
     if (b.fast && !dbl && !b.rs) {
         const std::pair<int, int> sv =
-            force_v >= 0 ? std::make_pair(force_v, force_v) : ctx->sep_variants_for(b.fast_n);
+            force_v >= 0 ? std::make_pair(force_v, force_v)
+                         : (probs[cand[0]].path == 1 ? std::make_pair(b.fast_v, b.fast_v) : ctx->sep_variants_for(b.fast_n));
         int cG, cB, cC, rG, rB, rC;
         bool sep = ocl::fast_supported(false, b.fast_n, sv.first, &cG, &cB, &cC) &&
                    ocl::fast_supported(false, b.fast_n, sv.second, &rG, &rB, &rC) && cB == rB;
@@ -683,9 +706,11 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
         d.py = (int)p.nyp;
         d.ox = (int)((p.nxp - p.g.nx) / 2);
         d.oy = (int)((p.nyp - p.g.ny) / 2);
-        int st = make_plan(ctx, d.px, dbl, &d.prow);
+        // a scheduled radix plan is for the square run length
+        const bool sr = !b.fast && p.path == 0 && p.nrad > 0 && p.nxp == p.nyp;
+        int st = make_plan(ctx, d.px, dbl, &d.prow, sr ? p.rad : nullptr, sr ? p.nrad : 0);
         if (st) return st;
-        st = make_plan(ctx, d.py, dbl, &d.pcol);
+        st = make_plan(ctx, d.py, dbl, &d.pcol, sr ? p.rad : nullptr, sr ? p.nrad : 0);
         if (st) return st;
         const bool rs = conv_engine(p.engine);
         Layout L;
@@ -1019,6 +1044,7 @@ int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* df
             if (st) p.status = st;
         }
     }
+    apply_schedules(ctx, dbl, probs);
     const int np = (int)probs.size();
     if (zero_flags) HIP_TRY(ctx, ocl::launch_fill_int(dflags, 0, np, ctx->stream));
     for (int i = 0; i < np; ++i) {
@@ -1049,7 +1075,8 @@ int launch_problems(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* df
     for (int i = 0; i < np; ++i) {
         const Prob& p = probs[i];
         if (p.status != OC_OK || !p.launch) continue;
-        const bool fast = !ctx->force_generic && p.nxp == p.nyp && ocl::fast_variants((int)p.nxp) > 0;
+        const bool fast =
+            !ctx->force_generic && p.nxp == p.nyp && ocl::fast_variants((int)p.nxp) > 0 && p.path != 0;
         if (!fast && p.engine != ENG_FRAUNHOFER && !fits_on_chip(p, dbl)) {
             big.push_back(i);
             continue;

@@ -168,6 +168,8 @@ struct oc_context {
     std::vector<std::vector<unsigned char>> arena_keep;
     std::map<std::string, const void*> arena_tables;
     std::map<std::pair<int, int>, int> fft_variant;  // (n, double?) -> fast-kernel layout variant
+    // batch schedules by (path pick?, length, double?, size class)
+    std::map<std::tuple<int, uint64_t, int, uint64_t>, oc_schedule> sched;
     std::map<int, std::pair<int, int>> sep_variant;  // n -> (column, row) layouts of the two-pass Fresnel
     // The autotuned layout if there is one, else the layout that timed
     // fastest for a masked-ASM + Fresnel round on MI355X (oc_autotune_fft
@@ -252,7 +254,17 @@ int noalloc_attach(oc_context* ctx);
 int ensure_pinned(oc_context* ctx, int slot, size_t bytes);
 int get_twiddles(oc_context* ctx, int n, bool dbl, const void** out);
 bool make_radices(int n, std::vector<int>& rad, bool& generic);
-int make_plan(oc_context* ctx, int n, bool dbl, FftPlanDev* pl);
+// Plan of length n: the default radices, or rad[0..nrad) when given.
+int make_plan(oc_context* ctx, int n, bool dbl, FftPlanDev* pl, const int* rad = nullptr, int nrad = 0);
+// A radix plan the in-place LDS stages run: radices 2, 3, 4, 5, 7, 8, 16
+// with product n, at most kMaxStages of them.
+bool valid_radices(int n, const int* rad, int nrad);
+// Batch schedules (oc_schedule_*): size class of a count, the context's
+// pick for (pad?, length, precision, class) or the nearest class's, and
+// their application to a call's prepared problems.
+uint64_t size_class(size_t n);
+const oc_schedule* find_sched(const oc_context* ctx, bool pad, uint64_t len, bool dbl, uint64_t cls);
+void apply_schedules(oc_context* ctx, bool dbl, std::vector<struct Prob>& probs);
 size_t plan_lds_elems(const FftPlanDev& pl);
 // Internal description of one propagation problem.
 struct Prob {
@@ -269,6 +281,13 @@ struct Prob {
     bool check_only = false; // finiteness check first (z == 0 copy / guard failure)
     int pending = OC_OK;     // status to report if the input is finite
     size_t nxp = 0, nyp = 0;
+    size_t ref_nxp = 0, ref_nyp = 0;  // the reference's padded size (nxp/nyp may run longer)
+    // schedule of the run length: -1 default, 0 LDS kernels with rad[0..nrad)
+    // (nrad 0: the default plan), 1 register-resident kernels in `variant`
+    int path = -1;
+    int variant = -1;
+    int nrad = 0;
+    int rad[kMaxStages] = {};
     oc_asm_report asm_rep{};
     oc_fresnel_report fr_rep{};
     oc_rs_report rs_rep{};

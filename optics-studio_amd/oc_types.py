@@ -156,6 +156,43 @@ class PropDesc(C.Structure):
                 ("medium", c_f64), ("mask", C.POINTER(PhaseMask)), ("in_", C.c_void_p), ("out", C.c_void_p)]
 
 
+SCHED_PAD, SCHED_FAST, SCHED_LDS = 0, 1, 2
+SCHED_KINDS = {SCHED_PAD: "pad", SCHED_FAST: "fast", SCHED_LDS: "lds"}
+
+
+class Schedule(C.Structure):
+    """oc_schedule: one pick of the batch schedule tuner (no reference counterpart)."""
+    _fields_ = [("length", c_u64), ("double_precision", c_i32), ("kind", c_i32), ("problems", c_u64),
+                ("run_length", c_u64), ("variant", c_i32), ("nradix", c_i32), ("radix", c_i32 * 12), ("ms", c_f64)]
+
+
+def schedule_dict(s):
+    d = dict(length=s.length, double=bool(s.double_precision), kind=SCHED_KINDS[s.kind], problems=s.problems,
+             ms=s.ms)
+    if s.kind == SCHED_PAD:
+        d["run_length"] = s.run_length
+    elif s.kind == SCHED_FAST:
+        d["variant"] = s.variant
+    else:
+        d["radix"] = list(s.radix[: s.nradix])
+    return d
+
+
+def schedule_from(d):
+    s = Schedule()
+    s.length = d["length"]
+    s.double_precision = 1 if d.get("double") else 0
+    s.kind = {v: k for k, v in SCHED_KINDS.items()}[d["kind"]]
+    s.problems = d.get("problems", 1)
+    s.run_length = d.get("run_length", 0)
+    s.variant = d.get("variant", 0)
+    rad = d.get("radix", [])
+    s.nradix = len(rad)
+    for i, r in enumerate(rad):
+        s.radix[i] = r
+    return s
+
+
 class GsProblem(C.Structure):
     """GsProblem + GsConfig (crates/optics-inverse/src/gs.rs:125-305), GridPhase carrier, MseToTarget merit."""
     _fields_ = [("geom", BandGeom), ("input", C.c_void_p), ("z_total", c_f64), ("target_amplitude", C.c_void_p),

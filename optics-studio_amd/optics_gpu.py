@@ -113,6 +113,15 @@ EXPORTS = {
     "oc_propagate_batch_submit_dev_c64": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_batch_submit_dev_c128": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_context_wait": ([vp], C.c_int),
+    "oc_autotune_batch_dev_c64": ([vp, P(T.PropDesc), c_sz, c_i32, P(T.Schedule), c_sz, P(c_sz), P(c_f64)], C.c_int),
+    "oc_autotune_batch_dev_c128": ([vp, P(T.PropDesc), c_sz, c_i32, P(T.Schedule), c_sz, P(c_sz), P(c_f64)],
+                                   C.c_int),
+    "oc_schedule_set": ([vp, P(T.Schedule)], C.c_int),
+    "oc_schedule_get": ([vp, c_sz, P(T.Schedule)], C.c_int),
+    "oc_schedule_count": ([vp], c_sz),
+    "oc_schedule_clear": ([vp], C.c_int),
+    "oc_schedule_save": ([vp, C.c_char_p], C.c_int),
+    "oc_schedule_load": ([vp, C.c_char_p, P(c_sz)], C.c_int),
     "oc_mtf": ([vp, vp, c_sz, c_sz, c_sz, c_f64, vp, vp], C.c_int),
     "oc_mtf_dev": ([vp, vp, c_sz, c_sz, c_sz, c_f64, vp, vp], C.c_int),
     "oc_render": ([vp, P(T.BandGeom), vp, c_sz, c_sz, P(T.RenderConfig), P(T.DesignStep), c_sz, P(T.AsmConfig), vp,
@@ -470,6 +479,53 @@ def propagate_batch_dev(descs, double=False, ctx=None):
     _torch().cuda.synchronize()
     fn = lib.oc_propagate_batch_dev_c128 if double else lib.oc_propagate_batch_dev_c64
     return fn(ctx.h, descs, len(descs))
+
+
+def autotune_batch_dev(descs, double=False, sweeps=1, ctx=None):
+    """oc_autotune_batch_dev_*: tunes the schedules of this batch (it runs
+    the batch several times).  Returns (status, [pick dicts], search_ms)."""
+    ctx = ctx or context()
+    _torch().cuda.synchronize()
+    fn = lib.oc_autotune_batch_dev_c128 if double else lib.oc_autotune_batch_dev_c64
+    picks = (T.Schedule * 64)()
+    n = c_sz(0)
+    ms = c_f64(0.0)
+    st = fn(ctx.h, descs, len(descs), sweeps, picks, 64, C.byref(n), C.byref(ms))
+    return st, [T.schedule_dict(picks[i]) for i in range(n.value)], ms.value
+
+
+def schedules(ctx=None):
+    """The context's schedule picks as dicts."""
+    ctx = ctx or context()
+    out = []
+    for i in range(lib.oc_schedule_count(ctx.h)):
+        s = T.Schedule()
+        if lib.oc_schedule_get(ctx.h, i, C.byref(s)) == 0:
+            out.append(T.schedule_dict(s))
+    return out
+
+
+def schedule_set(d, ctx=None):
+    ctx = ctx or context()
+    return lib.oc_schedule_set(ctx.h, C.byref(T.schedule_from(d)))
+
+
+def schedule_clear(ctx=None):
+    ctx = ctx or context()
+    return lib.oc_schedule_clear(ctx.h)
+
+
+def schedule_save(path, ctx=None):
+    ctx = ctx or context()
+    return lib.oc_schedule_save(ctx.h, os.fsencode(path))
+
+
+def schedule_load(path, ctx=None):
+    """Returns (status, entries installed)."""
+    ctx = ctx or context()
+    n = c_sz(0)
+    st = lib.oc_schedule_load(ctx.h, os.fsencode(path), C.byref(n))
+    return st, n.value
 
 
 def propagate_batch_noalloc_dev(descs, ctx=None):
