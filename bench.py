@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--bands", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the f64 and host drop-in legs")
-    ap.add_argument("--only", choices=["gs", "gs_asm", "adjoint", "rs", "render", "fd"],
+    ap.add_argument("--only", choices=["gs", "gs_asm", "frames", "adjoint", "rs", "render", "fd"],
                     help="run only that secondary workload and print its record (for profiling)")
     ap.add_argument("--autotune", action="store_true",
                     help="time every workgroup layout at P = 4096 first and run the fastest (default: the layouts "
@@ -159,9 +159,9 @@ def bench_gs_asm(G, W, T, np):
 def bench_adjoint(G, W, T, torch, np, cpu=True):
     """Workload 4: adjoint gradient of the intensity MSE w.r.t. both DOEs of
     DOE1 -> ASM 10 mm -> DOE2 -> ASM 10 mm at 1024^2 (f64, per-plane
-    checkpoints), against finite differences (central at 1e-3 and 2e-3 rad,
-    Richardson-extrapolated) on 256 seeded pixels and, with the CPU leg,
-    against the f64 oracle adjoint."""
+    checkpoints), against central finite differences at h = 1e-4 rad on 256
+    seeded pixels (also central at 1e-3 and 2e-3 rad, Richardson-
+    extrapolated) and, with the CPU leg, against the f64 oracle adjoint."""
     c = W.CFG4
     n, d, lam, z = c["n"], c["d"], c["lam"], c["z"]
     u, p1, p2, tgt = W.workload4()
@@ -184,19 +184,20 @@ def bench_adjoint(G, W, T, torch, np, cpu=True):
     t1 = time.perf_counter()
     g = grads.cpu().numpy()
     # finite differences on 2 x 128 seeded pixels through the batched FD
-    # gradient (fixed-order merit reductions), central differences at h and
-    # 2h combined by Richardson extrapolation: (4 g_h - g_2h) / 3 cancels the
-    # h^2 truncation term, so the FD side is accurate to O(h^4)
-    h = 10 * c["fd_step"]
+    # gradient (fixed-order merit reductions): central at h0 = 1e-4 rad, and
+    # central at h = 10 h0 and 2h combined by Richardson extrapolation
+    # ((4 g_h - g_2h) / 3 cancels the h^2 truncation term)
+    h0 = c["fd_step"]
+    h = 10 * h0
     px1, px2 = W.workload4_fd_pixels()
-    fd, ad, fd_h = [], [], []
+    fd, ad, fd_h, fd_0 = [], [], [], []
     f0 = time.perf_counter()
     for plane, pix in ((0, px1), (1, px2)):
         base = torch.as_tensor(p1 if plane == 0 else p2).cuda()
         for c0_ in range(0, len(pix), 32):
             chunk = pix[c0_:c0_ + 32]
             gs_ = []
-            for hh in (h, 2 * h):
+            for hh in (h0, h, 2 * h):
                 keep, designs, up, dn = [], [], [], []
                 for kk in chunk:
                     iy, ix = divmod(int(kk), n)
@@ -217,20 +218,24 @@ def bench_adjoint(G, W, T, torch, np, cpu=True):
                     raise RuntimeError(f"fd check failed: {st}")
                 gs_.append(gg)
                 del keep
-            fd.extend((4.0 * gs_[0] - gs_[1]) / 3.0)
-            fd_h.extend(gs_[0])
+            fd_0.extend(gs_[0])
+            fd.extend((4.0 * gs_[1] - gs_[2]) / 3.0)
+            fd_h.extend(gs_[1])
             for kk in chunk:
                 iy, ix = divmod(int(kk), n)
                 ad.append(g[plane, iy, ix])
     f1 = time.perf_counter()
-    fd, ad = np.array(fd), np.array(ad)
+    fd, ad, fd_0 = np.array(fd), np.array(ad), np.array(fd_0)
     r = {"workload": f"adjoint grid gradient {n}^2 f64, 2 DOEs + 2 ASM {z * 1e3:.0f} mm, MSE to a 16-spot target",
          "gradients_per_s": k / (t1 - t0), "ms_per_gradient": 1e3 * (t1 - t0) / k, "merit": merit,
-         "fd_rel_l2": float(np.linalg.norm(fd - ad) / np.linalg.norm(fd)),
-         "fd_max_abs_over_max": float(np.max(np.abs(fd - ad)) / np.max(np.abs(fd))),
-         "fd_max_rel": float(np.max(np.abs(fd - ad) / np.maximum(np.abs(fd), 1e-30))),
-         "fd_central_only_rel_l2": float(np.linalg.norm(np.array(fd_h) - ad) / np.linalg.norm(fd)),
-         "fd_step_rad": h, "fd_pixels": int(fd.size), "fd_seconds": f1 - f0}
+         "fd_step_rad": h0, "fd_pixels": int(fd.size),
+         "fd_rel_l2": float(np.linalg.norm(fd_0 - ad) / np.linalg.norm(fd_0)),
+         "fd_max_abs_over_max": float(np.max(np.abs(fd_0 - ad)) / np.max(np.abs(fd_0))),
+         "fd_richardson_1e-3_2e-3": {
+             "rel_l2": float(np.linalg.norm(fd - ad) / np.linalg.norm(fd)),
+             "max_rel": float(np.max(np.abs(fd - ad) / np.maximum(np.abs(fd), 1e-30))),
+             "central_1e-3_rel_l2": float(np.linalg.norm(np.array(fd_h) - ad) / np.linalg.norm(fd))},
+         "fd_seconds": f1 - f0}
     if cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import oracle
@@ -246,6 +251,66 @@ def bench_adjoint(G, W, T, torch, np, cpu=True):
     log(f"adjoint: {r['gradients_per_s']:.1f} gradients/s, FD rel L2 {r['fd_rel_l2']:.2e}"
         + (f", vs oracle {r['vs_f64_oracle_rel_l2']}" if cpu else ""))
     return r
+
+
+def bench_frames(G, W, T, torch, np):
+    """The multi-frame adjoint gradient (adjoint::grid_gradient with
+    MseToTarget summed over every (band, scene) frame): workload 4's design
+    (DOE1 -> ASM 10 mm -> DOE2 -> ASM 10 mm at 1024^2, P = 2048) under 3
+    bands x 2 scenes (Gaussian beams, centred and shifted), complex128 and
+    complex64 fields.  One gradient = 6 forward traces + 6 reverse sweeps,
+    24 propagations; roofline of the column pass from the library's own
+    per-pass timing (2 N P B algorithmic bytes per propagation)."""
+    c = W.CFG4
+    n, d, z = c["n"], c["d"], c["z"]
+    u, p1, p2, tgt = W.workload4()
+    bands = [1.50e-6, 1.55e-6, 1.60e-6]
+    scenes = [(0.0, 0.0), (40 * d, -25 * d)]
+    cfg = T.AsmConfig.default()
+    m1, k1 = G.dev_mask(p1, d, d)
+    m2, k2 = G.dev_mask(p2, d, d)
+    designs = [[(m1, z, 1.0), (m2, z, 1.0)] for _ in bands]
+    inp = np.zeros((len(bands), len(scenes), n, n), complex)
+    for b, lam in enumerate(bands):
+        for s_, cc in enumerate(scenes):
+            inp[b, s_] = W.gaussian_beam(n, d, lam, 0.3 * n * d, center=cc)
+    td = torch.as_tensor(tgt).cuda()
+    grads = torch.zeros((2, n, n), dtype=torch.float64, device="cuda")
+    ctx = G.context()
+    P, frames = 2 * n, len(bands) * len(scenes)
+    props = 4 * frames
+    out = {"workload": f"adjoint grid gradient over {len(bands)} bands x {len(scenes)} scenes, {n}^2, 2 DOEs + "
+                       f"2 ASM {z * 1e3:.0f} mm (P = {P}), MSE to a 16-spot target, per-element sums over frames",
+           "propagations_per_gradient": props}
+    for name, dt, B in (("c128", torch.complex128, 16), ("c64", torch.complex64, 8)):
+        ud = torch.as_tensor(inp).to(dt).cuda().contiguous()
+        st, merit = G.adjoint_grid_gradient_frames_dev(designs, ud, d, d, bands, cfg, td, grads, ctx=ctx)
+        if st:
+            raise RuntimeError(f"frames gradient failed: {st}")
+        k = 5
+        torch.cuda.synchronize()
+        ctx.pass_timing(reset=True)
+        ctx.set_timing_passes(["col"])
+        ctx.enable_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            G.adjoint_grid_gradient_frames_dev(designs, ud, d, d, bands, cfg, td, grads, ctx=ctx)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.enable_timing(False)
+        ctx.set_timing_passes(None)
+        col_ms, col_launches = ctx.pass_timing(reset=True)["col"]
+        alg = k * props * 2 * n * P * B  # column-pass bytes over the timed calls
+        achieved = alg / (col_ms * 1e-3) / 1e9 if col_ms > 0 else None
+        out[name] = {"gradients_per_s": k / (t1 - t0), "ms_per_gradient": 1e3 * (t1 - t0) / k, "merit": merit,
+                     "roofline": {"bound": "hbm", "kernel": "col", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                                  "traffic": None, "col_ms_per_gradient": col_ms / k,
+                                  "col_launches_per_gradient": col_launches / k}}
+        log(f"frames gradient {name}: {out[name]['gradients_per_s']:.1f} gradients/s, column pass "
+            f"{col_ms / k:.3f} ms per gradient at {achieved:.0f} GB/s")
+        del ud
+    return out
 
 
 def bench_rs(G, W, T, torch, np, schedule_cache=None):
@@ -415,11 +480,27 @@ def bench_fd(G, W, T, torch, np):
     return r
 
 
+def asm_only_rate(G, torch, arr, nb, k, dbl, ctx):
+    """Masked ASM propagations/s of one batch call over the bands."""
+    st = G.propagate_batch_dev(arr, double=dbl, ctx=ctx)
+    if st:
+        raise RuntimeError(f"ASM batch failed: {st}")
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    for _ in range(k):
+        G.propagate_batch_dev(arr, double=dbl, ctx=ctx)
+    torch.cuda.synchronize()
+    return nb * k / (time.perf_counter() - s0)
+
+
+TRAFFIC_FILE = "profiles/traffic_col_c64_workload2.txt"
+
+
 def read_traffic(variant):
     """HBM bytes per launch of the column pass for the layout variant in use,
     from the committed rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE,
     copy-kernel calibrated), if any."""
-    p = os.path.join(REPO, "profiles", "traffic_col_c64_workload2.txt")
+    p = os.path.join(REPO, TRAFFIC_FILE)
     if not os.path.exists(p):
         return None
     key = f"traffic_bytes_per_launch_v{variant}="
@@ -448,6 +529,7 @@ def main():
     if args.only:
         runners = {"gs": lambda: bench_gs(G, W, torch, np),
                    "gs_asm": lambda: bench_gs_asm(G, W, T, np),
+                   "frames": lambda: bench_frames(G, W, T, torch, np),
                    "adjoint": lambda: bench_adjoint(G, W, T, torch, np, cpu=not args.no_cpu),
                    "rs": lambda: bench_rs(G, W, T, torch, np, args.schedule_cache),
                    "render": lambda: bench_render(G, W, T, torch, np, cpu=not args.no_cpu),
@@ -606,6 +688,8 @@ def main():
             extra[name + "_propagations_per_s"] = 2 * nb * k / (time.perf_counter() - s0)
         log(f"chain with the ASM field written: {extra['chain_with_intermediate_propagations_per_s']:.0f}/s; "
             f"two batch calls: {extra['two_batch_calls_propagations_per_s']:.0f}/s")
+        # ASM alone: the 16 masked ASM propagations of a step as one batch call
+        extra["asm_only_c64_propagations_per_s"] = asm_only_rate(G, torch, d_asm, nb, k, False, ctx)
 
     # parity of this run's last band output against the f64 oracle (and the
     # oracle timed on the host as the CPU baseline)
@@ -679,6 +763,13 @@ def main():
         s1 = time.perf_counter()
         # the same chained call in complex128 (its three-pass steps linked through their rows)
         extra["c128_device_propagations_per_s"] = 2 * nb * k / (s1 - s0)
+        # ASM alone in complex128 on the device, and the FP32-bulk form's ratio over it
+        e_asm_full = descs(T.ENGINE_ASM, v_in, v_asm)
+        extra["asm_only_c128_device_propagations_per_s"] = asm_only_rate(G, torch, e_asm_full, nb, k, True, ctx)
+        extra["asm_only_c64_over_c128_device"] = (extra["asm_only_c64_propagations_per_s"] /
+                                                  extra["asm_only_c128_device_propagations_per_s"])
+        log(f"ASM only: c64 {extra['asm_only_c64_propagations_per_s']:.0f}/s, c128 device "
+            f"{extra['asm_only_c128_device_propagations_per_s']:.0f}/s, ratio {extra['asm_only_c64_over_c128_device']:.2f}")
         del v_in, v_asm, v_fr
         # f64 host drop-in (oc_asm_propagate: host memory in and out, PCIe included)
         um = (W.workload2_field(lams[0]) * np.exp(1j * phi)).astype(np.complex128)
@@ -694,6 +785,7 @@ def main():
         extra["gs"] = bench_gs(G, W, torch, np)
         extra["gs_asm"] = bench_gs_asm(G, W, T, np)
         extra["adjoint"] = bench_adjoint(G, W, T, torch, np, cpu=not args.no_cpu)
+        extra["adjoint_frames"] = bench_frames(G, W, T, torch, np)
         extra["rs_batch"] = bench_rs(G, W, T, torch, np, args.schedule_cache)
         extra["render"] = bench_render(G, W, T, torch, np, cpu=not args.no_cpu)
         extra["fd"] = bench_fd(G, W, T, torch, np)
@@ -717,6 +809,9 @@ def main():
                    "call": "chained (ASM field not written)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": (f"{TRAFFIC_FILE} (rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel and "
+                                        f"layout, copy-calibrated; committed, not measured in this run)"
+                                        if traffic is not None else None),
                      "alg_bytes_per_launch": alg[dom], "avg_launch_ms": avg_ms},
         "cpu_baseline": cpu,
         "extra": extra,

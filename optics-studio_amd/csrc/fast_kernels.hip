@@ -363,12 +363,40 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
                         v[m] = mg > R(0) ? cpx<R>{v[m].re * (a / mg), v[m].im * (a / mg)} : cpx<R>{a, R(0)};
                     }
                 }
-            } else if (EK == ENG_ASM) {
-                // the column's table entries once, the 16 rows' as a batch
+            } else if (EK == ENG_ASM || (EK == ENG_FRESNEL && sizeof(R) == 8)) {
+                // The ASM transfer and the complex128 Fresnel transfer depend
+                // on ky only through the axis table entry ay[ky], and
+                // ay[ky] = ay[N - ky] exactly (fftfreq negates exactly), so
+                // H(kx, ky) = H(kx, N - ky) bit for bit.  Each thread
+                // evaluates its rows t + mT for m < 8 and takes rows m >= 8
+                // from thread T - t, since N - (t + mT) = (T - t) + (15 - m) T.
+                // The values pass through LDS, free between the two
+                // transforms, as hs[j][T - t][g]; slot T holds thread 0's own
+                // rows (16 - m) T, row N/2 among them.  (The complex64
+                // Fresnel multiplies two table factors instead.)
                 const EngineDev& e = P.eng;
                 const double axk = at32(e.ax, kx);
+                auto tf = [&](int ky) -> cpx<R> {
+                    if constexpr (EK == ENG_ASM) return asm_tf<R>(e.kz, axk, at32(e.ay, ky));
+                    else return unit_phase<R>(e.kz - e.c * (axk + at32(e.ay, ky)));
+                };
+                cpx<R> h[8];
 #pragma unroll
-                for (int m = 0; m < 16; ++m) v[m] = asm_transfer<R>(e.kz, axk, at32(e.ay, t + m * T), v[m]);
+                for (int m = 0; m < 8; ++m) h[m] = tf(t + m * T);
+                cpx<R>* hs = lds;  // [8][T + 1][CB] within the exchange buffer
+#pragma unroll
+                for (int j = 0; j < 8; ++j) hs[(j * (T + 1) + t) * CB + g] = h[j];
+                if (t == 0) {
+                    const cpx<R> h8 = tf(8 * T);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) hs[(j * (T + 1) + T) * CB + g] = j < 7 ? h[j + 1] : h8;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], h[m]);
+#pragma unroll
+                for (int m = 8; m < 16; ++m) v[m] = cmul(v[m], hs[((15 - m) * (T + 1) + (T - t)) * CB + g]);
+                __syncthreads();  // before the inverse transform's exchanges reuse the buffer
             } else if (ek != ENG_NONE) {
 #pragma unroll
                 for (int m = 0; m < 16; ++m) v[m] = transfer<R, EK>(P.eng, kx, t + m * T, v[m]);

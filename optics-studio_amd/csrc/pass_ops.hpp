@@ -188,6 +188,17 @@ __device__ __forceinline__ cpx<R> asm_transfer(double kz, double axk, double ayk
     return dead ? cpx<R>{R(0), R(0)} : t;
 }
 
+// The ASM transfer value itself (asm.rs:273-281): zero when band-limited or
+// evanescent, else exp(i kz sqrt(s)); cmul(h, asm_tf(...)) equals
+// asm_transfer(..., h) wherever the value is not zero.
+template <typename R>
+__device__ __forceinline__ cpx<R> asm_tf(double kz, double axk, double ayk) {
+    const double s = (1.0 - axk) - ayk;
+    const bool dead = s < 0.0;
+    const cpx<R> u = unit_phase<R>(kz * asm_sqrt<R>(dead ? 0.0 : s));
+    return dead ? cpx<R>{R(0), R(0)} : u;
+}
+
 // EK: the engine when known at compile time (ENG_ASM / ENG_FRESNEL), else -1
 // and the descriptor's kind decides.
 template <typename R, int EK = -1>
