@@ -363,7 +363,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
                         v[m] = mg > R(0) ? cpx<R>{v[m].re * (a / mg), v[m].im * (a / mg)} : cpx<R>{a, R(0)};
                     }
                 }
-            } else if (EK == ENG_ASM || (EK == ENG_FRESNEL && sizeof(R) == 8)) {
+            } else if ((EK == ENG_ASM || (EK == ENG_FRESNEL && sizeof(R) == 8)) && (sizeof(R) == 8 || N >= 4096)) {
                 // The ASM transfer and the complex128 Fresnel transfer depend
                 // on ky only through the axis table entry ay[ky], and
                 // ay[ky] = ay[N - ky] exactly (fftfreq negates exactly), so
@@ -373,7 +373,11 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
                 // The values pass through LDS, free between the two
                 // transforms, as hs[j][T - t][g]; slot T holds thread 0's own
                 // rows (16 - m) T, row N/2 among them.  (The complex64
-                // Fresnel multiplies two table factors instead.)
+                // Fresnel multiplies two table factors instead.)  Measured
+                // on MI355X it pays in complex128 and from N = 4096 in
+                // complex64 (column pass 0.91 -> 0.87 ms at 4096); the
+                // shorter complex64 columns run faster without the extra
+                // exchange (0.45 against 0.58 ms at 2048).
                 const EngineDev& e = P.eng;
                 const double axk = at32(e.ax, kx);
                 auto tf = [&](int ky) -> cpx<R> {
@@ -397,6 +401,12 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
 #pragma unroll
                 for (int m = 8; m < 16; ++m) v[m] = cmul(v[m], hs[((15 - m) * (T + 1) + (T - t)) * CB + g]);
                 __syncthreads();  // before the inverse transform's exchanges reuse the buffer
+            } else if (EK == ENG_ASM) {
+                // the column's table entries once, the 16 rows' as a batch
+                const EngineDev& e = P.eng;
+                const double axk = at32(e.ax, kx);
+#pragma unroll
+                for (int m = 0; m < 16; ++m) v[m] = asm_transfer<R>(e.kz, axk, at32(e.ay, t + m * T), v[m]);
             } else if (ek != ENG_NONE) {
 #pragma unroll
                 for (int m = 0; m < 16; ++m) v[m] = transfer<R, EK>(P.eng, kx, t + m * T, v[m]);
