@@ -118,6 +118,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
+    TwKeep<R, N> tk;
+    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         int pi, lt;
         if (PK == PRE_TMUL && probs[0].ilv) {
@@ -185,7 +187,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
         if (bad) atomicOr(P.flag, 1);
         if (PK < 0 && P.ops.pre == PRE_SEED_MSE)
             accumulate4(acc, 0.0, 0.0, 0.0, P.ops.sums ? P.ops.sums + 3 : nullptr);
-        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw), tk);
         if (rowok) {
             cpx<R>* work = static_cast<cpx<R>*>(P.work);
             const unsigned ny = P.ny, b0 = ((unsigned)tq * ny + iy) * Q + tb;
@@ -211,6 +213,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
+    TwKeep<R, N> tk;
+    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     const bool ilv = probs[0].ilv != 0;
     auto locate = [&](int tile, int& pi, int& iy) {
         int lt;
@@ -267,7 +271,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
 #pragma unroll
         for (int m = 4; m < 12; ++m) bad |= !finite_c(v[m]);
         if (bad) atomicOr(Pc->flag, 1);
-        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(Pc->prow.tw));
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(Pc->prow.tw), tk);
         if (rowok) {
             cpx<R>* work = static_cast<cpx<R>*>(Pc->work);
             const unsigned ny = Pc->ny, b0 = ((unsigned)tq * ny + cy) * Q + tb;
@@ -279,7 +283,12 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
 
 // EK: engine (ENG_ASM, ENG_FRESNEL, ENG_GS_PROJECT, or -1 for the descriptor's); RS: the
 // Rayleigh-Sommerfeld product with the kernel spectrum instead.
-template <typename R, int N, int V, bool RS, int EK>
+// HW: every problem is N = P/2 on both axes, so the window is exactly
+// samples m = 4..11 of each thread's sixteen; the zeros outside it are
+// compile-time constants, and the compiler drops the forward transform's
+// first-stage work on them and the inverse's last-stage work on outputs never
+// stored.
+template <typename R, int N, int V, bool RS, int EK, bool HW = false>
 __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(const ProbDev* __restrict__ probs,
                                                                       int nprob, int total) {
     using LY = FastLayout<R, N, V>;
@@ -288,6 +297,13 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x;
     const int g = tid % CB, t = tid / CB;
+    // the half-window complex64 ASM form at N = 4096 in the 512-thread
+    // layouts (V2, V3) has the registers to keep its twiddles (table and
+    // hoist) within 128 VGPRs; that took the pass from 0.87 to 0.76 ms on
+    // MI355X.  Elsewhere the kept factors pushed the kernel past 128 VGPRs.
+    constexpr bool KEEP = HW && !RS && EK == ENG_ASM && sizeof(R) == 4 && N == 4096 && (V == 2 || V == 3);
+    TwKeep<R, N, KEEP, KEEP> tk;
+    tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
         const int kx = (tile - P.base_col) * CB + g;  // this thread's column
@@ -302,9 +318,9 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
             v[m] = {R(0), R(0)};
-            if (y >= oy && y < oy + ny) v[m] = at32(work, b0 + (unsigned)y * B);
+            if (HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) v[m] = at32(work, b0 + (unsigned)y * B);
         }
-        fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
+        fft_regs<R, N, CB, -1>(v, lds, g, t, tw, tk);
         if constexpr (RS) {
             // kernel row y is row min(y, N - y) of the stored half
             constexpr int KR = N / 2 + 1;
@@ -316,7 +332,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
                 const int y = t + m * T;
                 w[m] = at32(kb, k0 + (unsigned)min(y, N - y) * B);
             }
-            fft_regs<R, N, CB, -1>(w, lds, g, t, tw);
+            fft_regs<R, N, CB, -1>(w, lds, g, t, tw, tk);
 #pragma unroll
             for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], w[m]);
         } else {
@@ -412,11 +428,11 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
                 for (int m = 0; m < 16; ++m) v[m] = transfer<R, EK>(P.eng, kx, t + m * T, v[m]);
             }
         }
-        fft_regs<R, N, CB, +1>(v, lds, g, t, tw);
+        fft_regs<R, N, CB, +1>(v, lds, g, t, tw, tk);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int y = t + m * T;
-            if (y >= oy && y < oy + ny) at32(work, b0 + (unsigned)y * B) = v[m];
+            if (HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) at32(work, b0 + (unsigned)y * B) = v[m];
         }
     }
 }
@@ -434,6 +450,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
+    TwKeep<R, N> tk;
+    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
         const int iy = (tile - P.base_row) * G + g;
@@ -447,7 +465,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
             v[m] = {R(0), R(0)};
             if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
-        fft_regs<R, N, G, +1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw), tk);
         const R s = (R)P.scale;
         bool bad = false;
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
@@ -522,6 +540,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
+    TwKeep<R, N> tk;
+    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
         const int iy = (tile - P.base_row) * G + g;
@@ -535,7 +555,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(
             v[m] = {R(0), R(0)};
             if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
-        fft_regs<R, N, G, +1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw), tk);
         const R s = (R)P.scale;
         bool bad = false;
         double acc2 = 0.0;
@@ -595,7 +615,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(
         }
         if (bad) atomicOr(P.flag, 2);
         if (FULL) accumulate4(0.0, 0.0, acc2, 0.0, P.ops.sums);
-        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw), tk);
         if (rowok) {
 #pragma unroll
             for (int m = 0; m < 16; ++m) at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q) = v[m];
@@ -619,6 +639,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
+    TwKeep<R, N> tk;
+    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     constexpr int KR = N / 2 + 1;  // distinct kernel rows
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_KGEN>(probs, nprob, tile)];
@@ -628,7 +650,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
 #pragma unroll
         for (int m = 0; m < 16; ++m)
             v[m] = rowok ? conv_sample<R>(P.eng, t + m * T, y, N, N) : cpx<R>{R(0), R(0)};
-        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw));
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw), tk);
         if (rowok) {
             cpx<R>* kb = static_cast<cpx<R>*>(P.kbuf);
             const unsigned k0 = ((unsigned)tq * KR + y) * Q + tb;
@@ -662,6 +684,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x;
     const int g = tid % CB, t = tid / CB;
+    TwKeep<R, N> tk;
+    tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
         const int kx = (tile - P.base_col) * CB + g;  // input column
@@ -700,7 +724,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
         for (int m = 0; m < 16; ++m) bad |= !finite_c(v[m]);
         if (bad) atomicOr(P.flag, 1);
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
-        fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
+        fft_regs<R, N, CB, -1>(v, lds, g, t, tw, tk);
         const cpx<R>* hy = static_cast<const cpx<R>*>(P.eng.hy);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
@@ -708,7 +732,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
             h = at32(hy, t + m * T);
             v[m] = cmul(v[m], h);
         }
-        fft_regs<R, N, CB, +1>(v, lds, g, t, tw);
+        fft_regs<R, N, CB, +1>(v, lds, g, t, tw, tk);
         if (colok) {
             cpx<R>* work = static_cast<cpx<R>*>(P.work);
             const unsigned b0 = ((unsigned)(kx / B) * ny - oy) * B + kx % B;
@@ -733,6 +757,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
+    TwKeep<R, N> tk;
+    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
         const int iy = (tile - P.base_row) * G + g;
@@ -749,7 +775,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
             if (rowok && x >= ox && x < ox + nx) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.prow.tw);
-        fft_regs<R, N, G, -1, true>(v, lds, g, t, tw);
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, tw, tk);
         const cpx<R>* hx = static_cast<const cpx<R>*>(P.eng.hx);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
@@ -757,7 +783,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
             h = at32(hx, t + m * T);
             v[m] = cmul(v[m], h);
         }
-        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw);
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw, tk);
         const R s = (R)P.scale;
         bool bad = false;
         if (rowok) {
@@ -798,6 +824,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_lin
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
+    TwKeep<R, N> tk;
+    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(A[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const int pi = find_problem<PASS_ROW>(Bp, nprob, tile);
         const ProbDev& P = A[pi];
@@ -814,7 +842,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_lin
             if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.prow.tw);
-        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw);
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw, tk);
         const R s = (R)P.scale;
         cpx<R>* mid = static_cast<cpx<R>*>(P.out);
         bool bad = false;
@@ -832,7 +860,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_lin
             atomicOr(Nx.flag, 1);
         }
         row_pre<R, N, PK>(Nx, v, iy, rowok, t);
-        fft_regs<R, N, G, -1, true>(v, lds, g, t, tw);
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, tw, tk);
         if (rowok) {
             cpx<R>* nw = static_cast<cpx<R>*>(Nx.work);
 #pragma unroll
@@ -872,6 +900,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_cha
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
+    TwKeep<R, N> tk;
+    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(A[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const int pi = find_problem<PASS_ROW>(F, nprob, tile);  // F's row tiles (this kernel's G)
         const ProbDev& P = A[pi];
@@ -888,7 +918,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_cha
             if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.prow.tw);
-        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw);
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw, tk);
         // the first propagation's output, re-embedded where the Fresnel puts it
         const R s = (R)P.scale;
         cpx<R>* mid = static_cast<cpx<R>*>(P.out);
@@ -906,7 +936,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_cha
             atomicOr(P.flag, 2);
             atomicOr(Fp.flag, 1);
         }
-        fft_regs<R, N, G, -1, true>(v, lds, g, t, tw);
+        fft_regs<R, N, G, -1, true>(v, lds, g, t, tw, tk);
         const cpx<R>* hx = static_cast<const cpx<R>*>(Fp.eng.hx);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
@@ -914,7 +944,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_cha
             h = at32(hx, t + m * T);
             v[m] = cmul(v[m], h);
         }
-        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw);
+        fft_regs<R, N, G, +1, true>(v, lds, g, t, tw, tk);
         // column c = x - ox goes to block c / Q, lane c % Q = tb (ox % Q == 0)
         if (rowok) {
             cpx<R>* fw = static_cast<cpx<R>*>(Fp.work);
@@ -940,6 +970,8 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x;
     const int g = tid % CB, t = tid / CB;
+    TwKeep<R, N> tk;
+    tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
         const int kx = (tile - P.base_col) * CB + g;
@@ -956,7 +988,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
             if ((HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) && colok) v[m] = at32(work, b0 + (unsigned)y * B);
         }
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
-        fft_regs<R, N, CB, -1>(v, lds, g, t, tw);
+        fft_regs<R, N, CB, -1>(v, lds, g, t, tw, tk);
         const cpx<R>* hy = static_cast<const cpx<R>*>(P.eng.hy);
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
@@ -964,7 +996,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
             h = at32(hy, t + m * T);
             v[m] = cmul(v[m], h);
         }
-        fft_regs<R, N, CB, +1>(v, lds, g, t, tw);
+        fft_regs<R, N, CB, +1>(v, lds, g, t, tw, tk);
         const R s = (R)P.scale;
         bool bad = false;
         if (colok) {
@@ -996,13 +1028,23 @@ using Real = float;
 #define OC_FAST_FN(name) name##_c64
 #endif
 
+// Whether the half-window ASM column pass stays within its register budget:
+// in complex64 the 1024-thread layout V0 spills at N = 2048 and 8192 with it
+// (56 and 24 bytes) where the runtime-window kernel does not, so those run
+// the runtime window.
+template <typename R, int N, int V>
+constexpr bool col_half_fits() {
+    return !(sizeof(R) == 4 && V == 0 && (N == 2048 || N == 8192));
+}
+
 template <int N, int V>
 static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int total, int grid, hipStream_t st) {
     using R = Real;
     using LY = FastLayout<R, N, V>;
     constexpr int T = N / 16;
+    // exchange buffer, then the kept twiddle table
     const size_t row_lds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
-    const size_t col_lds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
+    const size_t col_lds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
     const dim3 brow(LY::G * T), bcol(LY::CB * T), g(grid);
     if (s.sep) {  // separable complex64 Fresnel (pass 1: columns, pass 2: rows)
         if constexpr (sizeof(R) == 4) {
@@ -1046,6 +1088,8 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
         case 1:
             if (s.rs)
                 hipLaunchKernelGGL((f_col<R, N, V, true, -1>), g, bcol, col_lds, st, d, np, total);
+            else if (s.col_kind == ENG_ASM && s.half && col_half_fits<R, N, V>())
+                hipLaunchKernelGGL((f_col<R, N, V, false, ENG_ASM, true>), g, bcol, col_lds, st, d, np, total);
             else if (s.col_kind == ENG_ASM)
                 hipLaunchKernelGGL((f_col<R, N, V, false, ENG_ASM>), g, bcol, col_lds, st, d, np, total);
             else if (s.col_kind == ENG_FRESNEL)
@@ -1084,7 +1128,7 @@ static hipError_t launch_chain_nv(int which, const ProbDev* a, const ProbDev* f,
         constexpr int T = N / 16;
         const dim3 g(grid);
         const size_t rlds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
-        const size_t clds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>);
+        const size_t clds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
         switch (which) {  // + 2: every problem is N = P/2
             case 0: hipLaunchKernelGGL((f_row_chain<R, N, V>), g, dim3(LY::G * T), rlds, st, a, f, np, total); break;
             case 1: hipLaunchKernelGGL((f_sep_col_fin<R, N, V>), g, dim3(LY::CB * T), clds, st, f, np, total); break;
@@ -1138,6 +1182,7 @@ static hipError_t attr_nv() {
                        (const void*)f_row_fwd<R, N, V, -1>,
                        (const void*)f_col<R, N, V, true, -1>,
                        (const void*)f_col<R, N, V, false, ENG_ASM>,
+                       (const void*)f_col<R, N, V, false, ENG_ASM, true>,
                        (const void*)f_col<R, N, V, false, ENG_FRESNEL>,
                        (const void*)f_col<R, N, V, false, ENG_GS_PROJECT>,
                        (const void*)f_col<R, N, V, false, -1>,

@@ -71,6 +71,78 @@ __device__ __forceinline__ void fft_exchange16(cpx<R> (&v)[16], cpx<R>* __restri
     __syncthreads();
 }
 
+// w^r for r = 1..15 (w[0] = 1) from the table entries w, w^2, w^4, w^8 with
+// at most three extra products each.  Every radix-16 twiddle stage forms its
+// factors this way, kept or not, so all kernels round them alike.
+template <typename R>
+__device__ __forceinline__ void tw_powers(cpx<R> w1, cpx<R> w2, cpx<R> w4, cpx<R> w8, cpx<R> (&w)[16]) {
+    w[0] = {R(1), R(0)};
+    w[1] = w1;
+    w[2] = w2;
+    w[3] = cmul(w1, w2);
+    w[4] = w4;
+    w[5] = cmul(w1, w4);
+    w[6] = cmul(w2, w4);
+    w[7] = cmul(w[3], w4);
+    w[8] = w8;
+#pragma unroll
+    for (int r = 9; r < 16; ++r) w[r] = cmul(w[r - 8], w8);
+}
+
+// Twiddles a kernel keeps for all of its tiles (every transform of one
+// launch has the same length N, so the same factors):
+//  - TAB: the second radix-16 stage (Ns = 16) multiplies input r of
+//    butterfly k = t mod 16 by w_k^r, w_k = W_N^{k N/256}; those 16 x 16
+//    factors sit in LDS as [r][k] (the lanes of a wave read consecutive k:
+//    no bank conflicts) and are read instead of formed per transform;
+//  - HOIST: the third radix-16 stage (Ns = 256, N = 4096 and 8192) has k =
+//    t mod 256, fixed for the thread, so its fifteen factors are formed once,
+//    before the tile loop.
+// Both hold the products of tw_powers, so a kernel gives the same bits with
+// or without them; each kernel enables what its registers allow.  The
+// inverse transform conjugates at use (the conjugate of a product of
+// conjugates is that product, exactly).  `init` is called by every thread of
+// the block before its first transform.
+template <typename R, int N, bool TAB_OK = false, bool HOIST_OK = false>
+struct TwKeep {
+    static constexpr bool TAB = TAB_OK && Pow2Plan<N>::NS16 >= 2;
+    static constexpr bool HOIST = HOIST_OK && Pow2Plan<N>::NS16 >= 3;
+    static constexpr int TAB_ELEMS = TAB ? 256 : 0;
+    const cpx<R>* tab = nullptr;
+    cpx<R> w3[16];
+    // `slot`: TAB_ELEMS elements of LDS beside the exchange buffer; `tw`: the
+    // length-N table exp(-2 pi i e/N); t: the thread's index in its
+    // transform; NT: the block size (a constant: with blockDim.x read inside
+    // the fill loop the compiler took later uniform values, gridDim.x among
+    // them, for divergent ones and moved the tile loop's descriptor reads to
+    // vector loads)
+    template <int NT>
+    __device__ __forceinline__ void init(cpx<R>* slot, const cpx<R>* __restrict__ tw, int t) {
+        if constexpr (TAB) {
+            for (int i = (int)threadIdx.x; i < 256; i += NT) {
+                const int r = i >> 4, e = (i & 15) * (N / 256);
+                cpx<R> w[16];
+                tw_powers(tw_at<-1>(tw, e), tw_at<-1>(tw, 2 * e), tw_at<-1>(tw, 4 * e), tw_at<-1>(tw, 8 * e), w);
+                cpx<R> x = w[0];
+#pragma unroll
+                for (int q = 1; q < 16; ++q) x = q == r ? w[q] : x;
+                slot[i] = x;
+            }
+            tab = slot;
+            __syncthreads();
+        }
+        if constexpr (HOIST) {
+            const int e = (t & 255) * (N / 4096);
+            tw_powers(tw_at<-1>(tw, e), tw_at<-1>(tw, 2 * e), tw_at<-1>(tw, 4 * e), tw_at<-1>(tw, 8 * e), w3);
+        }
+    }
+};
+// LDS bytes a kernel with table `tab` adds for it (after the exchange buffer).
+template <typename R, int N>
+constexpr size_t tw_keep_bytes(bool tab) {
+    return tab && Pow2Plan<N>::NS16 >= 2 ? (size_t)256 * sizeof(cpx<R>) : 0;
+}
+
 template <int DIR, typename R>
 __device__ __forceinline__ void dft_small(cpx<R>* w, int L) {
     if (L == 2) dft2<DIR>(w[0], w[1]);
@@ -81,17 +153,32 @@ __device__ __forceinline__ void dft_small(cpx<R>* w, int L) {
 // G transforms of length N in place in registers (see header).  `active`
 // threads participate in the arithmetic; every thread of the block must call
 // this (barriers).
-template <typename R, int N, int G, int DIR, bool RM = false>
+template <typename R, int N, int G, int DIR, bool RM = false, bool KT = false, bool KH = false>
 __device__ __forceinline__ void fft_regs(cpx<R> (&v)[16], cpx<R>* __restrict__ lds, int g, int t,
-                                         const cpx<R>* __restrict__ tw) {
+                                         const cpx<R>* __restrict__ tw, const TwKeep<R, N, KT, KH>& tk) {
     using PL = Pow2Plan<N>;
+    using TK = TwKeep<R, N, KT, KH>;
     constexpr int T = PL::T;
 #pragma unroll
     for (int s = 0; s < PL::NS16; ++s) {
         const int Ns = 1 << (4 * s);
-        if (s > 0) {
-            // w^r for r = 1..15 from the table entries w, w^2, w^4, w^8 with
-            // at most three extra products each.
+        if (s == 1 && TK::TAB) {
+            const cpx<R>* __restrict__ tb = tk.tab + (t & 15);
+#pragma unroll
+            for (int r = 1; r < 16; ++r) {
+                cpx<R> w = tb[r * 16];
+                if (DIR > 0) w.im = -w.im;
+                v[r] = cmul(v[r], w);
+            }
+        } else if (s == 2 && TK::HOIST) {
+#pragma unroll
+            for (int r = 1; r < 16; ++r) {
+                cpx<R> w = tk.w3[r];
+                if (DIR > 0) w.im = -w.im;
+                v[r] = cmul(v[r], w);
+            }
+        } else if (s > 0) {
+            // the products of tw_powers, interleaved with their use
             const int k = t & (Ns - 1);
             const int e = k * (N / (Ns * 16));
             const cpx<R> w1 = tw_at<DIR>(tw, e), w2 = tw_at<DIR>(tw, 2 * e);
