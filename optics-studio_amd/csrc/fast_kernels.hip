@@ -213,7 +213,9 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N> tk;
+    // the table (not the hoist: 138 VGPRs or a spill) at N = 4096
+    // (row pass 0.456 -> 0.435 ms on the 16-band 2048^2 batch)
+    TwKeep<R, N, N == 4096 && sizeof(R) == 4> tk;
     tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     const bool ilv = probs[0].ilv != 0;
     auto locate = [&](int tile, int& pi, int& iy) {
@@ -900,7 +902,9 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_cha
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N> tk;
+    // table and hoist in the half-window form at N = 4096 (117 VGPRs; the
+    // pass went from 0.72 to 0.62 ms on the 16-band 2048^2 chain)
+    TwKeep<R, N, HW && N == 4096, HW && N == 4096> tk;
     tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(A[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const int pi = find_problem<PASS_ROW>(F, nprob, tile);  // F's row tiles (this kernel's G)
@@ -970,6 +974,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x;
     const int g = tid % CB, t = tid / CB;
+    // (keeping the twiddles here measured no faster: 0.51-0.52 ms either way)
     TwKeep<R, N> tk;
     tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
@@ -1043,7 +1048,7 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
     using LY = FastLayout<R, N, V>;
     constexpr int T = N / 16;
     // exchange buffer, then the kept twiddle table
-    const size_t row_lds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
+    const size_t row_lds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
     const size_t col_lds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
     const dim3 brow(LY::G * T), bcol(LY::CB * T), g(grid);
     if (s.sep) {  // separable complex64 Fresnel (pass 1: columns, pass 2: rows)
@@ -1127,7 +1132,7 @@ static hipError_t launch_chain_nv(int which, const ProbDev* a, const ProbDev* f,
         using LY = FastLayout<R, N, V>;
         constexpr int T = N / 16;
         const dim3 g(grid);
-        const size_t rlds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
+        const size_t rlds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
         const size_t clds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
         switch (which) {  // + 2: every problem is N = P/2
             case 0: hipLaunchKernelGGL((f_row_chain<R, N, V>), g, dim3(LY::G * T), rlds, st, a, f, np, total); break;
