@@ -118,7 +118,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N> tk;
+    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && (PK == PRE_NONE || PK == PRE_TMUL)> tk;
     tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         int pi, lt;
@@ -452,7 +452,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N> tk;
+    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && QK == POST_NONE> tk;
     tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
@@ -641,7 +641,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N> tk;
+    TwKeep<R, N, N == 4096 && sizeof(R) == 4> tk;
     tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     constexpr int KR = N / 2 + 1;  // distinct kernel rows
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
@@ -686,7 +686,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x;
     const int g = tid % CB, t = tid / CB;
-    TwKeep<R, N> tk;
+    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && PK == PRE_NONE> tk;
     tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
@@ -759,7 +759,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N> tk;
+    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4> tk;
     tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
@@ -826,7 +826,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_lin
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N> tk;
+    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && PK == PRE_NONE> tk;
     tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(A[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const int pi = find_problem<PASS_ROW>(Bp, nprob, tile);
