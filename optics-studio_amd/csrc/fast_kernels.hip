@@ -20,6 +20,8 @@
 // complex128, otherwise complex64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fft_fast.hpp"
 #include "pass_ops.hpp"
 
@@ -324,19 +326,17 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
         }
         fft_regs<R, N, CB, -1>(v, lds, g, t, tw, tk);
         if constexpr (RS) {
-            // kernel row y is row min(y, N - y) of the stored half
+            // the kernel's 2-D spectrum from f_rs_kcol: W(kx, ky) is entry
+            // (min(kx, N - kx), min(ky, N - ky)) of the stored quarter
             constexpr int KR = N / 2 + 1;
-            const cpx<R>* kb = static_cast<const cpx<R>*>(P.kbuf);
-            const unsigned k0 = (unsigned)q * KR * B + bq;
-            cpx<R> w[16];
+            const cpx<R>* wb = static_cast<const cpx<R>*>(P.kbuf) + (size_t)N * KR;
+            const int kh = min(kx, N - kx);
+            const unsigned w0 = (unsigned)(kh / B) * KR * B + kh % B;
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
                 const int y = t + m * T;
-                w[m] = at32(kb, k0 + (unsigned)min(y, N - y) * B);
+                v[m] = cmul(v[m], at32(wb, w0 + (unsigned)min(y, N - y) * B));
             }
-            fft_regs<R, N, CB, -1>(w, lds, g, t, tw, tk);
-#pragma unroll
-            for (int m = 0; m < 16; ++m) v[m] = cmul(v[m], w[m]);
         } else {
             const int ek = EK >= 0 ? EK : P.eng.kind;
             if (ek == ENG_FRAUNHOFER) {
@@ -654,10 +654,60 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
             v[m] = rowok ? conv_sample<R>(P.eng, t + m * T, y, N, N) : cpx<R>{R(0), R(0)};
         fft_regs<R, N, G, -1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw), tk);
         if (rowok) {
+            // the kernel is even in x too, so only kx = t + mT <= N/2 (m < 8,
+            // and m = 8 at t = 0) is needed
             cpx<R>* kb = static_cast<cpx<R>*>(P.kbuf);
             const unsigned k0 = ((unsigned)tq * KR + y) * Q + tb;
 #pragma unroll
-            for (int m = 0; m < 16; ++m) at32(kb, k0 + (unsigned)(m * (T / Q)) * KR * Q) = v[m];
+            for (int m = 0; m < 9; ++m)
+                if (m < 8 || t == 0) at32(kb, k0 + (unsigned)(m * (T / Q)) * KR * Q) = v[m];
+        }
+    }
+}
+
+// Column transforms of the kernel's row spectra: the 2-D spectrum of the
+// sampled kernel, W(kx, ky).  The kernel depends on x and y only through x^2
+// and y^2, and the DFT-order coordinates of index i and N - i are exact
+// negatives, so W(kx, ky) = W(N - kx, ky) = W(kx, N - ky).  This pass
+// computes the distinct columns kx = 0..N/2 (each a transform over all N
+// rows, row y read from the stored row min(y, N - y)) and keeps rows
+// ky = 0..N/2.  W sits after the row spectra in the problem's kernel buffer,
+// blocked [ceil(KR/B)][KR][B] with KR = N/2 + 1.  The field's column pass
+// then reads W instead of transforming the kernel for every column: half
+// the kernel transforms, and one transform fewer held in its registers.
+template <typename R, int N, int V>
+__global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_rs_kcol(const ProbDev* __restrict__ probs,
+                                                                          int nprob, int total) {
+    using LY = FastLayout<R, N, V>;
+    constexpr int T = N / 16, B = LY::B, CB = LY::CB;
+    constexpr int KR = N / 2 + 1;
+    constexpr int TPP = (KR + CB - 1) / CB;  // tiles per problem
+    extern __shared__ unsigned char smem_raw[];
+    cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int g = tid % CB, t = tid / CB;
+    TwKeep<R, N> tk;
+    tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
+    for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
+        const ProbDev& P = probs[tile / TPP];
+        const int kx = (tile % TPP) * CB + g;  // 0..KR-1 where colok
+        const bool colok = kx < KR;
+        const cpx<R>* kb = static_cast<const cpx<R>*>(P.kbuf);
+        const unsigned k0 = (unsigned)(kx / B) * KR * B + kx % B;
+        cpx<R> v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int y = t + m * T;
+            v[m] = {R(0), R(0)};
+            if (colok) v[m] = at32(kb, k0 + (unsigned)min(y, N - y) * B);
+        }
+        fft_regs<R, N, CB, -1>(v, lds, g, t, static_cast<const cpx<R>*>(P.pcol.tw), tk);
+        if (colok) {
+            cpx<R>* wb = static_cast<cpx<R>*>(P.kbuf) + (size_t)N * KR;
+            const unsigned w0 = (unsigned)(kx / B) * KR * B + kx % B;
+#pragma unroll
+            for (int m = 0; m < 9; ++m)
+                if (m < 8 || t == 0) at32(wb, w0 + (unsigned)(t + m * T) * B) = v[m];
         }
     }
 }
@@ -1114,7 +1164,15 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
             else
                 hipLaunchKernelGGL((f_row_inv<R, N, V, -1>), g, brow, row_lds, st, d, np, total);
             break;
-        case 3: hipLaunchKernelGGL((f_rs_kgen<R, N, V>), g, brow, row_lds, st, d, np, total); break;
+        case 3: {  // the kernel's row spectra (total = row tiles), then its distinct columns
+            hipLaunchKernelGGL((f_rs_kgen<R, N, V>), g, brow, row_lds, st, d, np, total);
+            // persistent: at most the grid the caller chose when it capped
+            // the row tiles, else one workgroup per tile up to 2048
+            const int kt = np * ((N / 2 + 1 + LY::CB - 1) / LY::CB);
+            const int kg = grid < total ? std::min(kt, grid) : std::min(kt, std::max(grid, 2048));
+            hipLaunchKernelGGL((f_rs_kcol<R, N, V>), dim3(kg), bcol, col_lds, st, d, np, kt);
+            break;
+        }
         case 4: hipLaunchKernelGGL((f_row_gs<R, N, V, true>), g, brow, row_lds, st, d, np, total); break;
         case 5: hipLaunchKernelGGL((f_row_gs<R, N, V, false>), g, brow, row_lds, st, d, np, total); break;
         default: return hipErrorInvalidValue;
@@ -1196,6 +1254,7 @@ static hipError_t attr_nv() {
                        (const void*)f_row_inv<R, N, V, POST_ADJ_GRAD>,
                        (const void*)f_row_inv<R, N, V, -1>,
                        (const void*)f_rs_kgen<R, N, V>,
+                       (const void*)f_rs_kcol<R, N, V>,
                        (const void*)f_row_gs<R, N, V, true>,
                        (const void*)f_row_gs<R, N, V, false>,
                        (const void*)f_row_link<R, N, V, PRE_NONE>,
