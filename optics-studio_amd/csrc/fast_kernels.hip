@@ -304,8 +304,10 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
     // the half-window complex64 ASM form at N = 4096 in the 512-thread
     // layouts (V2, V3) has the registers to keep its twiddles (table and
     // hoist) within 128 VGPRs; that took the pass from 0.87 to 0.76 ms on
-    // MI355X.  Elsewhere the kept factors pushed the kernel past 128 VGPRs.
-    constexpr bool KEEP = HW && !RS && EK == ENG_ASM && sizeof(R) == 4 && N == 4096 && (V == 2 || V == 3);
+    // MI355X.  The half-window convolution form (RS) has them in V1-V3.
+    // Elsewhere the kept factors pushed the kernel past 128 VGPRs.
+    constexpr bool KEEP = HW && sizeof(R) == 4 && N == 4096 &&
+                          ((!RS && EK == ENG_ASM && (V == 2 || V == 3)) || (RS && V != 0));
     TwKeep<R, N, KEEP, KEEP> tk;
     tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
@@ -1141,7 +1143,9 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
                 hipLaunchKernelGGL((f_row_fwd<R, N, V, -1>), g, brow, row_lds, st, d, np, total);
             break;
         case 1:
-            if (s.rs)
+            if (s.rs && s.half)
+                hipLaunchKernelGGL((f_col<R, N, V, true, -1, true>), g, bcol, col_lds, st, d, np, total);
+            else if (s.rs)
                 hipLaunchKernelGGL((f_col<R, N, V, true, -1>), g, bcol, col_lds, st, d, np, total);
             else if (s.col_kind == ENG_ASM && s.half && col_half_fits<R, N, V>())
                 hipLaunchKernelGGL((f_col<R, N, V, false, ENG_ASM, true>), g, bcol, col_lds, st, d, np, total);
@@ -1244,6 +1248,7 @@ static hipError_t attr_nv() {
                        (const void*)f_row_fwd<R, N, V, PRE_MASKS>,
                        (const void*)f_row_fwd<R, N, V, -1>,
                        (const void*)f_col<R, N, V, true, -1>,
+                       (const void*)f_col<R, N, V, true, -1, true>,
                        (const void*)f_col<R, N, V, false, ENG_ASM>,
                        (const void*)f_col<R, N, V, false, ENG_ASM, true>,
                        (const void*)f_col<R, N, V, false, ENG_FRESNEL>,
