@@ -79,6 +79,7 @@ void oc_context_destroy(oc_context* ctx) {
         if (ctx->pinned[k]) (void)hipHostFree(ctx->pinned[k]);
         if (ctx->pinned_done[k]) (void)hipEventDestroy(ctx->pinned_done[k]);
     }
+    if (ctx->readback) (void)hipHostFree(ctx->readback);
     release_queued(ctx);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;

@@ -510,7 +510,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
                         ck = at32(static_cast<const cpx<R>*>(P.ops.ckpt), k);
                         const cpx<R> vv = cmul(ck, tt);
                         const double gim = (double)o.re * (double)vv.im + (-(double)o.im) * (double)vv.re;
-                        at32(P.ops.grad, k) += -2.0 * gim;
+                        at32(P.ops.grad, k) = 0.0 + -2.0 * gim;  // the plane's one term (no clearing pass)
                         at32(out, k) = cmul(o, cpx<R>{tt.re, -tt.im});
                     } else {
                         store_post<R>(P, x, iy, o, acc0, acc1, acc2);

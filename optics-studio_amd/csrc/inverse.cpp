@@ -721,23 +721,28 @@ int adjoint_impl(oc_context* ctx, bool dbl, const oc_band_geom& g, const AdjFram
         HIP_TRY(ctx, ocl::launch_seed_frames(dbl, det, target, co, n, (int)fr.nb, (int)fr.ns, fr.sel_band,
                                              fr.sel_scene, static_cast<double*>(sums), ctx->stream));
     }
+    // the merit sums and the flags come back through the context's pinned
+    // read-back buffer: two asynchronous copies and one wait
     auto finish = [&]() -> int {
-        double hs[4];
-        std::vector<int> hf(queued.size() * F + 1, 0);
+        const size_t nf = queued.size() * F;
+        void* rb = nullptr;
+        if (int s2 = ensure_readback(ctx, 32 + sizeof(int) * nf, &rb)) return s2;
+        double* hs = static_cast<double*>(rb);
+        int* hf = reinterpret_cast<int*>(static_cast<unsigned char*>(rb) + 32);
         HIP_TRY(ctx, hipMemcpyAsync(hs, sums, 32, hipMemcpyDeviceToHost, ctx->stream));
-        if (!queued.empty())
-            HIP_TRY(ctx, hipMemcpyAsync(hf.data(), dflags, sizeof(int) * queued.size() * F, hipMemcpyDeviceToHost,
-                                        ctx->stream));
+        if (nf) HIP_TRY(ctx, hipMemcpyAsync(hf, dflags, sizeof(int) * nf, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         collect_timing(ctx);
         if (merit) *merit = hs[0] / (double)n;
-        for (size_t i = 0; i < queued.size() * F; ++i)
+        for (size_t i = 0; i < nf; ++i)
             if (hf[i]) return fail(ctx, fwd_err, "non-finite samples in a propagation");
         return OC_OK;
     };
     if (!grads) return finish();
+    // every element plane (per frame with a slab) is written exactly once by
+    // the reverse sweep, as 0 + its term (the reference's zero-initialised
+    // sum of one term), so the planes need no clearing first
     double* const gbase = slab ? static_cast<double*>(slab) : grads;
-    if (ne) HIP_TRY(ctx, hipMemsetAsync(gbase, 0, (slab ? F : 1) * ne * n * 8, ctx->stream));
     auto gplane = [&](size_t f, size_t el) { return gbase + ((slab ? f * ne : 0) + el) * n; };
     // reverse sweep
     unsigned char* c0 = co;

@@ -519,7 +519,7 @@ __device__ __forceinline__ void store_post(const ProbDev& P, int ix, int iy, cpx
             const double mre = o.re, mim = o.im;
             // -2 Im(conj(mu) V), conj(mu) = (mre, -mim)
             const double gim = mre * (double)v.im + (-mim) * (double)v.re;
-            op.grad[k] += -2.0 * gim;
+            op.grad[k] = 0.0 + -2.0 * gim;  // the plane's one term (no clearing pass)
             // co *= conj(t): re = mu.re*t.re - mu.im*(-t.im); im = mu.im*t.re + mu.re*(-t.im)
             const R tr = t.re, ti = -t.im;
             cpx<R> c;

@@ -300,7 +300,8 @@ __global__ void k_seed(const cpx<R>* __restrict__ u, const double* __restrict__ 
 }
 
 // Element gradient at a plane with no propagation after it (z_after == 0):
-// grad += -2 Im(conj(co) V), co *= conj(t)   (adjoint.rs:222-236)
+// grad = 0 + -2 Im(conj(co) V) (the plane's one term: the reverse sweep
+// writes each element plane once), co *= conj(t)   (adjoint.rs:222-236)
 template <typename R>
 __global__ void k_adj_elem(cpx<R>* __restrict__ co, const cpx<R>* __restrict__ ckpt, double* __restrict__ grad,
                            MaskDev m, int nx, int ny) {
@@ -311,7 +312,7 @@ __global__ void k_adj_elem(cpx<R>* __restrict__ co, const cpx<R>* __restrict__ c
         const cpx<R> v = apply_mask(m, ix, iy, ckpt[i]);
         const cpx<R> o = co[i];
         const double gim = (double)o.re * (double)v.im + (-(double)o.im) * (double)v.re;
-        grad[i] += -2.0 * gim;
+        grad[i] = 0.0 + -2.0 * gim;
         const R tr = t.re, ti = -t.im;
         cpx<R> c;
         c.re = o.re * tr - o.im * ti;

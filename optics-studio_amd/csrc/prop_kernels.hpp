@@ -124,7 +124,8 @@ struct OpsDev {
     void* phase_io;  // element type R
     double beta_p, lo, hi;
     double* dphi_out;  // f64, may be null
-    // POST_ADJ_GRAD: mu = output; V = ckpt * t; grad += -2 Im(conj(mu) V); out = conj(t) mu
+    // POST_ADJ_GRAD: mu = output; V = ckpt * t; grad = 0 + -2 Im(conj(mu) V) (the element plane's one
+    // term); out = conj(t) mu
     const void* ckpt;
     double* grad;
     // ENG_GS_PROJECT / ENG_MEASURE: Fourier-plane target amplitude, blocked

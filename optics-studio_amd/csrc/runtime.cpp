@@ -106,6 +106,19 @@ int ensure_buf(oc_context* ctx, const std::string& name, size_t bytes, void** ou
     return OC_OK;
 }
 
+int ensure_readback(oc_context* ctx, size_t bytes, void** out) {
+    if (ctx->readback_cap < bytes) {
+        if (ctx->readback) HIP_TRY(ctx, hipHostFree(ctx->readback));
+        ctx->readback = nullptr;
+        ctx->readback_cap = 0;
+        const size_t cap = std::max(bytes, (size_t)1 << 16);
+        HIP_TRY(ctx, hipHostMalloc(&ctx->readback, cap, hipHostMallocDefault));
+        ctx->readback_cap = cap;
+    }
+    *out = ctx->readback;
+    return OC_OK;
+}
+
 int ensure_pinned(oc_context* ctx, int slot, size_t bytes) {
     if (ctx->pinned_cap[slot] >= bytes) return OC_OK;
     if (!ctx->pinned[slot] && bytes <= (size_t)(1 << 18)) {

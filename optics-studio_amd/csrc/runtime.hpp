@@ -142,6 +142,10 @@ struct oc_context {
     size_t pinned_cap[kPinSlots] = {};
     hipEvent_t pinned_done[kPinSlots] = {};
     int pin_next = 0;
+    // pinned host buffer for small read-backs at the end of a call (a
+    // pageable destination makes the runtime stage and wait per copy)
+    void* readback = nullptr;
+    size_t readback_cap = 0;
     // per-pass timing
     bool timing = false;
     uint32_t timing_mask = 0xFFFFFFFFu;  // pass kinds that get timing brackets
@@ -252,6 +256,7 @@ int noalloc_attach(oc_context* ctx);
 // Staging slot `slot` with room for `bytes` (the caller has waited for the
 // slot's previous copy).
 int ensure_pinned(oc_context* ctx, int slot, size_t bytes);
+int ensure_readback(oc_context* ctx, size_t bytes, void** out);
 int get_twiddles(oc_context* ctx, int n, bool dbl, const void** out);
 bool make_radices(int n, std::vector<int>& rad, bool& generic);
 // Plan of length n: the default radices, or rad[0..nrad) when given.

@@ -781,6 +781,36 @@ def adjoint_grid_gradient_dev(dev_steps, u_dev, dx, dy, lam, asm_cfg, target_dev
     return st, merit.value
 
 
+def adjoint_grid_gradient_dev_prepared(dev_steps, u_dev, dx, dy, lam, asm_cfg, target_dev, grads_dev, ctx=None):
+    """The call of adjoint_grid_gradient_dev with its arguments marshalled
+    once, as a compiled caller holds them: returns a function that runs
+    oc_adjoint_grid_gradient_dev_c128 and returns (status, merit).  The
+    caller orders its own work on other streams before each call (no torch
+    synchronisation here)."""
+    ctx = ctx or context()
+    arr = (T.DesignStep * len(dev_steps))()
+    for i, (ms, z, med) in enumerate(dev_steps):
+        arr[i].has_element = 1 if ms is not None else 0
+        if ms is not None:
+            arr[i].mask = ms
+        arr[i].z_after = z
+        arr[i].medium = med
+    ny, nx = u_dev.shape
+    g = T.geom(nx, ny, dx, dy, lam)
+    merit = c_f64()
+    fn = lib.oc_adjoint_grid_gradient_dev_c128
+    args = (ctx.h, arr, len(dev_steps), C.byref(g), u_dev.data_ptr(), C.byref(asm_cfg), target_dev.data_ptr(),
+            grads_dev.data_ptr(), C.byref(merit))
+    keep = (arr, g, merit, asm_cfg, u_dev, target_dev, grads_dev)
+
+    def call():
+        _ = keep
+        st = fn(*args)
+        return st, merit.value
+
+    return call
+
+
 def adjoint_grid_gradient_frames_dev(designs, inputs_dev, dx, dy, bands, asm_cfg, target_dev, grads_dev=None,
                                      sel_band=-1, sel_scene=-1, ctx=None):
     """adjoint::grid_gradient (adjoint.rs:268-340) with MseToTarget over every
