@@ -80,6 +80,7 @@ void oc_context_destroy(oc_context* ctx) {
         if (ctx->pinned_done[k]) (void)hipEventDestroy(ctx->pinned_done[k]);
     }
     if (ctx->readback) (void)hipHostFree(ctx->readback);
+    if (ctx->desc_pool) (void)hipFree(ctx->desc_pool);
     release_queued(ctx);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;

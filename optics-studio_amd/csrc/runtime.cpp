@@ -835,7 +835,32 @@ int build_batch(oc_context* ctx, bool dbl, std::vector<Prob>& probs, int* dflags
     for (const ProbDev& d : b.pd) fraun = fraun && d.eng.kind == ENG_FRAUNHOFER;
     if (fraun) b.pc[2].total = 0;  // the column pass writes the far field
     const size_t bytes = b.pd.size() * sizeof(ProbDev);
-    int st = ctx->arena ? OC_OK : batch_alloc(ctx, b, persistent, "probs" + tag, bytes, &b.dprobs);
+    const bool cached = !ctx->arena && !persistent && bytes <= oc_context::kDescSlotBytes;
+    int st = OC_OK;
+    if (cached) {
+        if (!ctx->desc_pool)
+            HIP_TRY(ctx, hipMalloc(&ctx->desc_pool, oc_context::kDescSlots * oc_context::kDescSlotBytes));
+        const unsigned char* hb = reinterpret_cast<const unsigned char*>(b.pd.data());
+        uint64_t h = 1469598103934665603ull;  // FNV-1a
+        for (size_t i = 0; i < bytes; ++i) h = (h ^ hb[i]) * 1099511628211ull;
+        int hit = -1, lru = 0;
+        for (int k = 0; k < oc_context::kDescSlots; ++k) {
+            if (ctx->desc_hash[k] == h && ctx->desc_host[k].size() == bytes &&
+                std::memcmp(ctx->desc_host[k].data(), hb, bytes) == 0) {
+                hit = k;
+                break;
+            }
+            if (ctx->desc_used[k] < ctx->desc_used[lru]) lru = k;
+        }
+        const int k = hit >= 0 ? hit : lru;
+        ctx->desc_used[k] = ++ctx->desc_clock;
+        b.dprobs = static_cast<unsigned char*>(ctx->desc_pool) + (size_t)k * oc_context::kDescSlotBytes;
+        if (hit >= 0) return OC_OK;
+        ctx->desc_hash[k] = h;
+        ctx->desc_host[k].assign(hb, hb + bytes);
+    } else if (!ctx->arena) {
+        st = batch_alloc(ctx, b, persistent, "probs" + tag, bytes, &b.dprobs);
+    }
     if (st) return st;
     if (ctx->arena) {
         const void* d = nullptr;

@@ -146,6 +146,19 @@ struct oc_context {
     // pageable destination makes the runtime stage and wait per copy)
     void* readback = nullptr;
     size_t readback_cap = 0;
+    // Device copies of recent transient descriptor arrays.  A batch whose
+    // descriptors are byte-identical to a kept copy (the same problems on
+    // the same buffers, as in an inverse-design loop) launches on that copy
+    // instead of uploading again.  Slots are reused least recently used
+    // first; an overwrite is queued on the stream after every kernel that
+    // reads the old contents.
+    static constexpr int kDescSlots = 64;
+    static constexpr size_t kDescSlotBytes = 16384;
+    void* desc_pool = nullptr;
+    std::vector<unsigned char> desc_host[kDescSlots];
+    uint64_t desc_hash[kDescSlots] = {};
+    uint64_t desc_used[kDescSlots] = {};
+    uint64_t desc_clock = 0;
     // per-pass timing
     bool timing = false;
     uint32_t timing_mask = 0xFFFFFFFFu;  // pass kinds that get timing brackets
