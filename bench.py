@@ -239,10 +239,31 @@ def bench_adjoint(G, W, T, torch, np, cpu=True):
                 ad.append(g[plane, iy, ix])
     f1 = time.perf_counter()
     fd, ad, fd_0 = np.array(fd), np.array(ad), np.array(fd_0)
+    # the same gradient in the complex64 form (one band, one scene of the
+    # multi-frame entry point): FP32 fields, FP64 phases and sums
+    u64 = torch.as_tensor(u[None, None]).to(torch.complex64).cuda().contiguous()
+    g64 = torch.zeros_like(grads)
+    call64 = G.adjoint_grid_gradient_frames_dev_prepared([steps], u64, d, d, [lam], cfg, td, g64)
+    st64, merit64 = call64()
+    if st64:
+        raise RuntimeError(f"complex64 adjoint failed: {st64}")
+    for _ in range(5):
+        call64()
+    torch.cuda.synchronize()
+    s0 = time.perf_counter()
+    for _ in range(k):
+        call64()
+    torch.cuda.synchronize()
+    s1 = time.perf_counter()
+    h64 = g64.cpu().numpy()
+    ad64 = np.array([h64[plane, int(kk) // n, int(kk) % n] for plane, pix in ((0, px1), (1, px2)) for kk in pix])
     r = {"workload": f"adjoint grid gradient {n}^2 f64, 2 DOEs + 2 ASM {z * 1e3:.0f} mm, MSE to a 16-spot target",
          "gradients_per_s": k / (t1 - t0), "ms_per_gradient": 1e3 * (t1 - t0) / k, "merit": merit,
          "call": "C ABI, arguments marshalled once",
          "python_wrapper_gradients_per_s": 1.0 / wrapped_s,
+         "c64": {"gradients_per_s": k / (s1 - s0), "ms_per_gradient": 1e3 * (s1 - s0) / k, "merit": merit64,
+                 "rel_l2_vs_c128_gradient": float(np.linalg.norm(h64 - g) / np.linalg.norm(g)),
+                 "fd_rel_l2": float(np.linalg.norm(fd_0 - ad64) / np.linalg.norm(fd_0))},
          "fd_step_rad": h0, "fd_pixels": int(fd.size),
          "fd_rel_l2": float(np.linalg.norm(fd_0 - ad) / np.linalg.norm(fd_0)),
          "fd_max_abs_over_max": float(np.max(np.abs(fd_0 - ad)) / np.max(np.abs(fd_0))),
@@ -263,7 +284,8 @@ def bench_adjoint(G, W, T, torch, np, cpu=True):
         r["vs_f64_oracle_rel_l2"] = [float(np.linalg.norm(g[e] - ref[e]) / np.linalg.norm(ref[e])) for e in range(2)]
         r["cpu_baseline"] = {"gradients_per_s": 1.0 / (c1 - c0), "cores": 1, "kind": "port",
                              "sample": f"one gradient (trace + push_back) with the f64 oracle, {c1 - c0:.1f} s"}
-    log(f"adjoint: {r['gradients_per_s']:.1f} gradients/s (C ABI; {r['python_wrapper_gradients_per_s']:.1f}/s through the Python wrapper), FD rel L2 {r['fd_rel_l2']:.2e}"
+    log(f"adjoint: {r['gradients_per_s']:.1f} gradients/s (C ABI; {r['python_wrapper_gradients_per_s']:.1f}/s through the Python wrapper), FD rel L2 {r['fd_rel_l2']:.2e}; "
+        f"complex64 {r['c64']['gradients_per_s']:.1f}/s, rel L2 {r['c64']['rel_l2_vs_c128_gradient']:.1e} vs complex128, FD {r['c64']['fd_rel_l2']:.1e}"
         + (f", vs oracle {r['vs_f64_oracle_rel_l2']}" if cpu else ""))
     return r
 

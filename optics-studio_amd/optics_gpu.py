@@ -836,6 +836,35 @@ def adjoint_grid_gradient_frames_dev(designs, inputs_dev, dx, dy, bands, asm_cfg
     return st, merit.value
 
 
+def adjoint_grid_gradient_frames_dev_prepared(designs, inputs_dev, dx, dy, bands, asm_cfg, target_dev, grads_dev,
+                                              ctx=None):
+    """adjoint_grid_gradient_frames_dev with its arguments marshalled once:
+    returns a function that runs the C entry point and returns (status,
+    merit).  No torch synchronisation per call."""
+    torch = _torch()
+    ctx = ctx or context()
+    nb, ns, ny, nx = inputs_dev.shape
+    if not inputs_dev.is_contiguous() or target_dev.dtype != torch.float64 or not target_dev.is_contiguous():
+        raise ValueError("inputs_dev and target_dev must be contiguous (target float64)")
+    fn = {torch.complex128: lib.oc_adjoint_grid_gradient_frames_dev_c128,
+          torch.complex64: lib.oc_adjoint_grid_gradient_frames_dev_c64}[inputs_dev.dtype]
+    nsteps = len(designs[0])
+    table = _design_table([[steps] for steps in designs])
+    b = np.ascontiguousarray(bands, np.float64)
+    g = T.geom(nx, ny, dx, dy, 0.0)
+    merit = c_f64()
+    args = (ctx.h, C.byref(g), b.ctypes.data, nb, ns, table, nsteps, C.byref(asm_cfg), inputs_dev.data_ptr(),
+            target_dev.data_ptr(), -1, -1, grads_dev.data_ptr(), C.byref(merit))
+    keep = (table, b, g, merit, asm_cfg, inputs_dev, target_dev, grads_dev)
+
+    def call():
+        _ = keep
+        st = fn(*args)
+        return st, merit.value
+
+    return call
+
+
 def _design_table(designs):
     """designs[band][variant] = [(mask struct or PhaseMask or None, z_after, medium), ...]
     -> flat DesignStep array [band][variant][nsteps]."""
