@@ -121,7 +121,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
     TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && (PK == PRE_NONE || PK == PRE_TMUL)> tk;
-    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
+    tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         int pi, lt;
         if (PK == PRE_TMUL && probs[0].ilv) {
@@ -218,7 +218,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     // the table (not the hoist: 138 VGPRs or a spill) at N = 4096
     // (row pass 0.456 -> 0.435 ms on the 16-band 2048^2 batch)
     TwKeep<R, N, N == 4096 && sizeof(R) == 4> tk;
-    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
+    tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     const bool ilv = probs[0].ilv != 0;
     auto locate = [&](int tile, int& pi, int& iy) {
         int lt;
@@ -455,7 +455,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
     TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && QK == POST_NONE> tk;
-    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
+    tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
         const int iy = (tile - P.base_row) * G + g;
@@ -545,7 +545,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
     TwKeep<R, N> tk;
-    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
+    tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
         const int iy = (tile - P.base_row) * G + g;
@@ -644,7 +644,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
     TwKeep<R, N, N == 4096 && sizeof(R) == 4> tk;
-    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
+    tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     constexpr int KR = N / 2 + 1;  // distinct kernel rows
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_KGEN>(probs, nprob, tile)];
@@ -812,7 +812,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
     TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4> tk;
-    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
+    tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
         const int iy = (tile - P.base_row) * G + g;
@@ -879,7 +879,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_lin
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
     TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && PK == PRE_NONE> tk;
-    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(A[0].prow.tw), t);
+    tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(A[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const int pi = find_problem<PASS_ROW>(Bp, nprob, tile);
         const ProbDev& P = A[pi];
@@ -957,7 +957,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_cha
     // table and hoist in the half-window form at N = 4096 (117 VGPRs; the
     // pass went from 0.72 to 0.62 ms on the 16-band 2048^2 chain)
     TwKeep<R, N, HW && N == 4096, HW && N == 4096> tk;
-    tk.template init<G * T>(lds + G * Pow2Plan<N>::RSTRIDE, static_cast<const cpx<R>*>(A[0].prow.tw), t);
+    tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(A[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const int pi = find_problem<PASS_ROW>(F, nprob, tile);  // F's row tiles (this kernel's G)
         const ProbDev& P = A[pi];
@@ -1100,7 +1100,7 @@ static hipError_t launch_nv(const FastSel& s, const ProbDev* d, int np, int tota
     using LY = FastLayout<R, N, V>;
     constexpr int T = N / 16;
     // exchange buffer, then the kept twiddle table
-    const size_t row_lds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
+    const size_t row_lds = (size_t)LY::G * RowStride<N, LY::G>::value * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
     const size_t col_lds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
     const dim3 brow(LY::G * T), bcol(LY::CB * T), g(grid);
     if (s.sep) {  // separable complex64 Fresnel (pass 1: columns, pass 2: rows)
@@ -1194,7 +1194,7 @@ static hipError_t launch_chain_nv(int which, const ProbDev* a, const ProbDev* f,
         using LY = FastLayout<R, N, V>;
         constexpr int T = N / 16;
         const dim3 g(grid);
-        const size_t rlds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
+        const size_t rlds = (size_t)LY::G * RowStride<N, LY::G>::value * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
         const size_t clds = (size_t)LY::CB * Pow2Plan<N>::SLOTS * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
         switch (which) {  // + 2: every problem is N = P/2
             case 0: hipLaunchKernelGGL((f_row_chain<R, N, V>), g, dim3(LY::G * T), rlds, st, a, f, np, total); break;
@@ -1221,7 +1221,7 @@ static hipError_t launch_link_nv(int pk, const ProbDev* a, const ProbDev* b, int
     using R = Real;
     using LY = FastLayout<R, N, V>;
     constexpr int T = N / 16;
-    const size_t lds = (size_t)LY::G * Pow2Plan<N>::RSTRIDE * sizeof(cpx<R>);
+    const size_t lds = (size_t)LY::G * RowStride<N, LY::G>::value * sizeof(cpx<R>) + tw_keep_bytes<R, N>(true);
     const dim3 g(grid), blk(LY::G * T);
     switch (pk) {
         case PRE_NONE: hipLaunchKernelGGL((f_row_link<R, N, V, PRE_NONE>), g, blk, lds, st, a, b, np, total); break;

@@ -28,12 +28,26 @@ struct Pow2Plan {
     static constexpr int T = N / 16;
     static constexpr int NSTAGES = NS16 + (L > 1 ? 1 : 0);
     static constexpr int SLOTS = N + N / 16;  // LDS elements per transform
-    // Row-major exchange layout (transform g at g * RSTRIDE): the stride is
-    // SLOTS rounded up to 8 mod 32 elements, which puts consecutive
-    // transforms 64 bytes (complex64) apart in the bank space, so the row
-    // kernels' lanes (running along the transform first) hit distinct banks.
+    // Row-major exchange stride (transform g at g * stride) for workgroups of
+    // four or more transforms: SLOTS rounded up to 8 mod 32 elements, which
+    // puts consecutive transforms 64 bytes (complex64) apart in the bank
+    // space, so the row kernels' lanes (running along the transform first)
+    // hit distinct banks.  RowStride picks the stride per workgroup size.
     static constexpr int RSTRIDE = SLOTS + (40 - SLOTS % 32) % 32;
     static_assert((1 << LOG) == N && N >= 32, "power of two >= 32");
+};
+
+// The row layout's stride for a workgroup of G transforms.  The row
+// kernels' lanes run (tb, g, tq): a half-wave of 32 lanes holds 16 slots of
+// each of two transforms when G = 2, and 8 of each of four when G >= 4.  Two
+// transforms 128 bytes apart in the bank space (stride 16 mod 32 elements)
+// then use disjoint banks, as four 64 bytes apart (8 mod 32) do; the other
+// choice gives 2-way conflicts (measured on the 4096 row passes: G = 2 runs
+// 2% faster with 16 mod 32, G = 4 9% slower).
+template <int N, int G>
+struct RowStride {
+    static constexpr int S = Pow2Plan<N>::SLOTS;
+    static constexpr int value = G >= 4 ? Pow2Plan<N>::RSTRIDE : S + (48 - S % 32) % 32;
 };
 
 __device__ __forceinline__ int fslot(int idx) { return idx + (idx >> 4); }
@@ -45,14 +59,14 @@ __device__ __forceinline__ int fslot(int idx) { return idx + (idx >> 4); }
 // multiple of 16, slot(b + r Ns) = slot(b) + r (Ns + Ns/16); likewise
 // slot(t + m T) = slot(t) + m (T + T/16) when 16 | T.  The accesses then use
 // immediate offsets from a single address register instead of sixteen.
-// RM: transform g's slots are contiguous at g * RSTRIDE (row kernels, whose
+// RM: transform g's slots are contiguous at g * RowStride (row kernels, whose
 // lanes run along a transform first); else interleaved, slot * G + g
 // (column kernels, whose lanes run across transforms first).
 template <typename R, int N, int G, bool RM = false>
 __device__ __forceinline__ void fft_exchange16(cpx<R> (&v)[16], cpx<R>* __restrict__ lds, int g, int t, int Ns) {
     constexpr int T = N / 16;
     constexpr int SG = RM ? 1 : G;                              // element step of one slot
-    cpx<R>* __restrict__ own = lds + (RM ? g * Pow2Plan<N>::RSTRIDE : g);  // this transform's slot 0
+    cpx<R>* __restrict__ own = lds + (RM ? g * RowStride<N, G>::value : g);  // this transform's slot 0
     const int k = t & (Ns - 1);
     const int base = (t - k) * 16 + k;
     const int wstep = Ns == 1 ? 1 : Ns + Ns / 16;

@@ -222,3 +222,37 @@ def test_chain_argument_checks(G):
     assert G.lib.oc_propagate_chain_dev_c64(ctx.h, empty, None, 1) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
     assert G.lib.oc_propagate_chain_dev_c64(None, empty, empty, 1) == T.CODE["OC_ERR_INVALID_ARGUMENT"]
     ctx.close()
+
+
+@pytest.mark.parametrize("second_mask", [False, True])
+def test_linked_asm_chain_4096_equals_two_batches(G, second_mask):
+    """complex64 ASM into ASM at P = 4096: the steps run linked through their
+    rows (f_row_link, which keeps the second radix-16 stage's twiddles in
+    LDS beside its exchange buffer); bit for bit the two batch calls."""
+    import torch
+    n, d = 2048, 2e-6
+    lams = [480e-9, 620e-9]
+    rng = np.random.default_rng(21)
+    mask, keep = G.dev_mask(rng.uniform(0, 2 * np.pi, (n, n)), d, d)
+    mask2, keep2 = G.dev_mask(rng.uniform(0, 2 * np.pi, (n, n)), d, d)
+    m2 = mask2 if second_mask else None
+    u = [torch.as_tensor(field(n, 50 + b)).to(torch.complex64).cuda() for b in range(len(lams))]
+    ctx = G.Context(0)
+    z1, z2 = 8e-3, 12e-3
+    mid_ref = [torch.empty_like(x) for x in u]
+    out_ref = [torch.empty_like(x) for x in u]
+    assert G.propagate_batch_dev(descs(T.ENGINE_ASM, lams, n, d, u, mid_ref, z1, mask), ctx=ctx) == 0
+    assert G.propagate_batch_dev(descs(T.ENGINE_ASM, lams, n, d, mid_ref, out_ref, z2, m2), ctx=ctx) == 0
+    mid = [torch.empty_like(x) for x in u]
+    out = [torch.empty_like(x) for x in u]
+    ctx.enable_timing(True)
+    ctx.pass_timing(reset=True)
+    first = descs(T.ENGINE_ASM, lams, n, d, u, mid, z1, mask)
+    second = descs(T.ENGINE_ASM, lams, n, d, None, out, z2, m2)
+    assert G.propagate_chain_dev(first, second, ctx=ctx) == 0
+    tm = ctx.pass_timing()
+    say("linked ASM chain 4096", second_mask, {k: v[1] for k, v in tm.items()})
+    assert tm["row_fwd"][1] == 1  # one forward row pass: the second runs inside the link
+    for b in range(len(lams)):
+        assert torch.equal(mid[b], mid_ref[b]) and torch.equal(out[b], out_ref[b]), (second_mask, b)
+    ctx.close()
