@@ -11,8 +11,10 @@ Fresnel propagations of their results, as one chained call
 (oc_propagate_chain_dev_c64: every problem of a pass in one persistent
 launch; the ASM's inverse row pass and the Fresnel's row pass fused, the ASM
 field kept on chip).  value = propagations/s (32 per step), inputs resident
-in HBM.  The same chain with the ASM field written out, and the two stages
-as two batch calls, are reported beside it.
+in HBM.  The same steps queued without waiting
+(oc_propagate_chain_submit_dev_c64, one oc_context_wait), the chain with the
+ASM field written out, and the two stages as two batch calls are reported
+beside it.
 
 Also reported: per-pass kernel times from HIP events on the library stream
 (roofline of the dominant kernel), the complex128 form, the f64 host drop-in
@@ -613,6 +615,18 @@ def main():
         if st:
             raise RuntimeError(f"chain failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
 
+    def chain_submit(first, second):
+        # queued without waiting: the next step's host work overlaps this
+        # step's kernels; the statuses of every queued step come at the wait
+        st = G.propagate_chain_submit_dev(first, second, ctx=ctx, sync_torch=False)
+        if st:
+            raise RuntimeError(f"chain failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
+
+    def chain_wait():
+        st = G.wait_dev(ctx)
+        if st:
+            raise RuntimeError(f"queued chains failed: {T.STATUS.get(st, st)} {ctx.last_error()}")
+
     def step(da, df, dbl=False):
         # both batches queued, one wait: the Fresnel batch reads the ASM
         # batch's outputs and runs after it on the context stream
@@ -654,6 +668,14 @@ def main():
     t1 = time.perf_counter()
     ctx.enable_timing(False)
     timing_col = ctx.pass_timing(reset=True)
+    # the same steps queued with one wait (oc_propagate_chain_submit_dev_c64)
+    torch.cuda.synchronize()
+    y0 = time.perf_counter()
+    for _ in range(args.steps):
+        chain_submit(ch_first, ch_second)
+    chain_wait()
+    torch.cuda.synchronize()
+    queued_ms_step = 1e3 * (time.perf_counter() - y0) / args.steps
     # per-pass breakdown: a few more steps with every pass bracketed (untimed)
     ctx.set_timing_passes(None)
     ctx.enable_timing(True)
@@ -667,7 +689,8 @@ def main():
     props = 2 * nb * args.steps
     value = props / elapsed
     ms_step = 1e3 * elapsed / args.steps
-    log(f"c64: {ms_step:.3f} ms/step, {value:.1f} propagations/s; passes {timing}")
+    log(f"c64: {ms_step:.3f} ms/step, {value:.1f} propagations/s ({queued_ms_step:.3f} ms/step with the steps "
+        f"queued); passes {timing}")
 
     # Algorithmic bytes per launch (minimal-sweep model, complex64 = 8 B),
     # per band, 16 bands per launch.  The chained step: row_fwd reads N^2
@@ -706,6 +729,8 @@ def main():
     # complex64 transmission once, and row_fwd reads it once (L2 across bands)
     step_bytes = sum(alg[k] * timing[k][1] for k in alg) / nbreak + 3 * n * n * 8
     extra = {"passes": passes, "fft_variant": variant, "autotune_ms": variant_ms,
+             "queued_steps_ms_per_step": queued_ms_step,
+             "queued_steps_propagations_per_s": 2 * nb / (queued_ms_step * 1e-3),
              "fresnel_sep_layout": list(sep_layout), "step_alg_GB": step_bytes / 1e9,
              "device_copy_GBps": copy_gbps,
              "step_alg_GBps": step_bytes / (elapsed / args.steps) / 1e9}

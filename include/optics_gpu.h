@@ -524,6 +524,20 @@ int oc_propagate_chain_dev_c64(oc_context* ctx, oc_prop_desc* first,
 int oc_propagate_chain_dev_c128(oc_context* ctx, oc_prop_desc* first,
                                 oc_prop_desc* second, size_t count);
 
+/* The chain without waiting (no counterpart in the reference: the
+   pipelining form for serving loops and throughput runs).  A chain that
+   runs fused or linked is queued on the context stream with its
+   finiteness flags in a buffer of its own: host-side checks fill the
+   statuses now, and oc_context_wait fills the device-side ones (a failed
+   first stage fails the chain) into first[] and second[], which must stay
+   valid until then.  A chain that runs as two batches waits between them,
+   so its statuses are final when the call returns.  Queued work runs in
+   submit order.  Returns the first host-detected status or OC_OK. */
+int oc_propagate_chain_submit_dev_c64(oc_context* ctx, oc_prop_desc* first,
+                                      oc_prop_desc* second, size_t count);
+int oc_propagate_chain_submit_dev_c128(oc_context* ctx, oc_prop_desc* first,
+                                       oc_prop_desc* second, size_t count);
+
 /* oc_propagate_batch_dev_c64 without device allocation.  No counterpart in
    the reference: it is for callers that cannot allocate (code under stream
    capture, tight serving loops, harnesses that forbid hipMalloc).  Every

@@ -400,7 +400,10 @@ static int submit(oc_context* ctx, bool dbl, oc_prop_desc* d, size_t count) {
 // the blocked intermediate (f_row_chain, f_sep_col_fin).  Three-pass chains
 // (complex128, or complex64 ASM into ASM) at one power-of-two length run
 // linked through their rows (f_row_link); all others as two batches.
-static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, size_t count) {
+// async: the submit form.  A fused or linked chain is queued with its flags
+// in a buffer of its own and resolved by oc_context_wait; a chain that runs
+// as two batches waits between them, so its statuses are final on return.
+static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, size_t count, bool async = false) {
     if (!ctx || ((!a || !f) && count)) return OC_ERR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     if (count == 0) return OC_OK;
@@ -428,9 +431,31 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
                y.nyp == x.nyp && x.nxp == qa[0].nxp && ocl::fast_variants((int)x.nxp) > 0;
     }
     void* fl;
-    st = ensure_buf(ctx, "chain_flags", sizeof(int) * 2 * count, &fl);
+    QueuedSet* qs = async ? &queued_set(ctx) : nullptr;
+    st = ensure_buf(ctx, async ? "queued_flags_" + std::to_string(qs->batches.size()) : std::string("chain_flags"),
+                    sizeof(int) * 2 * count, &fl);
     if (st) return st;
     int* dflags = static_cast<int*>(fl);
+    // the submit form: queue the one-launch chain (all = first stages, then
+    // second stages), host-detected statuses now, the rest at the wait
+    auto enqueue = [&](std::vector<Prob>& all) -> int {
+        Queued q;
+        q.probs = all;
+        q.dflags = dflags;
+        q.descs = a;
+        q.second = f;
+        qs->batches.push_back(std::move(q));
+        int first = OC_OK;
+        for (int i = 0; i < np; ++i) {
+            const int sa = all[i].status;
+            const int sf = sa != OC_OK ? sa : all[np + i].status;
+            a[i].status = sa;
+            f[i].status = sf;
+            if (first == OC_OK && sa != OC_OK) first = sa;
+            if (first == OC_OK && sf != OC_OK) first = sf;
+        }
+        return first;
+    };
     // per-problem statuses of a one-launch chain (all = first stages, then
     // second stages, flags at dflags): a failed first stage fails the chain
     auto resolve = [&](std::vector<Prob>& all) -> int {
@@ -483,7 +508,7 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
             if (e != hipSuccess) return fail(ctx, OC_ERR_DEVICE, std::string("chain launch: ") + hipGetErrorString(e));
             std::vector<Prob> all = qa;
             all.insert(all.end(), qf.begin(), qf.end());
-            return resolve(all);
+            return async ? enqueue(all) : resolve(all);
         }
         // the batches were only built (and any shared transmission computed): fall through
     }
@@ -497,7 +522,7 @@ static int chain(oc_context* ctx, bool dbl, oc_prop_desc* a, oc_prop_desc* f, si
         if (sl == 0) {
             std::vector<Prob> all = done[0];
             all.insert(all.end(), done[1].begin(), done[1].end());
-            return resolve(all);
+            return async ? enqueue(all) : resolve(all);
         }
     }
     // two batches: the first stage into the caller's buffer or a scratch one
@@ -556,6 +581,14 @@ int oc_propagate_chain_dev_c128(oc_context* ctx, oc_prop_desc* first, oc_prop_de
     OC_CONTEXT_LOCK(ctx);
     return chain(ctx, true, first, second, count);
 }
+int oc_propagate_chain_submit_dev_c64(oc_context* ctx, oc_prop_desc* first, oc_prop_desc* second, size_t count) {
+    OC_CONTEXT_LOCK(ctx);
+    return chain(ctx, false, first, second, count, true);
+}
+int oc_propagate_chain_submit_dev_c128(oc_context* ctx, oc_prop_desc* first, oc_prop_desc* second, size_t count) {
+    OC_CONTEXT_LOCK(ctx);
+    return chain(ctx, true, first, second, count, true);
+}
 
 int oc_propagate_batch_dev_c64(oc_context* ctx, oc_prop_desc* d, size_t count) {
     OC_CONTEXT_LOCK(ctx);
@@ -583,6 +616,19 @@ int oc_context_wait(oc_context* ctx) {
     int first = OC_OK;
     for (Queued& q : qs.batches) {
         const int st = finish_problems(ctx, q.probs, q.dflags);
+        if (q.second) {  // a chain: a failed first stage fails the chain
+            const size_t np = q.probs.size() / 2;
+            for (size_t i = 0; i < np; ++i) {
+                const int sa = q.probs[i].status;
+                const int sf = sa != OC_OK ? sa : q.probs[np + i].status;
+                q.descs[i].status = sa;
+                q.second[i].status = sf;
+                if (first == OC_OK && sa != OC_OK) first = sa;
+                if (first == OC_OK && sf != OC_OK) first = sf;
+            }
+            if (first == OC_OK && st != OC_OK) first = st;
+            continue;
+        }
         for (size_t i = 0; i < q.probs.size(); ++i) q.descs[i].status = q.probs[i].status;
         if (first == OC_OK && st != OC_OK) first = st;
     }

@@ -363,6 +363,10 @@ struct Queued {
     std::vector<Prob> probs;
     int* dflags = nullptr;
     oc_prop_desc* descs = nullptr;
+    // a submitted chain: probs holds the first stages then the second
+    // stages; the statuses go to descs (first) and second (a failed first
+    // stage fails the chain)
+    oc_prop_desc* second = nullptr;
 };
 
 struct QueuedSet {

@@ -109,6 +109,8 @@ EXPORTS = {
     "oc_adam_step_dev": ([vp, c_sz, vp, vp, vp, vp, vp, vp, C.c_uint64, P(T.AdamConfig)], C.c_int),
     "oc_propagate_chain_dev_c64": ([vp, P(T.PropDesc), P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_chain_dev_c128": ([vp, P(T.PropDesc), P(T.PropDesc), c_sz], C.c_int),
+    "oc_propagate_chain_submit_dev_c64": ([vp, P(T.PropDesc), P(T.PropDesc), c_sz], C.c_int),
+    "oc_propagate_chain_submit_dev_c128": ([vp, P(T.PropDesc), P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_batch_noalloc_dev_c64": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_batch_submit_dev_c64": ([vp, P(T.PropDesc), c_sz], C.c_int),
     "oc_propagate_batch_submit_dev_c128": ([vp, P(T.PropDesc), c_sz], C.c_int),
@@ -544,6 +546,19 @@ def propagate_chain_dev(first, second, double=False, ctx=None):
     if len(first) != len(second):
         raise ValueError("first and second must have one length")
     fn = lib.oc_propagate_chain_dev_c128 if double else lib.oc_propagate_chain_dev_c64
+    return fn(ctx.h, first, second, len(first))
+
+
+def propagate_chain_submit_dev(first, second, double=False, ctx=None, sync_torch=True):
+    """Queues a chain without waiting (first and second must stay alive
+    until wait_dev); returns the first host-detected status.  sync_torch:
+    synchronise torch's stream first (the caller's inputs written by torch)."""
+    ctx = ctx or context()
+    if sync_torch:
+        _torch().cuda.synchronize()
+    if len(first) != len(second):
+        raise ValueError("first and second must have one length")
+    fn = lib.oc_propagate_chain_submit_dev_c128 if double else lib.oc_propagate_chain_submit_dev_c64
     return fn(ctx.h, first, second, len(first))
 
 

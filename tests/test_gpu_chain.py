@@ -16,6 +16,7 @@ import oc_types as T
 pytestmark = pytest.mark.gpu
 
 C64_TOL = 1e-5
+NON_FINITE = 5  # OC_NON_FINITE_SAMPLE
 
 
 def say(*a):
@@ -255,4 +256,57 @@ def test_linked_asm_chain_4096_equals_two_batches(G, second_mask):
     assert tm["row_fwd"][1] == 1  # one forward row pass: the second runs inside the link
     for b in range(len(lams)):
         assert torch.equal(mid[b], mid_ref[b]) and torch.equal(out[b], out_ref[b]), (second_mask, b)
+    ctx.close()
+
+
+@pytest.mark.parametrize("case", ["fused", "linked_c128", "two_batches"])
+def test_submitted_chains_equal_synchronous(G, case):
+    """oc_propagate_chain_submit_dev_*: chains queued without waiting give
+    the synchronous chain's outputs bit for bit; statuses after the wait.
+    A non-finite input sample fails that problem's chain (both stages) at
+    the wait and leaves the others OK."""
+    import torch
+    n = 777 if case == "two_batches" else 512
+    d = 3e-6
+    lams = [480e-9, 560e-9, 640e-9]
+    dbl = case == "linked_c128"
+    cdt = torch.complex128 if dbl else torch.complex64
+    rng = np.random.default_rng(31)
+    mask, keep = G.dev_mask(rng.uniform(0, 2 * np.pi, (n, n)), d, d)
+    ctx = G.Context(0)
+    z1, z2 = 4e-3, 0.04
+    reps = 3
+    ins = [[torch.as_tensor(field(n, 60 + 10 * r + b)).to(cdt).cuda() for b in range(len(lams))]
+           for r in range(reps)]
+    ins[1][1][n // 2, n // 3] = float("nan")
+    refs, stats = [], []
+    for r in range(reps):
+        out = [torch.empty_like(x) for x in ins[r]]
+        first = descs(T.ENGINE_ASM, lams, n, d, ins[r], None, z1, mask)
+        second = descs(T.ENGINE_FRESNEL, lams, n, d, None, out, z2)
+        G.propagate_chain_dev(first, second, double=dbl, ctx=ctx)
+        refs.append(out)
+        stats.append(([first[i].status for i in range(len(lams))], [second[i].status for i in range(len(lams))]))
+    say("synchronous statuses", case, stats)
+    assert stats[1][0][1] == NON_FINITE and stats[1][1][1] == NON_FINITE
+    outs, keep_descs = [], []
+    torch.cuda.synchronize()
+    for r in range(reps):
+        out = [torch.empty_like(x) for x in ins[r]]
+        first = descs(T.ENGINE_ASM, lams, n, d, ins[r], None, z1, mask)
+        second = descs(T.ENGINE_FRESNEL, lams, n, d, None, out, z2)
+        st = G.propagate_chain_submit_dev(first, second, double=dbl, ctx=ctx, sync_torch=False)
+        assert st == 0 or case == "two_batches", (case, r, st)
+        outs.append(out)
+        keep_descs.append((first, second))
+    st = G.wait_dev(ctx=ctx)
+    # a chain that runs as two batches (odd size) had its statuses at the submit
+    assert st == (0 if case == "two_batches" else NON_FINITE), st
+    for r in range(reps):
+        first, second = keep_descs[r]
+        got = ([first[i].status for i in range(len(lams))], [second[i].status for i in range(len(lams))])
+        assert got == stats[r], (case, r, got, stats[r])
+        for b in range(len(lams)):
+            if stats[r][1][b] == 0:
+                assert torch.equal(outs[r][b], refs[r][b]), (case, r, b)
     ctx.close()
