@@ -31,19 +31,19 @@ struct Pow2Plan {
     // Row-major exchange stride (transform g at g * stride) for workgroups of
     // four or more transforms: SLOTS rounded up to 8 mod 32 elements, which
     // puts consecutive transforms 64 bytes (complex64) apart in the bank
-    // space, so the row kernels' lanes (running along the transform first)
-    // hit distinct banks.  RowStride picks the stride per workgroup size.
+    // space.  RowStride picks the stride per workgroup size.
     static constexpr int RSTRIDE = SLOTS + (40 - SLOTS % 32) % 32;
     static_assert((1 << LOG) == N && N >= 32, "power of two >= 32");
 };
 
-// The row layout's stride for a workgroup of G transforms.  The row
-// kernels' lanes run (tb, g, tq): a half-wave of 32 lanes holds 16 slots of
-// each of two transforms when G = 2, and 8 of each of four when G >= 4.  Two
-// transforms 128 bytes apart in the bank space (stride 16 mod 32 elements)
-// then use disjoint banks, as four 64 bytes apart (8 mod 32) do; the other
-// choice gives 2-way conflicts (measured on the 4096 row passes: G = 2 runs
-// 2% faster with 16 mod 32, G = 4 9% slower).
+// The row layout's stride for a workgroup of G transforms (the row kernels'
+// lanes run (tb, g, tq), so a wave holds a few slots of each transform).
+// Chosen by measurement on the complex64 4096 row passes over the residues
+// 4..28 mod 32: 8 mod 32 for G >= 4 (0.443-0.448 ms against 0.455-0.458 for the
+// others) and 16 mod 32 for G = 2 (0.607 against 0.611-0.64 ms).  The LDS
+// bank-conflict counter does not track these times: the 16 mod 32 choice
+// for G = 2 counts twice the conflict cycles of 8 mod 32, and the 4 mod 32
+// choice for G = 4 half those of 8 mod 32.
 template <int N, int G>
 struct RowStride {
     static constexpr int S = Pow2Plan<N>::SLOTS;
