@@ -608,12 +608,16 @@ int adjoint_impl(oc_context* ctx, bool dbl, const oc_band_geom& g, const AdjFram
         }
     // the context's cached buffers (a gradient per call in an inverse loop
     // must not pay an allocation and a free)
-    void *cur, *nxt, *ckpt = nullptr, *sums, *flags, *slab = nullptr;
+    void *cur, *nxt, *ckpt = nullptr, *sums, *flags, *slab = nullptr, *fsbuf;
+    // the flags, then the merit sums, in one buffer, so that one copy brings
+    // back the flags and the merit
+    const size_t flag_bytes = (sizeof(int) * (2 * nsteps + 1) * F + 63) / 64 * 64;
     int st;
     if ((st = ensure_buf(ctx, "adj_cur", F * fb, &cur)) || (st = ensure_buf(ctx, "adj_nxt", F * fb, &nxt)) ||
-        (st = ensure_buf(ctx, "adj_sums", sizeof(double) * (8 + ocd::kSeedParts), &sums)) ||
-        (st = ensure_buf(ctx, "adj_flags", sizeof(int) * (2 * nsteps + 1) * F, &flags)))
+        (st = ensure_buf(ctx, "adj_flags_sums", flag_bytes + sizeof(double) * (8 + ocd::kSeedParts), &fsbuf)))
         return st;
+    flags = fsbuf;
+    sums = static_cast<unsigned char*>(fsbuf) + flag_bytes;
     if (ne && (st = ensure_buf(ctx, "adj_ckpt", ne * F * fb, &ckpt))) return st;
     if (ne && grads && F > 1 && (st = ensure_buf(ctx, "adj_slab", sizeof(double) * F * ne * n, &slab))) return st;
     // A field set: frame f's field at base + f * fb, or the inputs (base null).
@@ -721,16 +725,16 @@ int adjoint_impl(oc_context* ctx, bool dbl, const oc_band_geom& g, const AdjFram
         HIP_TRY(ctx, ocl::launch_seed_frames(dbl, det, target, co, n, (int)fr.nb, (int)fr.ns, fr.sel_band,
                                              fr.sel_scene, static_cast<double*>(sums), ctx->stream));
     }
-    // the merit sums and the flags come back through the context's pinned
-    // read-back buffer: two asynchronous copies and one wait
+    // the flags and the merit sums come back through the context's pinned
+    // read-back buffer: one asynchronous copy of the flags and the first
+    // four sums (adjacent in the device buffer), and one wait
     auto finish = [&]() -> int {
         const size_t nf = queued.size() * F;
         void* rb = nullptr;
-        if (int s2 = ensure_readback(ctx, 32 + sizeof(int) * nf, &rb)) return s2;
-        double* hs = static_cast<double*>(rb);
-        int* hf = reinterpret_cast<int*>(static_cast<unsigned char*>(rb) + 32);
-        HIP_TRY(ctx, hipMemcpyAsync(hs, sums, 32, hipMemcpyDeviceToHost, ctx->stream));
-        if (nf) HIP_TRY(ctx, hipMemcpyAsync(hf, dflags, sizeof(int) * nf, hipMemcpyDeviceToHost, ctx->stream));
+        if (int s2 = ensure_readback(ctx, flag_bytes + 32, &rb)) return s2;
+        const int* hf = static_cast<const int*>(rb);
+        const double* hs = reinterpret_cast<const double*>(static_cast<unsigned char*>(rb) + flag_bytes);
+        HIP_TRY(ctx, hipMemcpyAsync(rb, flags, flag_bytes + 32, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
         collect_timing(ctx);
         if (merit) *merit = hs[0] / (double)n;
