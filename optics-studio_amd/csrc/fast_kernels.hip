@@ -120,7 +120,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && (PK == PRE_NONE || PK == PRE_TMUL)> tk;
+    TwKeep<R, N, (N == 1024 || N == 2048 || N == 4096) && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && (PK == PRE_NONE || PK == PRE_TMUL)> tk;
     tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         int pi, lt;
@@ -217,7 +217,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_fwd
     const int t = tb + Q * tq;
     // the table (not the hoist: 138 VGPRs or a spill) at N = 4096
     // (row pass 0.456 -> 0.435 ms on the 16-band 2048^2 batch)
-    TwKeep<R, N, N == 4096 && sizeof(R) == 4> tk;
+    TwKeep<R, N, (N == 1024 || N == 2048 || N == 4096) && sizeof(R) == 4> tk;
     tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     const bool ilv = probs[0].ilv != 0;
     auto locate = [&](int tile, int& pi, int& iy) {
@@ -305,10 +305,15 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
     // layouts (V2, V3) has the registers to keep its twiddles (table and
     // hoist) within 128 VGPRs; that took the pass from 0.87 to 0.76 ms on
     // MI355X.  The half-window convolution form (RS) has them in V1-V3.
-    // Elsewhere the kept factors pushed the kernel past 128 VGPRs.
+    // Elsewhere at 4096 the kept factors pushed the kernel past 128 VGPRs.
     constexpr bool KEEP = HW && sizeof(R) == 4 && N == 4096 &&
                           ((!RS && EK == ENG_ASM && (V == 2 || V == 3)) || (RS && V != 0));
-    TwKeep<R, N, KEEP, KEEP> tk;
+    // complex64 at 1024 and 2048 (two radix-16 stages: the table only)
+    // keeps it in every form; the table replaces the factor products, so
+    // these kernels use 2-10 fewer VGPRs with it (the GS projection pass at
+    // 1024: 6% fewer VALU instructions per tile).
+    constexpr bool KEEP_TAB = KEEP || (sizeof(R) == 4 && (N == 1024 || N == 2048));
+    TwKeep<R, N, KEEP_TAB, KEEP> tk;
     tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
@@ -454,7 +459,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_inv
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && QK == POST_NONE> tk;
+    TwKeep<R, N, (N == 1024 || N == 2048 || N == 4096) && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && QK == POST_NONE> tk;
     tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
@@ -544,7 +549,10 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N> tk;
+    // complex64 at 1024 keeps the second stage's factors in the LDS table
+    // (103 VGPRs; the FFT-form GS loop at 1024^2 went from 103k to 106k
+    // restart-iterations/s on MI355X)
+    TwKeep<R, N, sizeof(R) == 4 && N == 1024> tk;
     tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
@@ -643,7 +651,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_rs_kgen
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N, N == 4096 && sizeof(R) == 4> tk;
+    TwKeep<R, N, (N == 1024 || N == 2048 || N == 4096) && sizeof(R) == 4> tk;
     tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     constexpr int KR = N / 2 + 1;  // distinct kernel rows
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
@@ -688,7 +696,9 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_rs_kco
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x;
     const int g = tid % CB, t = tid / CB;
-    TwKeep<R, N> tk;
+    // complex64 keeps the table from 1024 and the third stage's factors at
+    // 4096 (90-110 VGPRs)
+    TwKeep<R, N, sizeof(R) == 4 && N >= 1024 && N <= 4096, sizeof(R) == 4 && N == 4096> tk;
     tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[tile / TPP];
@@ -738,7 +748,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_sep_co
     cpx<R>* lds = reinterpret_cast<cpx<R>*>(smem_raw);
     const int tid = threadIdx.x;
     const int g = tid % CB, t = tid / CB;
-    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && PK == PRE_NONE> tk;
+    TwKeep<R, N, (N == 1024 || N == 2048 || N == 4096) && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && PK == PRE_NONE> tk;
     tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
@@ -811,7 +821,7 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_sep_row
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4> tk;
+    TwKeep<R, N, (N == 1024 || N == 2048 || N == 4096) && sizeof(R) == 4, N == 4096 && sizeof(R) == 4> tk;
     tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(probs[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_ROW>(probs, nprob, tile)];
@@ -878,7 +888,10 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_lin
     const int tid = threadIdx.x;
     const int tb = tid % Q, g = (tid / Q) % G, tq = tid / (Q * G);
     const int t = tb + Q * tq;
-    TwKeep<R, N, N == 4096 && sizeof(R) == 4, N == 4096 && sizeof(R) == 4 && PK == PRE_NONE> tk;
+    // (at 2048 the 1024-thread layouts V0/V2 spill more with the table)
+    TwKeep<R, N, (N == 1024 || N == 4096 || (N == 2048 && (V == 1 || V == 3))) && sizeof(R) == 4,
+           N == 4096 && sizeof(R) == 4 && PK == PRE_NONE>
+        tk;
     tk.template init<G * T>(lds + G * RowStride<N, G>::value, static_cast<const cpx<R>*>(A[0].prow.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const int pi = find_problem<PASS_ROW>(Bp, nprob, tile);
