@@ -592,8 +592,9 @@ int oc_phasemask_apply_dev_c64(oc_context* ctx, const oc_band_geom* geom,
 
 /* GsProblem + GsConfig (gs.rs:125-305) for a GridPhase carrier with the
    MseToTarget merit (merit.rs:112-141) as the merit seam.  Host pointers.
-   init_kind: 0 = GsInit::Zero (carrier_values), 2 = GsInit::Given
-   (init_phase); GsInit::Random (rand 0.8 StdRng) is not offered.
+   init_kind: 0 = GsInit::Zero (carrier_values), 1 = GsInit::Random{seed}
+   (rand 0.8 StdRng::seed_from_u64(seed), Uniform::new(0, span), drawn on
+   the host: gs.rs:449-458), 2 = GsInit::Given (init_phase).
    variant: 0 = Classic, 1 = Dvr{beta}, 2 = Fpr{beta}. */
 typedef struct oc_gs_problem {
     oc_band_geom geom;
@@ -612,6 +613,7 @@ typedef struct oc_gs_problem {
     int32_t init_kind;
     double beta;
     const double* init_phase;       /* nx*ny when init_kind == 2 */
+    uint64_t seed;                  /* init_kind == 1 */
 } oc_gs_problem;
 
 /* GsRecord (gs.rs:254-268). */

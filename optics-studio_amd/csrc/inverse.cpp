@@ -2,6 +2,7 @@
 #include <climits>
 
 #include "runtime.hpp"
+#include "std_rng.hpp"
 
 using namespace ocr;
 
@@ -239,7 +240,7 @@ int gs_validate(const oc_gs_problem* pb) {
     if (pb->variant == 1 && !(std::isfinite(pb->beta) && pb->beta > 0.0 && pb->beta < 2.0)) return OC_INV_RENDER;
     if (pb->variant == 2 && !(std::isfinite(pb->beta) && pb->beta > 0.0 && pb->beta <= 1.0)) return OC_INV_RENDER;
     if (pb->max_iterations == 0) return OC_INV_RENDER;
-    if (pb->init_kind != 0 && pb->init_kind != 2) return OC_ERR_UNSUPPORTED;
+    if (pb->init_kind < 0 || pb->init_kind > 2) return OC_ERR_INVALID_ARGUMENT;
     if (pb->init_kind == 0 && !pb->carrier_values) return OC_ERR_INVALID_ARGUMENT;
     if (pb->init_kind == 2 && !pb->init_phase) return OC_ERR_INVALID_ARGUMENT;
     return OC_OK;
@@ -310,8 +311,13 @@ int gs_batch_impl(oc_context* ctx, const oc_gs_problem* pbs, oc_gs_solution* sol
                 open[iy * nx + ix] = contains_h(pb->aperture, x - pb->cx, y - pb->cy) ? 1 : 0;
             }
         HIP_TRY(ctx, hipMemcpy(at(d_open, r, 1), open.data(), n, hipMemcpyHostToDevice));
-        const double* src = pb->init_kind == 0 ? pb->carrier_values : pb->init_phase;
-        for (size_t k = 0; k < n; ++k) phase[k] = encode_bounds_h(src[k], pb->lo, pb->hi);
+        if (pb->init_kind == 1) {  // GsInit::Random, drawn on the host as the reference does
+            och::gs_random_phase(pb->seed, pb->lo, pb->hi, n, phase.data());
+        } else {
+            const double* src = pb->init_kind == 0 ? pb->carrier_values : pb->init_phase;
+            std::copy(src, src + n, phase.begin());
+        }
+        for (size_t k = 0; k < n; ++k) phase[k] = encode_bounds_h(phase[k], pb->lo, pb->hi);
         HIP_TRY(ctx, hipMemcpy(at(d_phase, r, 8), phase.data(), n * 8, hipMemcpyHostToDevice));
     }
     // illumination arriving at the DOE plane (gs.rs:436-444), every restart at once

@@ -199,7 +199,8 @@ class GsProblem(C.Structure):
                 ("merit_target", C.c_void_p), ("asm_cfg", AsmConfig), ("aperture", ClearAperture),
                 ("cx", c_f64), ("cy", c_f64), ("lo", c_f64), ("hi", c_f64), ("carrier_values", C.c_void_p),
                 ("max_iterations", c_u64), ("patience", c_u64), ("merit_tolerance", c_f64),
-                ("variant", c_i32), ("init_kind", c_i32), ("beta", c_f64), ("init_phase", C.c_void_p)]
+                ("variant", c_i32), ("init_kind", c_i32), ("beta", c_f64), ("init_phase", C.c_void_p),
+                ("seed", c_u64)]
 
 
 class GsRecord(C.Structure):

@@ -636,8 +636,10 @@ class DevEngine:
 
 def gs_run(u_in, dx, dy, lam, z_total, target_amplitude, merit_target, asm_cfg, aperture=(0, 0.0, 0.0),
            center=(0.0, 0.0), lo=-np.pi, hi=np.pi, carrier_values=None, max_iterations=100, patience=10,
-           merit_tolerance=1e-4, variant=0, beta=1.0, init_phase=None, ctx=None):
-    """gs::run (gs.rs:347) with the MseToTarget merit; same call shape as the oracle's."""
+           merit_tolerance=1e-4, variant=0, beta=1.0, init_phase=None, init_seed=None, ctx=None):
+    """gs::run (gs.rs:347) with the MseToTarget merit; same call shape as the
+    oracle's.  init_phase: GsInit::Given; init_seed: GsInit::Random{seed};
+    neither: GsInit::Zero (the carrier values)."""
     ctx = ctx or context()
     u_in = _c128(u_in)
     ny, nx = u_in.shape
@@ -648,7 +650,7 @@ def gs_run(u_in, dx, dy, lam, z_total, target_amplitude, merit_target, asm_cfg, 
     prob = T.GsProblem(T.geom(nx, ny, dx, dy, lam), u_in.ctypes.data, z_total, ta.ctypes.data, mt.ctypes.data,
                        asm_cfg, T.ClearAperture(aperture[0], 0, aperture[1], aperture[2]), center[0], center[1],
                        lo, hi, cv.ctypes.data, max_iterations, patience, merit_tolerance, variant,
-                       2 if init_phase is not None else 0, beta, ip.ctypes.data)
+                       _init_kind(init_phase, init_seed), beta, ip.ctypes.data, init_seed or 0)
     phase, ti, tae, pe, il = (np.empty((ny, nx)) for _ in range(5))
     hist = (T.GsRecord * max_iterations)()
     sol = T.GsSolution(phase.ctypes.data, 0.0, 0, 0.0, 0, 0, 0, C.cast(hist, vp), ti.ctypes.data,
@@ -664,6 +666,11 @@ def gs_run(u_in, dx, dy, lam, z_total, target_amplitude, merit_target, asm_cfg, 
                     target_intensity=ti, target_amplitude_error=tae, phase_error=pe, illumination_amplitude=il)
 
 
+def _init_kind(init_phase, init_seed):
+    """GsInit as oc_gs_problem.init_kind: Given (2), Random (1) or Zero (0)."""
+    return 2 if init_phase is not None else (1 if init_seed is not None else 0)
+
+
 def gs_run_batch(problems, graph_chunk=0, ctx=None):
     """oc_gs_run_batch: problems = list of dicts with gs_run's arguments
     (u_in, dx, dy, lam, z_total, target_amplitude, merit_target, asm_cfg and
@@ -674,7 +681,8 @@ def gs_run_batch(problems, graph_chunk=0, ctx=None):
         u_in = _c128(p["u_in"])
         ny, nx = u_in.shape
         kw = dict(aperture=(0, 0.0, 0.0), center=(0.0, 0.0), lo=-np.pi, hi=np.pi, carrier_values=None,
-                  max_iterations=100, patience=10, merit_tolerance=1e-4, variant=0, beta=1.0, init_phase=None)
+                  max_iterations=100, patience=10, merit_tolerance=1e-4, variant=0, beta=1.0, init_phase=None,
+                  init_seed=None)
         kw.update({k: v for k, v in p.items() if k in kw})
         ta = np.ascontiguousarray(p["target_amplitude"], np.float64)
         mt = np.ascontiguousarray(p["merit_target"], np.float64)
@@ -687,7 +695,8 @@ def gs_run_batch(problems, graph_chunk=0, ctx=None):
                                ta.ctypes.data, mt.ctypes.data, p["asm_cfg"], T.ClearAperture(ap[0], 0, ap[1], ap[2]),
                                kw["center"][0], kw["center"][1], kw["lo"], kw["hi"], cv.ctypes.data,
                                kw["max_iterations"], kw["patience"], kw["merit_tolerance"], kw["variant"],
-                               2 if kw["init_phase"] is not None else 0, kw["beta"], ip.ctypes.data)
+                               _init_kind(kw["init_phase"], kw["init_seed"]), kw["beta"], ip.ctypes.data,
+                               kw["init_seed"] or 0)
         arrays = [np.empty((ny, nx)) for _ in range(5)]
         hist = (T.GsRecord * kw["max_iterations"])()
         sols[i] = T.GsSolution(arrays[0].ctypes.data, 0.0, 0, 0.0, 0, 0, 0, C.cast(hist, vp), arrays[1].ctypes.data,

@@ -33,9 +33,14 @@
  *   gs run .................... convention pins (encode_bounds literals,
  *                               monotone best-so-far, single-iteration step
  *                               against a numpy restatement, top-hat
- *                               convergence).  GsInit::Random (rand 0.8
- *                               StdRng) is not restated: parity unpinned, not
- *                               offered.
+ *                               convergence).  GsInit::Random: the
+ *                               ChaCha block function is pinned by the RFC
+ *                               7539 ChaCha20 known answers; the seeding
+ *                               (PCG32), the 12-round stream order and the
+ *                               Uniform<f64> mapping have no vector here
+ *                               (rand is not vendored): parity unpinned for
+ *                               the stream, checked only for its range,
+ *                               determinism and moments.
  *   gs fft form ............... added form (no reference counterpart); pinned
  *                               by a numpy restatement of one iteration and by
  *                               monotone error decrease.
@@ -1064,6 +1069,115 @@ void or_splitmix_uniform(uint64_t seed, uint64_t count, double* out) {
     }
 }
 
+/* rand 0.8 StdRng (not vendored with the reference; Cargo.lock pins rand
+ * 0.8.8, rand_chacha 0.3.1, rand_core 0.6.4).  Restated from the published
+ * algorithms:
+ *  - ChaCha block (Bernstein; RFC 7539 section 2.3 with the original 64-bit
+ *    counter / 64-bit nonce split that rand_chacha uses): state = constants,
+ *    key words, counter lo/hi, stream lo/hi; double rounds of column then
+ *    diagonal quarter rounds; output = state after the rounds + input state;
+ *  - seed_from_u64 (rand_core 0.6): 8 PCG32 XSH-RR outputs, state advanced
+ *    first (mul 6364136223846793005, inc 11634580027462260723), stored
+ *    little-endian as the 32-byte key;
+ *  - ChaCha12Rng: blocks 0, 1, 2, ... of stream 0, 6 double rounds, words in
+ *    order; next_u64 = word[2j] | word[2j+1] << 32;
+ *  - Uniform::new(low, high) for f64: scale = high - low, decreased one ulp
+ *    at a time while scale * (1 - 2^-52) + low >= high; sample = (f64 with
+ *    exponent 0 and mantissa u64 >> 12) - 1, times scale, plus low. */
+static inline uint32_t chacha_rotl(uint32_t v, unsigned c) { return (v << c) | (v >> (32u - c)); }
+
+void or_chacha_block(const uint32_t* key, uint64_t counter, uint64_t stream, int double_rounds, uint32_t* out) {
+    uint32_t st[16];
+    st[0] = 0x61707865u;  // "expa"
+    st[1] = 0x3320646eu;  // "nd 3"
+    st[2] = 0x79622d32u;  // "2-by"
+    st[3] = 0x6b206574u;  // "te k"
+    for (int i = 0; i < 8; ++i) st[4 + i] = key[i];
+    st[12] = (uint32_t)(counter & 0xffffffffu);
+    st[13] = (uint32_t)(counter >> 32);
+    st[14] = (uint32_t)(stream & 0xffffffffu);
+    st[15] = (uint32_t)(stream >> 32);
+    uint32_t w[16];
+    for (int i = 0; i < 16; ++i) w[i] = st[i];
+    static const int QR[8][4] = {{0, 4, 8, 12}, {1, 5, 9, 13},  {2, 6, 10, 14}, {3, 7, 11, 15},
+                                 {0, 5, 10, 15}, {1, 6, 11, 12}, {2, 7, 8, 13},  {3, 4, 9, 14}};
+    for (int r = 0; r < double_rounds; ++r)
+        for (int q = 0; q < 8; ++q) {
+            uint32_t &a = w[QR[q][0]], &b = w[QR[q][1]], &c = w[QR[q][2]], &d = w[QR[q][3]];
+            a += b; d ^= a; d = chacha_rotl(d, 16);
+            c += d; b ^= c; b = chacha_rotl(b, 12);
+            a += b; d ^= a; d = chacha_rotl(d, 8);
+            c += d; b ^= c; b = chacha_rotl(b, 7);
+        }
+    for (int i = 0; i < 16; ++i) out[i] = w[i] + st[i];
+}
+
+namespace {
+struct OrStdRng {
+    uint32_t key[8];
+    uint32_t block[16];
+    uint64_t next_block = 0;
+    unsigned used = 16;
+    explicit OrStdRng(uint64_t s) {
+        const uint64_t mul = 6364136223846793005ull, inc = 11634580027462260723ull;
+        for (int i = 0; i < 8; ++i) {
+            s = s * mul + inc;
+            uint32_t xorshifted = (uint32_t)(((s >> 18) ^ s) >> 27);
+            unsigned rot = (unsigned)(s >> 59);
+            key[i] = rot ? (xorshifted >> rot) | (xorshifted << (32u - rot)) : xorshifted;
+        }
+    }
+    uint64_t u64() {
+        if (used == 16) {
+            or_chacha_block(key, next_block++, 0, 6, block);
+            used = 0;
+        }
+        uint64_t v = (uint64_t)block[used] | ((uint64_t)block[used + 1] << 32);
+        used += 2;
+        return v;
+    }
+};
+}  // namespace
+
+void or_std_rng_u64(uint64_t seed, uint64_t count, uint64_t* out) {
+    OrStdRng g(seed);
+    for (uint64_t i = 0; i < count; ++i) out[i] = g.u64();
+}
+
+void or_gs_random_phase(uint64_t seed, double lo, double hi, uint64_t n, double* out) {
+    const double TAU = 6.283185307179586;
+    double span = hi - lo;  // (hi - lo).clamp(0.0, TAU); NaN stays NaN
+    if (span < 0.0) span = 0.0;
+    if (span > TAU) span = TAU;
+    if (!(span > 0.0)) {
+        for (uint64_t k = 0; k < n; ++k) out[k] = lo;
+        return;
+    }
+    // Uniform::new(0.0, span)
+    const double low = 0.0, high = span;
+    double max_rand;
+    {
+        uint64_t b = (~0ull >> 12) | (1023ull << 52);
+        std::memcpy(&max_rand, &b, 8);
+        max_rand -= 1.0;
+    }
+    double scale = high - low;
+    while (scale * max_rand + low >= high) {
+        uint64_t b;
+        std::memcpy(&b, &scale, 8);
+        b -= 1;
+        std::memcpy(&scale, &b, 8);
+    }
+    OrStdRng g(seed);
+    for (uint64_t k = 0; k < n; ++k) {
+        uint64_t b = (g.u64() >> 12) | (1023ull << 52);
+        double v12;
+        std::memcpy(&v12, &b, 8);
+        double v01 = v12 - 1.0;
+        out[k] = lo + (v01 * scale + low);
+    }
+}
+
 /* ---------------------------------------------------------------------- */
 /* Gerchberg-Saxton (crates/optics-inverse/src/gs.rs)                      */
 /* ---------------------------------------------------------------------- */
@@ -1114,6 +1228,8 @@ int or_gs_run(const or_gs_problem* p, double* phase_out, double* merit_out, uint
         for (size_t k = 0; k < n; ++k) phase[k] = p->carrier_values[k];
     } else if (p->init_kind == 2) {
         for (size_t k = 0; k < n; ++k) phase[k] = p->init_phase[k];
+    } else if (p->init_kind == 1) {  // gs.rs:449-458
+        or_gs_random_phase(p->seed, lo, hi, n, phase.data());
     } else {
         return OC_ERR_UNSUPPORTED;
     }

@@ -95,6 +95,19 @@ int or_plane_wave(double theta_x, double theta_y, double amplitude,
    u53 * 2^-53 uniforms in [0, 1). */
 void or_splitmix_uniform(uint64_t seed, uint64_t count, double* out);
 
+/* GsInit::Random's generator (gs.rs:449-458): rand 0.8 StdRng =
+   rand_chacha 0.3 ChaCha12Rng seeded by rand_core 0.6 seed_from_u64, and
+   Uniform<f64>.  or_chacha_block is the ChaCha block function with a 64-bit
+   counter and 64-bit stream id and `double_rounds` double rounds (10 for
+   ChaCha20, 6 for the ChaCha12 of StdRng), for known-answer checks;
+   or_std_rng_u64 draws `count` next_u64 values; or_gs_random_phase fills
+   the starting phases for slot bounds lo, hi. */
+void or_chacha_block(const uint32_t* key, uint64_t counter, uint64_t stream,
+                     int double_rounds, uint32_t* out);
+void or_std_rng_u64(uint64_t seed, uint64_t count, uint64_t* out);
+void or_gs_random_phase(uint64_t seed, double lo, double hi, uint64_t n,
+                        double* out);
+
 /* optics-inverse gs.rs (ASM half-transforms; merit = MseToTarget on the
    intensity frame with `merit_target`). */
 typedef struct or_gs_problem {
@@ -113,9 +126,10 @@ typedef struct or_gs_problem {
     uint64_t max_iterations, patience;
     double merit_tolerance;
     int32_t variant;              /* 0 Classic, 1 Dvr, 2 Fpr */
-    int32_t init_kind;            /* 0 Zero, 2 Given (Random unsupported) */
+    int32_t init_kind;            /* 0 Zero, 1 Random{seed}, 2 Given */
     double beta;
     const double* init_phase;     /* n when init_kind == 2 */
+    uint64_t seed;                /* init_kind == 1 */
 } or_gs_problem;
 
 typedef struct or_gs_record {

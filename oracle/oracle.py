@@ -76,6 +76,12 @@ def _declare(L):
     L.or_gaussian.argtypes = [c_f64, c_f64, c_f64, c_f64, c_f64, c_f64, c_u64, c_u64, c_f64, c_f64, vp]
     L.or_plane_wave.argtypes = [c_f64, c_f64, c_f64, c_u64, c_u64, c_f64, c_f64, c_f64, vp]
     L.or_splitmix_uniform.argtypes = [c_u64, c_u64, vp]
+    L.or_chacha_block.argtypes = [vp, c_u64, c_u64, C.c_int, vp]
+    L.or_chacha_block.restype = None
+    L.or_std_rng_u64.argtypes = [c_u64, c_u64, vp]
+    L.or_std_rng_u64.restype = None
+    L.or_gs_random_phase.argtypes = [c_u64, c_f64, c_f64, c_u64, vp]
+    L.or_gs_random_phase.restype = None
     L.or_gs_run.argtypes = [P(GsProblem), vp, P(c_f64), P(c_u64), P(c_f64), P(c_i32), P(c_u64), vp, vp, vp,
                             vp, vp]
     L.or_gs_fft.argtypes = [c_u64, vp, vp, vp, c_u64, c_u64, vp]
@@ -293,6 +299,28 @@ def plane_wave(theta_x, theta_y, amplitude, nx, ny, dx, dy, lam):
     return out
 
 
+def chacha_block(key, counter, stream, double_rounds):
+    """The ChaCha block function (16 output words) for known-answer checks."""
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.empty(16, np.uint32)
+    lib().or_chacha_block(_ptr(k), counter, stream, double_rounds, _ptr(out))
+    return out
+
+
+def std_rng_u64(seed, count):
+    """rand 0.8 StdRng::seed_from_u64(seed): the first `count` next_u64 values."""
+    out = np.empty(count, np.uint64)
+    lib().or_std_rng_u64(seed, count, _ptr(out))
+    return out
+
+
+def gs_random_phase(seed, lo, hi, n):
+    """GsInit::Random's starting phases (gs.rs:449-458)."""
+    out = np.empty(n)
+    lib().or_gs_random_phase(seed, lo, hi, n, _ptr(out))
+    return out
+
+
 def splitmix_uniform(seed, count):
     out = np.empty(count)
     lib().or_splitmix_uniform(seed, count, _ptr(out))
@@ -301,8 +329,9 @@ def splitmix_uniform(seed, count):
 
 def gs_run(u_in, dx, dy, lam, z_total, target_amplitude, merit_target, asm_cfg, aperture=(0, 0.0, 0.0),
            center=(0.0, 0.0), lo=-np.pi, hi=np.pi, carrier_values=None, max_iterations=100, patience=10,
-           merit_tolerance=1e-4, variant=0, beta=1.0, init_phase=None):
-    """gs::run with GsInit::Given (init_phase) or GsInit::Zero (carrier values)."""
+           merit_tolerance=1e-4, variant=0, beta=1.0, init_phase=None, init_seed=None):
+    """gs::run with GsInit::Given (init_phase), GsInit::Random{seed} (init_seed)
+    or GsInit::Zero (carrier values)."""
     u_in = _c128(u_in)
     ny, nx = u_in.shape
     n = nx * ny
@@ -313,7 +342,8 @@ def gs_run(u_in, dx, dy, lam, z_total, target_amplitude, merit_target, asm_cfg, 
     prob = GsProblem(T.geom(nx, ny, dx, dy, lam), u_in.ctypes.data, z_total, ta.ctypes.data, mt.ctypes.data,
                      asm_cfg, T.ClearAperture(aperture[0], 0, aperture[1], aperture[2]), center[0], center[1],
                      lo, hi, cv.ctypes.data, max_iterations, patience, merit_tolerance, variant,
-                     2 if init_phase is not None else 0, beta, ip.ctypes.data)
+                     2 if init_phase is not None else (1 if init_seed is not None else 0), beta, ip.ctypes.data,
+                     init_seed or 0)
     phase = np.empty((ny, nx))
     merit, best_it, amse, conv, iters = c_f64(), c_u64(), c_f64(), c_i32(), c_u64()
     hist = (GsRecord * max_iterations)()
