@@ -86,12 +86,6 @@ __device__ __forceinline__ cpx<R> rot(cpx<R> a) {
     if (DIR < 0) return {a.im, -a.re};
     return {-a.im, a.re};
 }
-// a * w where w = (c, DIR*s) for a twiddle exp(DIR * i * theta) given (c, s)=(cos, sin)
-template <int DIR, typename R>
-__device__ __forceinline__ cpx<R> twmul(cpx<R> a, R c, R s) {
-    const R si = DIR < 0 ? -s : s;
-    return {fmadd(a.re, c, -(a.im * si)), fmadd(a.re, si, a.im * c)};
-}
 
 // LDS index with a one-element pad every 32 elements (spreads strided accesses
 // of the early stages over the banks).
@@ -189,47 +183,83 @@ __device__ __forceinline__ void dft7(cpx<R>* v) {
 }
 
 // Radix 8 as 2 x 4: X[q + 4s] = DFT2_r( W8^{rq} * DFT4_m(v[r + 2m]) ).
+// W8^1 = h (1 + D i) and W8^3 = h (-1 + D i) (h = cos(pi/4), D = DIR): the
+// products are formed unscaled and h is applied by the FMAs of the last
+// layer.
 template <int DIR, typename R>
 __device__ __forceinline__ void dft8(cpx<R>* v) {
+    constexpr R D = DIR < 0 ? R(-1) : R(1);
     const R h = R(0.70710678118654752440084436210485);
     dft4<DIR>(v[0], v[2], v[4], v[6]);
     dft4<DIR>(v[1], v[3], v[5], v[7]);
-    // twiddles W8^{q} on the r=1 branch, q = 1,2,3 (v[3], v[5], v[7])
-    v[3] = twmul<DIR>(v[3], h, h);
-    v[5] = rot<DIR>(v[5]);
-    v[7] = twmul<DIR>(v[7], -h, h);
+    const cpx<R> t1 = {v[3].re - D * v[3].im, v[3].im + D * v[3].re};     // v3 (1 + D i)
+    const cpx<R> t3 = {-v[7].re - D * v[7].im, D * v[7].re - v[7].im};    // v7 (-1 + D i)
+    const cpx<R> r2 = rot<DIR>(v[5]);
     cpx<R> o[8];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        cpx<R> a = v[2 * q], b = v[2 * q + 1];
-        o[q] = cadd(a, b);
-        o[q + 4] = csub(a, b);
-    }
+    o[0] = cadd(v[0], v[1]);
+    o[4] = csub(v[0], v[1]);
+    o[1] = {fmadd(h, t1.re, v[2].re), fmadd(h, t1.im, v[2].im)};
+    o[5] = {fmadd(-h, t1.re, v[2].re), fmadd(-h, t1.im, v[2].im)};
+    o[2] = cadd(v[4], r2);
+    o[6] = csub(v[4], r2);
+    o[3] = {fmadd(h, t3.re, v[6].re), fmadd(h, t3.im, v[6].im)};
+    o[7] = {fmadd(-h, t3.re, v[6].re), fmadd(-h, t3.im, v[6].im)};
 #pragma unroll
     for (int i = 0; i < 8; ++i) v[i] = o[i];
 }
 
+// The last layer of a row q of dft16p: X0 = a02 + s u, X2 = a02 - s u,
+// X1 = s02 + s (D i) w, X3 = s02 - s (D i) w, the scale s by FMA.
+template <int DIR, typename R>
+__device__ __forceinline__ void dft16_row_out(cpx<R>* o, cpx<R> a02, cpx<R> s02, cpx<R> u, cpx<R> w, R s) {
+    const cpx<R> rw = rot<DIR>(w);
+    o[0] = {fmadd(s, u.re, a02.re), fmadd(s, u.im, a02.im)};
+    o[1] = {fmadd(s, rw.re, s02.re), fmadd(s, rw.im, s02.im)};
+    o[2] = {fmadd(-s, u.re, a02.re), fmadd(-s, u.im, a02.im)};
+    o[3] = {fmadd(-s, rw.re, s02.re), fmadd(-s, rw.im, s02.im)};
+}
+
 // Radix 16 as 4 x 4: X[q + 4s] = DFT4_r( W16^{rq} * DFT4_m(v[r + 4m]) ).
+// The twiddles of the second layer are written as a real scale times a
+// factor with a unit part (c = cos(pi/8), t = tan(pi/8), h = cos(pi/4),
+// D = DIR): W16^1 = c (1 + D i t), W16^3 = c (t + D i), W16^9 = -W16^1,
+// W16^2 = h (1 + D i), W16^4 = D i, W16^6 = h (-1 + D i).  A row's two
+// entries of one scale are combined unscaled and the scale is applied by the
+// FMAs that form the row's outputs: 144 real operations per radix-16
+// butterfly instead of 160, the same transform to rounding.
 template <int DIR, typename R>
 __device__ __forceinline__ void dft16p(cpx<R>* v) {
+    constexpr R D = DIR < 0 ? R(-1) : R(1);
     const R c1 = R(0.92387953251128675612818318939679);  // cos(pi/8)
-    const R s1 = R(0.38268343236508977172845998403040);  // sin(pi/8)
+    const R tn = R(0.41421356237309504880168872420970);  // tan(pi/8)
     const R h = R(0.70710678118654752440084436210485);
 #pragma unroll
     for (int r = 0; r < 4; ++r) dft4<DIR>(v[r], v[r + 4], v[r + 8], v[r + 12]);
-    // A[r][q] now sits at v[r + 4q]; apply W16^{rq}
-    // r=1: q=1,2,3 -> e=1,2,3 ; r=2: e=2,4,6 ; r=3: e=3,6,9
-    v[1 + 4] = twmul<DIR>(v[1 + 4], c1, s1);
-    v[1 + 8] = twmul<DIR>(v[1 + 8], h, h);
-    v[1 + 12] = twmul<DIR>(v[1 + 12], s1, c1);
-    v[2 + 4] = twmul<DIR>(v[2 + 4], h, h);
-    v[2 + 8] = rot<DIR>(v[2 + 8]);
-    v[2 + 12] = twmul<DIR>(v[2 + 12], -h, h);
-    v[3 + 4] = twmul<DIR>(v[3 + 4], s1, c1);
-    v[3 + 8] = twmul<DIR>(v[3 + 8], -h, h);
-    v[3 + 12] = twmul<DIR>(v[3 + 12], -c1, -s1);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) dft4<DIR>(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    // A[r][q] now sits at v[r + 4q]; row q takes A[r][q] W16^{rq}, r = 0..3
+    dft4<DIR>(v[0], v[1], v[2], v[3]);
+    {  // q = 1: W^1 (scale c), W^2 (h), W^3 (c)
+        const cpx<R> y1 = {fmadd(-D * tn, v[5].im, v[5].re), fmadd(D * tn, v[5].re, v[5].im)};
+        const cpx<R> y3 = {fmadd(tn, v[7].re, -D * v[7].im), fmadd(tn, v[7].im, D * v[7].re)};
+        const cpx<R> t2 = {v[6].re - D * v[6].im, v[6].im + D * v[6].re};
+        const cpx<R> a02 = {fmadd(h, t2.re, v[4].re), fmadd(h, t2.im, v[4].im)};
+        const cpx<R> s02 = {fmadd(-h, t2.re, v[4].re), fmadd(-h, t2.im, v[4].im)};
+        dft16_row_out<DIR>(v + 4, a02, s02, cadd(y1, y3), csub(y1, y3), c1);
+    }
+    {  // q = 2: W^2 (h), W^4, W^6 (h)
+        const cpx<R> t1 = {v[9].re - D * v[9].im, v[9].im + D * v[9].re};
+        const cpx<R> t3 = {-v[11].re - D * v[11].im, D * v[11].re - v[11].im};
+        const cpx<R> x2 = rot<DIR>(v[10]);
+        const cpx<R> a02 = cadd(v[8], x2), s02 = csub(v[8], x2);
+        dft16_row_out<DIR>(v + 8, a02, s02, cadd(t1, t3), csub(t1, t3), h);
+    }
+    {  // q = 3: W^3 (c), W^6 (h), W^9 (c)
+        const cpx<R> y1 = {fmadd(tn, v[13].re, -D * v[13].im), fmadd(tn, v[13].im, D * v[13].re)};
+        const cpx<R> y3 = {fmadd(D * tn, v[15].im, -v[15].re), fmadd(-D * tn, v[15].re, -v[15].im)};
+        const cpx<R> t2 = {-v[14].re - D * v[14].im, D * v[14].re - v[14].im};
+        const cpx<R> a02 = {fmadd(h, t2.re, v[12].re), fmadd(h, t2.im, v[12].im)};
+        const cpx<R> s02 = {fmadd(-h, t2.re, v[12].re), fmadd(-h, t2.im, v[12].im)};
+        dft16_row_out<DIR>(v + 12, a02, s02, cadd(y1, y3), csub(y1, y3), c1);
+    }
     // X[q + 4s] now sits in v[4q + s]
 }
 
