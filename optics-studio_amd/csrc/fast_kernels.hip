@@ -567,24 +567,40 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::G * (N / 16))) void f_row_gs(
             v[m] = {R(0), R(0)};
             if (rowok) v[m] = at32(work, b0 + (unsigned)(m * (T / Q)) * ny * Q);
         }
+        const R* amp = static_cast<const R*>(P.ops.amp);
+        const unsigned char* open = P.ops.open;
+        const unsigned o0 = (unsigned)iy * nx;
+        // !FULL: the source amplitudes of the tile's samples are loaded here,
+        // behind the row, so that the inverse transform covers their latency
+        // (loaded after it, they waited behind its barriers: 0.161 ms per
+        // pass at 1024^2 x 32 restarts on MI355X before the change)
+        R av[16];
+        if constexpr (!FULL) {
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int x = t + m * T - ox;
+                av[m] = R(0);
+                if (rowok && x >= 0 && x < nx) {
+                    const unsigned k = o0 + x;
+                    R a = at32(amp, k);
+                    if (open && !at32(open, k)) a = R(0);
+                    av[m] = a;
+                }
+            }
+        }
         fft_regs<R, N, G, +1, true>(v, lds, g, t, static_cast<const cpx<R>*>(P.prow.tw), tk);
         const R s = (R)P.scale;
         bool bad = false;
         double acc2 = 0.0;
         R* ph = static_cast<R*>(P.ops.phase_io);
-        const R* amp = static_cast<const R*>(P.ops.amp);
-        const unsigned char* open = P.ops.open;
         cpx<R>* out = static_cast<cpx<R>*>(P.out);
-        const unsigned o0 = (unsigned)iy * nx;
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int x = t + m * T - ox;
             if (rowok && x >= 0 && x < nx && !FULL) {
                 const cpx<R> o = {v[m].re * s, v[m].im * s};
                 bad |= !finite_c(o);
-                const unsigned k = o0 + x;
-                R a = at32(amp, k);
-                if (open && !at32(open, k)) a = R(0);
+                const R a = av[m];
                 if constexpr (sizeof(R) == 4) {
                     const float r2 = fmadd(o.re, o.re, o.im * o.im);
                     const float sc = a * __builtin_amdgcn_rsqf(r2);
