@@ -317,7 +317,15 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
     tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
         const ProbDev& P = probs[find_problem<PASS_COL>(probs, nprob, tile)];
-        const int kx = (tile - P.base_col) * CB + g;  // this thread's column
+        int lt = tile - P.base_col;
+        if (RS && !(P.tiles_col & 1)) {
+            // mirror-paired order: local tiles 2i and 2i + 1 take column
+            // blocks i and Q - 1 - i, which read the same kernel-spectrum
+            // columns min(kx, N - kx) (all but one), and run together on one
+            // XCD, so the pair's second read comes from that XCD's L2
+            lt = (lt & 1) ? P.tiles_col - 1 - (lt >> 1) : (lt >> 1);
+        }
+        const int kx = lt * CB + g;                 // this thread's column
         const int q = kx / B, bq = kx % B;          // its block and place in it
         const int ny = P.ny, oy = P.oy;
         const cpx<R>* tw = static_cast<const cpx<R>*>(P.pcol.tw);
