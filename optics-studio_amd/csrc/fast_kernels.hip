@@ -306,16 +306,13 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
     // hoist) within 128 VGPRs; that took the pass from 0.87 to 0.76 ms on
     // MI355X.  The half-window convolution form (RS) has them in V1-V3.
     // Elsewhere at 4096 the kept factors pushed the kernel past 128 VGPRs.
-    // The half-window convolution form at 4096 (V1-V3) keeps the table but
-    // spends the hoist's registers on its kernel-spectrum loads instead
-    // (RS_PRE): issued before the forward transform, their latency is covered.
-    constexpr bool RS_PRE = RS && HW && sizeof(R) == 4 && N == 4096 && V != 0;
-    constexpr bool KEEP = HW && sizeof(R) == 4 && N == 4096 && !RS && EK == ENG_ASM && (V == 2 || V == 3);
+    constexpr bool KEEP = HW && sizeof(R) == 4 && N == 4096 &&
+                          ((!RS && EK == ENG_ASM && (V == 2 || V == 3)) || (RS && V != 0));
     // complex64 at 1024 and 2048 (two radix-16 stages: the table only)
     // keeps it in every form; the table replaces the factor products, so
     // these kernels use 2-10 fewer VGPRs with it (the GS projection pass at
     // 1024: 6% fewer VALU instructions per tile).
-    constexpr bool KEEP_TAB = KEEP || RS_PRE || (sizeof(R) == 4 && (N == 1024 || N == 2048));
+    constexpr bool KEEP_TAB = KEEP || (sizeof(R) == 4 && (N == 1024 || N == 2048));
     TwKeep<R, N, KEEP_TAB, KEEP> tk;
     tk.template init<CB * T>(lds + CB * Pow2Plan<N>::SLOTS, static_cast<const cpx<R>*>(probs[0].pcol.tw), t);
     for (int tile = xcd_block(); tile < total; tile += gridDim.x) {
@@ -342,26 +339,18 @@ __global__ __launch_bounds__((FastLayout<R, N, V>::CB * (N / 16))) void f_col(co
             v[m] = {R(0), R(0)};
             if (HW ? (m >= 4 && m < 12) : (y >= oy && y < oy + ny)) v[m] = at32(work, b0 + (unsigned)y * B);
         }
-        // the kernel's 2-D spectrum from f_rs_kcol: W(kx, ky) is entry
-        // (min(kx, N - kx), min(ky, N - ky)) of the stored quarter
-        constexpr int KR = N / 2 + 1;
-        const cpx<R>* wb = RS ? static_cast<const cpx<R>*>(P.kbuf) + (size_t)N * KR : nullptr;
-        const int kh = min(kx, N - kx);
-        const unsigned w0 = (unsigned)(kh / B) * KR * B + kh % B;
-        cpx<R> wv[16];
-        if constexpr (RS_PRE) {
-#pragma unroll
-            for (int m = 0; m < 16; ++m) wv[m] = at32(wb, w0 + (unsigned)min(t + m * T, N - (t + m * T)) * B);
-        }
         fft_regs<R, N, CB, -1>(v, lds, g, t, tw, tk);
         if constexpr (RS) {
+            // the kernel's 2-D spectrum from f_rs_kcol: W(kx, ky) is entry
+            // (min(kx, N - kx), min(ky, N - ky)) of the stored quarter
+            constexpr int KR = N / 2 + 1;
+            const cpx<R>* wb = static_cast<const cpx<R>*>(P.kbuf) + (size_t)N * KR;
+            const int kh = min(kx, N - kx);
+            const unsigned w0 = (unsigned)(kh / B) * KR * B + kh % B;
 #pragma unroll
             for (int m = 0; m < 16; ++m) {
                 const int y = t + m * T;
-                cpx<R> w;
-                if constexpr (RS_PRE) w = wv[m];
-                else w = at32(wb, w0 + (unsigned)min(y, N - y) * B);
-                v[m] = cmul(v[m], w);
+                v[m] = cmul(v[m], at32(wb, w0 + (unsigned)min(y, N - y) * B));
             }
         } else {
             const int ek = EK >= 0 ? EK : P.eng.kind;
