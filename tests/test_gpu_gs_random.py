@@ -122,3 +122,22 @@ def test_batch_random_starts_equal_single_calls(G):
         assert r["iterations"] == one["iterations"] and r["best_iteration"] == one["best_iteration"]
         assert r["history"] == one["history"]
         assert np.array_equal(r["phase"], one["phase"])
+
+
+@pytest.mark.parametrize("lo,hi", [(-0.5, 2.0), (0.0, 10.0)], ids=["span2.5", "span-clamped"])
+def test_random_start_bounds_match_oracle(G, orc, lo, hi):
+    """Slot bounds other than [-pi, pi]: the span is (hi - lo) clamped to
+    [0, 2 pi] (gs.rs:451) and the start is encoded into the bounds."""
+    n, d, z = 40, 8e-6, 4e-3
+    u, target, _ = tophat(n, d, 150e-6, 100e-6)
+    kw = dict(max_iterations=6, patience=10, merit_tolerance=1e-4, lo=lo, hi=hi, init_seed=21)
+    say("gs random start with bounds", lo, hi)
+    st, ref = orc.gs_run(u, d, d, LAM, z, target, target ** 2, ASM_PLAIN, **kw)
+    assert st == 0
+    st, got = G.gs_run(u, d, d, LAM, z, target, target ** 2, ASM_PLAIN, **kw)
+    assert st == 0, T.STATUS[st]
+    assert got["iterations"] == ref["iterations"] and got["best_iteration"] == ref["best_iteration"]
+    for a, b in zip(got["history"], ref["history"]):
+        for x, y in zip(a[1:], b[1:]):
+            assert abs(x - y) <= 1e-9 * max(abs(y), 1e-30), (a, b)
+    assert np.all(got["phase"] >= lo) and np.all(got["phase"] <= hi)
